@@ -1,0 +1,254 @@
+"""Pins the CPU oracle: upstream golden vectors, upstream known answers, invariants, brute force.
+
+The oracle is the checker for the HIP path, so these tests are what make its verdicts mean
+anything.  Each test names the upstream function it pins.
+"""
+import numpy as np
+import pytest
+
+from dac_vtsa_amd.config import load_config, with_overrides
+from tests._helpers import CFG1, MID, SMALL, VARIANTS as VARIANTS_OVER, golden, oracle_model, rel, state_from
+
+VARIANTS = list(golden("rhs_small.npz")["variants"])
+
+
+# ------------------------------------------------------------------ closures (known answers + golden)
+def test_isotherm_known_answers():
+    """physics/isotherms.py values at 288.15 K, 55 % RH, 400 ppm (upstream tests :53-65)."""
+    from oracle.model import Equilibrium, arden_buck
+    cfg = load_config()
+    eq = Equilibrium(cfg)
+    T = 288.15
+    ps = float(arden_buck(T))
+    pw = 0.55 * ps
+    pc = 4e-4 * (101325 - pw)
+    qw = float(eq.q_h2o(pw, T, ps))
+    assert abs(qw - 4.091) < 5e-3
+    assert abs(float(eq.q_co2_dry(pc, T)) - 1.0226) < 2e-3
+    assert abs(float(eq.q_co2(pc, T, qw)) - 1.511) < 3e-3
+    mech = Equilibrium(with_overrides(cfg, {"isotherm.coadsorption.model": "mechanistic"}))
+    assert abs(float(mech.q_co2(pc, T, qw)) - 1.462) < 3e-3
+
+
+def test_isotherm_negative_iterates_and_psat():
+    from oracle.model import Equilibrium, arden_buck
+    eq = Equilibrium(load_config())
+    q = eq.q_co2(np.array([-1.0, 0.0, 1e-9]), 300.0, np.array([0.0, 1.0, 2.0]))
+    assert np.all(np.isfinite(q)) and q[0] == 0.0 and q[1] == 0.0
+    assert abs(float(arden_buck(373.15)) - 101325) / 101325 < 3e-3
+
+
+def test_closures_match_golden():
+    from oracle.model import Equilibrium, Gas, arden_buck
+    z = golden("closures.npz")
+    cfg = load_config()
+    assert np.array_equal(arden_buck(z["T"]), z["psat"])
+    TT, PP, QW = z["grid_T"], z["grid_p"], z["grid_qw"]
+    for model in ("none", "wadst", "mechanistic", "stampi_bombelli"):
+        eq = Equilibrium(with_overrides(cfg, {"isotherm.coadsorption.model": model}))
+        np.testing.assert_allclose(eq.q_co2(PP, TT, QW), z[f"qco2_{model}"], rtol=1e-14, atol=0)
+    eq = Equilibrium(cfg)
+    np.testing.assert_allclose(eq.q_h2o(PP * 1e-2, TT, arden_buck(TT)), z["qh2o"], rtol=1e-14)
+    gas = Gas(["CO2", "H2O", "N2"], 330.0)
+    mu, k = gas.mu_k_mix(TT, z["y"])
+    np.testing.assert_allclose(mu, z["mu"], rtol=1e-14)
+    np.testing.assert_allclose(k, z["k"], rtol=1e-14)
+    np.testing.assert_allclose(gas.D_co2_mix(TT, PP, z["y"]), z["D"], rtol=1e-14)
+    assert np.array_equal(gas.cp, z["cp"])
+
+
+def test_ergun_inversion_and_softplus_split():
+    """physics/pressure.py:39-47 round trip; boundaries/conditions.py:135 p_in - p_out = dp."""
+    from oracle.model import ergun_dp_dn, ergun_velocity, softplus
+    rng = np.random.default_rng(0)
+    g = rng.standard_normal(1000) * 10.0 ** rng.uniform(-3, 8, 1000)
+    a, b = 10.0 ** rng.uniform(-2, 3, 1000), 10.0 ** rng.uniform(-1, 4, 1000)
+    np.testing.assert_allclose(ergun_dp_dn(ergun_velocity(g, a, b), a, b), g, rtol=1e-9, atol=1e-12)
+    dp = rng.standard_normal(1000) * 10.0 ** rng.uniform(-3, 4, 1000)
+    np.testing.assert_allclose(softplus(dp, 0.2) - softplus(-dp, 0.2), dp, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------ rhs vs upstream golden vectors
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_rhs_matches_golden_small(variant):
+    """TVSAModel.rhs (core/residual.py:211-417) on 3x4x6, every step type, option variants."""
+    z = golden("rhs_small.npz")
+    st = state_from(z, f"{variant}/")
+    for si in range(4):
+        cfg, m = oracle_model({**SMALL, **VARIANTS_OVER[variant]}, si, st, T_jacket_prev=300.0)
+        p = f"{variant}/s{si}/"
+        y = m.pack(st)
+        np.testing.assert_allclose(y, z[p + "y"], rtol=1e-15, atol=0)
+        f, acc = m.rhs(12.5, z[p + "y"])
+        assert rel(f, z[p + "f"]) < 1e-13, (variant, si, rel(f, z[p + "f"]))
+        np.testing.assert_allclose(acc, z[p + "acc"], rtol=1e-12, atol=1e-13 * np.abs(z[p + "acc"]).max())
+
+
+def test_rhs_diag_matches_golden():
+    z = golden("rhs_small.npz")
+    st = state_from(z, "base/")
+    for si in range(4):
+        cfg, m = oracle_model(SMALL, si, st, T_jacket_prev=300.0)
+        d = m.rhs(12.5, z[f"base/s{si}/y"], diag=True)
+        p = f"base/s{si}/d_"
+        for k in ("T", "P", "y", "qstar", "uz", "ur", "uth", "Q_jac", "Q_wb", "Q_amb", "R_bed"):
+            np.testing.assert_allclose(d[k], z[p + k], rtol=1e-12, atol=1e-300)
+        for end in ("top", "bottom"):
+            for k, v in d["bnd"][end].items():
+                np.testing.assert_allclose(v, z[p + f"bnd_{end}_{k}"], rtol=1e-12, atol=1e-300)
+
+
+def test_rhs_and_stage_residual_cfg1():
+    z = golden("rhs_cfg1.npz")
+    st = state_from(z)
+    cfg, m = oracle_model(CFG1, 0, st)
+    y = m.pack(st)
+    np.testing.assert_allclose(y, z["y"], rtol=1e-15)
+    np.testing.assert_allclose(m.atol, z["atol"], rtol=0)
+    f, acc = m.rhs(5.0, z["y"])
+    assert rel(f, z["f"]) < 1e-13
+    np.testing.assert_allclose(acc, z["acc"], rtol=1e-12, atol=1e-13 * np.abs(z["acc"]).max())
+    R = m.stage_residual(5.0, z["y"], 0.37, z["rhs_known"])
+    assert rel(R, z["R"]) < 1e-13
+
+
+def test_rhs_midcycle_state_all_steps():
+    z = golden("rhs_mid.npz")
+    st = state_from(z)
+    for si in range(4):
+        cfg, m = oracle_model(MID, si, st)
+        f, acc = m.rhs(3.0, z[f"s{si}/y"])
+        assert rel(f, z[f"s{si}/f"]) < 1e-13, si
+        np.testing.assert_allclose(acc, z[f"s{si}/acc"], rtol=1e-11, atol=1e-12 * np.abs(z[f"s{si}/acc"]).max())
+
+
+# ------------------------------------------------------------------ invariants
+@pytest.mark.parametrize("step_index", [0, 1, 2, 3])
+def test_rhs_conserves_species_and_energy(step_index):
+    """Sum V (eps_t dc + rho_b dq) = net boundary flow; energy analogue (upstream tests :79-104)."""
+    z = golden("rhs_small.npz")
+    st = state_from(z, "base/")
+    cfg, m = oracle_model(SMALL, step_index, st)
+    g = m.grid
+    f, acc = m.rhs(1.0, m.pack(st))
+    ai = m.acc_index
+    dc = f[:m.n_c].reshape((m.ng,) + g.shape)
+    dq = f[m.off_q:m.off_e].reshape((m.na,) + g.shape)
+    for k, s in enumerate(m.species):
+        dinv = np.sum(m.eps_t * dc[k] * g.V)
+        if s in m.adsorbed:
+            dinv += np.sum(m.rho_b * dq[m.adsorbed.index(s)] * g.V)
+        net = sum(acc[ai[f"n_in_{v}_{s}"]] - acc[ai[f"n_out_{v}_{s}"]] for v in ("V1", "V2", "V3"))
+        assert abs(dinv - net) <= 1e-12 * max(1.0, abs(net)) + 1e-18
+    de = f[m.off_e:m.off_w].reshape(g.shape)
+    dTw = f[m.off_w:m.n_core].reshape(g.wall_shape)
+    dE = np.sum(de * g.V) + np.sum(m.wall_C * dTw)
+    netH = (sum(acc[ai[f"H_in_{v}"]] - acc[ai[f"H_out_{v}"]] for v in ("V1", "V2", "V3"))
+            + acc[ai["Q_jac_pos"]] + acc[ai["Q_jac_neg"]] + acc[ai["Q_amb"]] + acc[ai["Q_hx_pos"]]
+            + acc[ai["Q_hx_neg"]])
+    assert abs(dE - netH) <= 1e-9 * max(1.0, abs(netH))
+
+
+def test_temperature_energy_roundtrip_and_axisymmetry():
+    from oracle.runner import initial_state, make
+    cfg = load_config(override_dict={"grid.Nr": 3, "grid.Ntheta": 4, "grid.Nz": 6})
+    g, gas = make(cfg)
+    st = initial_state(cfg, g, gas)
+    rng = np.random.default_rng(5)
+    st.T = st.T + 5.0 * rng.random((g.Nr, 1, g.Nz))
+    from oracle.model import OracleModel
+    m = OracleModel(cfg, g, gas, cfg.cycle[0], st)
+    c, q, e, _ = m.unpack(m.pack(st))
+    assert np.max(np.abs(m.temperature(e, c, q) - st.T)) < 1e-9
+    f, _ = m.rhs(10.0, m.pack(st), want_acc=False)
+    c, q, e, Tw = m.unpack(f)
+    for arr in (c, q, e[None]):
+        spread = np.abs(arr - arr.mean(axis=-2, keepdims=True)).max()
+        assert spread <= 1e-9 * max(np.abs(arr).max(), 1e-30)
+
+
+# ------------------------------------------------------------------ Jacobian
+@pytest.mark.parametrize("name,over", [("jac_small.npz", SMALL), ("jac_cfg1.npz", CFG1)])
+def test_fd_jacobian_matches_golden(name, over):
+    """FDJacobian.__call__ and theta_block (core/jacobian.py:71-91)."""
+    from oracle.jacobian import OracleFDJacobian
+    z = golden(name)
+    cfg, m = oracle_model(over, 0, state_from(z))
+    jac = OracleFDJacobian(m)
+    assert np.array_equal(jac.indptr, z["indptr"]) and np.array_equal(jac.indices, z["indices"])
+    assert np.array_equal(jac.typ, z["typ"])
+    J = jac(5.0, z["y"])
+    # the greedy colouring reproduces the upstream one, so the values are bit-identical
+    assert np.array_equal(jac.color, z["color"])
+    np.testing.assert_allclose(J.data, z["data"], rtol=1e-12, atol=1e-12 * np.abs(z["data"]).max())
+    Jt = jac.theta_block(J)
+    assert np.array_equal(Jt.indices, z["theta_indices"]) and np.array_equal(Jt.indptr, z["theta_indptr"])
+
+
+def test_coloring_valid_and_fd_matches_one_column_fd():
+    """Colouring validity + 12 columns against one-at-a-time FD (upstream tests :118-138)."""
+    from oracle.jacobian import OracleFDJacobian, verify_coloring
+    z = golden("jac_small.npz")
+    cfg, m = oracle_model(SMALL, 0, state_from(z))
+    jac = OracleFDJacobian(m)
+    assert verify_coloring(jac.P, jac.color)
+    y = z["y"]
+    J = jac(5.0, y).toarray()
+    f0, _ = m.rhs(5.0, y, want_acc=False)
+    h = jac.steps(y)
+    for j in np.random.default_rng(2).choice(len(y), 12, replace=False):
+        yp = y.copy()
+        yp[j] += h[j]
+        col = (m.rhs(5.0, yp, want_acc=False)[0] - f0) / h[j]
+        # the coloured FD entry equals the single-column FD entry bit for bit
+        rows = np.nonzero(jac.P[:, j].toarray().ravel())[0]
+        assert np.array_equal(col[rows], J[rows, j])
+        off = np.setdiff1d(np.arange(len(y)), rows)
+        assert np.all(col[off] == 0.0)
+
+
+# ------------------------------------------------------------------ integrator
+@pytest.mark.slow
+@pytest.mark.parametrize("scheme", ["sdirk2", "trbdf2"])
+def test_integrator_matches_golden(scheme):
+    """TRBDF2.integrate (core/solver.py:370-511): 20 s of adsorption from the seeded 4x4x20 state."""
+    from oracle.solver import OracleTRBDF2
+    z = golden("integ_cfg1.npz")
+    cfg, m = oracle_model({**CFG1, "solver.scheme": scheme}, 0, state_from(z))
+    y0 = m.pack(state_from(z))
+    np.testing.assert_allclose(y0, z[f"{scheme}/y0"], rtol=1e-15)
+    times = []
+    res = OracleTRBDF2(m, cfg, h_max=4.0).integrate(y0, 20.0, np.linspace(0, 20.0, 5),
+                                                   on_step=lambda t, y, a: times.append(t))
+    cnt = np.array([res.n_steps, res.n_reject, res.n_newton_fail, res.n_krylov, res.n_jac, res.n_lu, res.nfev])
+    assert np.array_equal(cnt, z[f"{scheme}/counters"]), (cnt, z[f"{scheme}/counters"])
+    np.testing.assert_allclose(times, z[f"{scheme}/t_acc"], rtol=1e-12)
+    sc = m.atol + 1e-4 * np.abs(z[f"{scheme}/y"])
+    assert np.max(np.abs(res.y - z[f"{scheme}/y"]) / sc) < 1e-6
+    np.testing.assert_allclose(res.acc, z[f"{scheme}/acc"], rtol=1e-9, atol=1e-12 * np.abs(z[f"{scheme}/acc"]).max())
+
+
+def test_integrator_matches_rk4_short_transient():
+    """SDIRK2 Newton-Krylov vs classical RK4 on 2 ms of blowdown, 2x1x4 (upstream tests :182-200)."""
+    from oracle.runner import initial_state, make
+    from oracle.model import OracleModel
+    from oracle.solver import OracleTRBDF2
+    cfg = load_config(override_dict={"grid.Nr": 2, "grid.Ntheta": 1, "grid.Nz": 4,
+                                     "solver.rtol": 1e-6, "solver.newton_kappa": 1e-3})
+    g, gas = make(cfg)
+    st = initial_state(cfg, g, gas)
+    m = OracleModel(cfg, g, gas, cfg.cycle[1], st)
+    y0 = m.pack(st)
+    t_end, n = 2e-3, 6000
+    dt = t_end / n
+    y = y0.copy()
+    for i in range(n):
+        t = i * dt
+        k1 = m.rhs(t, y, want_acc=False)[0]
+        k2 = m.rhs(t + 0.5 * dt, y + 0.5 * dt * k1, want_acc=False)[0]
+        k3 = m.rhs(t + 0.5 * dt, y + 0.5 * dt * k2, want_acc=False)[0]
+        k4 = m.rhs(t + dt, y + dt * k3, want_acc=False)[0]
+        y = y + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    res = OracleTRBDF2(m, cfg, h_max=t_end, h_init=1e-6).integrate(y0, t_end, np.array([t_end]))
+    sc = m.atol + 1e-4 * np.abs(y)
+    assert np.sqrt(np.mean(((res.y - y) / sc) ** 2)) < 1.0
