@@ -1,0 +1,301 @@
+// C ABI of libvtsa: handle lifetime, residual / Jacobian / preconditioner / GMRES / Newton entry
+// points.  Host code here only sizes buffers, launches kernels and moves scalars.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "vtsa_internal.h"
+
+namespace vtsa {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return -1;
+}
+
+template <typename T>
+static int dalloc(T** p, size_t count) {
+  if (count == 0) count = 1;
+  return check(hipMalloc((void**)p, sizeof(T) * count), "hipMalloc");
+}
+
+static int upload(double** dst, const double* src, size_t count) {
+  if (dalloc(dst, count)) return -1;
+  if (src && count) return check(hipMemcpy(*dst, src, sizeof(double) * count, hipMemcpyHostToDevice), "hipMemcpy");
+  return check(hipMemset(*dst, 0, sizeof(double) * (count ? count : 1)), "hipMemset");
+}
+
+static void free_handle(Handle* h) {
+  if (!h) return;
+  double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2,
+                    h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jcw, h->Jw, h->Jwc, h->Fw, h->s0,
+                    h->Vb, h->red.partial, h->red.result};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  if (h->F) (void)hipFree(h->F);
+  if (h->gs) (void)hipFree(h->gs);
+  if (h->red.counter) (void)hipFree(h->red.counter);
+  if (h->gs_host) (void)hipHostFree(h->gs_host);
+  if (h->host_scalars) (void)hipHostFree(h->host_scalars);
+  delete h;
+}
+
+static int grow_basis(Handle& h, int restart) {
+  if (restart + 1 <= h.vb_cols) return 0;
+  if (h.Vb) (void)hipFree(h.Vb);
+  h.Vb = nullptr;
+  h.vb_cols = 0;
+  if (dalloc(&h.Vb, (size_t)(restart + 1) * h.N)) return -1;
+  h.vb_cols = restart + 1;
+  return 0;
+}
+
+}  // namespace vtsa
+
+using namespace vtsa;
+
+struct vtsa_handle {
+  Handle h;
+};
+
+static inline hipStream_t S(vtsa_stream s) { return (hipStream_t)s; }
+
+static int finish(hipStream_t s) {
+  (void)s;
+  hipError_t e = hipGetLastError();
+  return check(e, "kernel launch");
+}
+
+extern "C" {
+
+const char* vtsa_version(void) { return "vtsa 0.1.0 (gfx950, FP64 residual)"; }
+const char* vtsa_last_error(void) { return g_err.c_str(); }
+
+int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, const double* Ath, const double* arc,
+                const double* Ar, const double* drc, const double* feed_flux, const double* N_nom, const double* cov,
+                vtsa_handle** out) {
+  if (!desc || !out) { set_error("vtsa_create: null argument"); return -1; }
+  const int Nr = desc->Nr, Nth = desc->Nth, Nz = desc->Nz;
+  if (Nr < 1 || Nth < 1 || Nz < 2 || Nth == 2) { set_error("vtsa_create: invalid grid"); return -2; }
+  vtsa_handle* H = new vtsa_handle();
+  Handle& h = H->h;
+  h.desc = *desc;
+  h.n = Nr * Nth * Nz;
+  h.nw = desc->wall_on ? Nth * Nz : 0;
+  h.N = 6 * h.n + h.nw;
+  h.ncol = Nr * Nth;
+  const size_t nwall = (size_t)Nth * Nz;
+  int rc = 0;
+  rc |= upload(&h.V, V, Nr);
+  rc |= upload(&h.Az, Az, Nr);
+  rc |= upload(&h.Ath, Ath, Nr);
+  rc |= upload(&h.arc, arc, Nr);
+  rc |= upload(&h.Ar, Nr > 1 ? Ar : nullptr, Nr > 1 ? Nr - 1 : 1);
+  rc |= upload(&h.drc, Nr > 1 ? drc : nullptr, Nr > 1 ? Nr - 1 : 1);
+  rc |= upload(&h.feed_flux, feed_flux, (size_t)Nr * Nth);
+  rc |= upload(&h.N_nom, N_nom, (size_t)Nr * Nth);
+  rc |= upload(&h.cov, cov, nwall);
+  rc |= dalloc(&h.pb, (size_t)NPROP * h.n);
+  rc |= dalloc(&h.pb2, (size_t)NPROP * h.n);
+  rc |= dalloc(&h.ytmp, h.N);
+  rc |= dalloc(&h.ftmp, h.N);
+  rc |= dalloc(&h.vtmp, h.N);
+  rc |= dalloc(&h.vtmp2, h.N);
+  rc |= dalloc(&h.accbuf, NACC);
+  rc |= dalloc(&h.Jc, (size_t)h.n * NSLOT * 36);
+  rc |= dalloc(&h.Jcw, nwall * 4);
+  rc |= dalloc(&h.Jw, nwall * 5);
+  rc |= dalloc(&h.Jwc, nwall * 6);
+  rc |= check(hipMalloc(&h.F, sizeof(double) * (size_t)h.n * 180), "hipMalloc");
+  rc |= dalloc(&h.Fw, nwall);
+  rc |= dalloc(&h.s0, h.N);
+  rc |= dalloc(&h.gs, 1);
+  rc |= check(hipHostMalloc((void**)&h.gs_host, sizeof(GmresScalars)), "hipHostMalloc");
+  rc |= check(hipHostMalloc((void**)&h.host_scalars, sizeof(double) * 64), "hipHostMalloc");
+  h.red.max_blocks = 512;
+  rc |= dalloc(&h.red.partial, partial_size(h.ncol, h.nw));
+  rc |= dalloc(&h.red.counter, 16);
+  rc |= dalloc(&h.red.result, 64);
+  if (rc == 0) {
+    rc |= check(hipMemset(h.red.counter, 0, sizeof(unsigned) * 16), "hipMemset");
+    rc |= check(hipMemset(h.gs, 0, sizeof(GmresScalars)), "hipMemset");
+    rc |= check(hipMemset(h.Jc, 0, sizeof(double) * (size_t)h.n * NSLOT * 36), "hipMemset");
+  }
+  if (rc == 0) rc = grow_basis(h, 30);
+  if (rc != 0) {
+    free_handle(new Handle(h));
+    delete H;
+    return -3;
+  }
+  h.m.d = h.desc;
+  h.m.n = h.n;
+  h.m.nw = h.nw;
+  h.m.N = h.N;
+  h.m.V = h.V; h.m.Az = h.Az; h.m.Ath = h.Ath; h.m.arc = h.arc; h.m.Ar = h.Ar; h.m.drc = h.drc;
+  h.m.feed_flux = h.feed_flux; h.m.N_nom = h.N_nom; h.m.cov = h.cov;
+  *out = H;
+  return 0;
+}
+
+int vtsa_destroy(vtsa_handle* H) {
+  if (!H) return 0;
+  (void)hipDeviceSynchronize();
+  Handle* copy = new Handle(H->h);
+  free_handle(copy);
+  delete H;
+  return 0;
+}
+
+int vtsa_sizes(const vtsa_handle* H, int32_t* n_cells, int32_t* n_wall, int32_t* n_unknowns) {
+  if (!H) return -1;
+  if (n_cells) *n_cells = H->h.n;
+  if (n_wall) *n_wall = H->h.nw;
+  if (n_unknowns) *n_unknowns = H->h.N;
+  return 0;
+}
+
+int vtsa_rhs(vtsa_handle* H, double t, const double* y, double* f, double* acc, vtsa_stream s) {
+  if (!H || !y || !f) { set_error("vtsa_rhs: null argument"); return -1; }
+  Handle& h = H->h;
+  launch_props(h, y, h.pb, f, nullptr, S(s));
+  launch_flux(h, t, y, h.pb, f, S(s));
+  if (acc) launch_acc(h, t, y, h.pb, acc, S(s));
+  return finish(S(s));
+}
+
+int vtsa_stage_residual(vtsa_handle* H, double t, const double* y, double gamma_h, const double* rhs_known, double* R,
+                        vtsa_stream s) {
+  if (!H || !y || !rhs_known || !R) { set_error("vtsa_stage_residual: null argument"); return -1; }
+  Handle& h = H->h;
+  launch_props(h, y, h.pb, h.ftmp, nullptr, S(s));
+  launch_flux(h, t, y, h.pb, h.ftmp, S(s));
+  launch_residual_combine(h, y, h.ftmp, gamma_h, rhs_known, R, S(s));
+  return finish(S(s));
+}
+
+int vtsa_diag(vtsa_handle* H, double t, const double* y, double* cell, double* uz, double* ur, double* uth,
+              double* bnd, double* wall, vtsa_stream s) {
+  if (!H || !y || !cell || !uz || !ur || !uth || !bnd || !wall) { set_error("vtsa_diag: null argument"); return -1; }
+  Handle& h = H->h;
+  launch_diag(h, t, y, h.pb, cell, uz, ur, uth, bnd, wall, S(s));
+  return finish(S(s));
+}
+
+int vtsa_equilibrium(vtsa_handle* H, int32_t count, const double* pc, const double* pw, const double* T,
+                     const double* qw, double* qc, double* qwe, vtsa_stream s) {
+  if (!H || count < 0) { set_error("vtsa_equilibrium: bad argument"); return -1; }
+  if (count == 0) return 0;
+  launch_equilibrium(H->h, count, pc, pw, T, qw, qc, qwe, S(s));
+  return finish(S(s));
+}
+
+int vtsa_fd_jacobian(vtsa_handle* H, double t, const double* y, const double* f0, vtsa_stream s) {
+  if (!H || !y || !f0) { set_error("vtsa_fd_jacobian: null argument"); return -1; }
+  launch_fdjac(H->h, t, y, f0, S(s));
+  return finish(S(s));
+}
+
+int vtsa_jacobian_get(vtsa_handle* H, double* Jc, double* Jcw, double* Jw, double* Jwc) {
+  if (!H) return -1;
+  Handle& h = H->h;
+  const size_t nwall = (size_t)h.desc.Nth * h.desc.Nz;
+  if (check(hipDeviceSynchronize(), "sync")) return -1;
+  if (Jc && check(hipMemcpy(Jc, h.Jc, sizeof(double) * (size_t)h.n * NSLOT * 36, hipMemcpyDeviceToHost), "memcpy"))
+    return -1;
+  if (h.nw > 0) {
+    if (Jcw && check(hipMemcpy(Jcw, h.Jcw, sizeof(double) * nwall * 4, hipMemcpyDeviceToHost), "memcpy")) return -1;
+    if (Jw && check(hipMemcpy(Jw, h.Jw, sizeof(double) * nwall * 5, hipMemcpyDeviceToHost), "memcpy")) return -1;
+    if (Jwc && check(hipMemcpy(Jwc, h.Jwc, sizeof(double) * nwall * 6, hipMemcpyDeviceToHost), "memcpy")) return -1;
+  }
+  return 0;
+}
+
+int vtsa_precond_setup(vtsa_handle* H, double dh, const double* scale, int32_t kind, int32_t precision,
+                       vtsa_stream s) {
+  if (!H || !scale) { set_error("vtsa_precond_setup: null argument"); return -1; }
+  if (kind != VTSA_PREC_BLOCK_JACOBI && kind != VTSA_PREC_ZLINE) { set_error("unknown preconditioner"); return -2; }
+  if (precision != VTSA_FP64 && precision != VTSA_FP32) { set_error("unknown precision"); return -2; }
+  launch_prec_setup(H->h, dh, scale, kind, precision, S(s));
+  return finish(S(s));
+}
+
+int vtsa_precond_apply(vtsa_handle* H, const double* r, double* out, vtsa_stream s) {
+  if (!H || !r || !out) { set_error("vtsa_precond_apply: null argument"); return -1; }
+  if (!H->h.prec_ready) { set_error("vtsa_precond_apply: no factorisation"); return -2; }
+  launch_prec_apply(H->h, r, nullptr, nullptr, nullptr, nullptr, out, false, 0.0, S(s));
+  return finish(S(s));
+}
+
+int vtsa_gmres(vtsa_handle* H, double t, const double* z, const double* f, const double* sc, double dh,
+               double zs_norm, const double* b, double* w, double rtol, double atol, int32_t restart,
+               vtsa_gmres_info* info, vtsa_stream s) {
+  if (!H || !z || !f || !sc || !b || !w || !info) { set_error("vtsa_gmres: null argument"); return -1; }
+  Handle& h = H->h;
+  if (!h.prec_ready) { set_error("vtsa_gmres: no factorisation"); return -2; }
+  if (restart < 1 || restart > MAX_RESTART) { set_error("vtsa_gmres: restart out of range"); return -2; }
+  if (grow_basis(h, restart)) return -3;
+  int rc = gmres_cycle(h, t, z, f, sc, dh, zs_norm, b, w, rtol, atol, restart, info, S(s));
+  if (rc) return rc;
+  return finish(S(s));
+}
+
+int vtsa_scale(vtsa_handle* H, const double* z, double rtol, double* s, vtsa_stream st) {
+  if (!H || !z || !s) return -1;
+  scale_impl(H->h, z, rtol, s, S(st));
+  return finish(S(st));
+}
+
+int vtsa_rms(vtsa_handle* H, const double* a, const double* s, int32_t len, double* out, vtsa_stream st) {
+  if (!H || !a || !out) return -1;
+  return reduce_rms(H->h, a, s, len, out, S(st));
+}
+
+int vtsa_newton_eval(vtsa_handle* H, double t, const double* z, double dh, const double* rk, const double* s,
+                     double* f, double* R, double* rn, double* pn, vtsa_stream st) {
+  if (!H || !z || !rk || !s || !f || !R || !rn || !pn) return -1;
+  if (!H->h.prec_ready) { set_error("vtsa_newton_eval: no factorisation"); return -2; }
+  return newton_eval_impl(H->h, t, z, dh, rk, s, f, R, rn, pn, S(st));
+}
+
+int vtsa_newton_try(vtsa_handle* H, double t, const double* z, const double* dz, double lam, double dh,
+                    const double* rk, const double* s, double* zt, double* ft, double* Rt, double* rn, double* pn,
+                    double* step, vtsa_stream st) {
+  if (!H || !z || !dz || !rk || !s || !zt || !ft || !Rt) return -1;
+  if (!H->h.prec_ready) { set_error("vtsa_newton_try: no factorisation"); return -2; }
+  return newton_try_impl(H->h, t, z, dz, lam, dh, rk, s, zt, ft, Rt, rn, pn, step, S(st));
+}
+
+int vtsa_cons(vtsa_handle* H, const double* R, const double* s, double* wr, double* ws, vtsa_stream st) {
+  if (!H || !R || !s || !wr || !ws) return -1;
+  double out[8];
+  int rc = reduce_weighted(H->h, R, s, 0, 0, out, S(st));
+  for (int q = 0; q < 4; ++q) { wr[q] = out[q]; ws[q] = out[4 + q]; }
+  return rc;
+}
+
+int vtsa_inventories(vtsa_handle* H, const double* y, double* out, vtsa_stream st) {
+  if (!H || !y || !out) return -1;
+  return reduce_weighted(H->h, y, nullptr, 0, 1, out, S(st));
+}
+
+int vtsa_vec_op(vtsa_handle* H, int32_t op, double a, double b, double c, double e, const double* x, const double* y,
+                const double* z, const double* w, double* out, vtsa_stream st) {
+  if (!H || !out || op < 0 || op > 9) return -1;
+  vec_op_impl(H->h, op, 5 * H->h.n, a, b, c, e, x, y, z, w, out, S(st));
+  return finish(S(st));
+}
+
+int vtsa_error_norm(vtsa_handle* H, const double* y, const double* z2, const double* y3, double rtol, double* en,
+                    vtsa_stream st) {
+  if (!H || !y || !z2 || !y3 || !en) return -1;
+  if (!H->h.prec_ready) { set_error("vtsa_error_norm: no factorisation"); return -2; }
+  return error_norm_impl(H->h, y, z2, y3, rtol, en, S(st));
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// Internal declarations shared by the translation units of libvtsa (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "vtsa_model.h"
+
+namespace vtsa {
+
+constexpr int NACC = 36;
+constexpr int BLK = 256;          // default threads per block for cell-parallel kernels
+constexpr int MAX_RESTART = 64;
+
+// Device-resident scalars of one GMRES cycle and of the reductions around it.
+struct GmresScalars {
+  double H[MAX_RESTART + 1][MAX_RESTART];   // H[row][col] (Hessenberg after Givens)
+  double cs[MAX_RESTART], sn[MAX_RESTART], S[MAX_RESTART + 1], yv[MAX_RESTART];
+  double h_tmp[MAX_RESTART + 1];            // Gram-Schmidt coefficients of the current column
+  double h0sq;                              // |w|^2 before orthogonalisation
+  double sigma;                             // FD step of the current matvec
+  double unorm;                             // rms(u) of the current matvec
+  double inv_h1;                            // 1 / |w| after orthogonalisation
+  double presid, ptol, bnorm, atol_eff, rnorm;
+  int done, iters, breakdown, zero_u;
+};
+
+// Reduction scratch: per-block partial sums and a completion counter (last-block pattern).
+struct Reduce {
+  double* partial = nullptr;   // [max_blocks * max_k]
+  unsigned* counter = nullptr; // [8]
+  double* result = nullptr;    // [64] device results
+  int max_blocks = 0, max_k = 0;
+};
+
+struct Handle {
+  Model m{};
+  vtsa_model_desc desc{};
+  int n = 0, nw = 0, N = 0, ncol = 0;
+  // device model arrays
+  double *V = nullptr, *Az = nullptr, *Ath = nullptr, *arc = nullptr, *Ar = nullptr, *drc = nullptr;
+  double *feed_flux = nullptr, *N_nom = nullptr, *cov = nullptr;
+  // work buffers
+  double* pb = nullptr;      // properties at the last residual point [12n]
+  double* pb2 = nullptr;     // properties of a second point (matvec) [12n]
+  double* ytmp = nullptr;    // perturbed state [N]
+  double* ftmp = nullptr;    // residual at perturbed state [N]
+  double* vtmp = nullptr;    // scratch [N]
+  double* vtmp2 = nullptr;   // scratch [N]
+  double* accbuf = nullptr;  // [NACC]
+  // Jacobian blocks
+  double *Jc = nullptr, *Jcw = nullptr, *Jw = nullptr, *Jwc = nullptr;
+  // preconditioner factors (FP64 or FP32), scale at factorisation
+  void* F = nullptr;          // [n][5][36] blocks (L2, L1, Dinv, U1, U2) in the chosen precision
+  double* Fw = nullptr;       // [nw] wall diagonal of M
+  double* s0 = nullptr;       // [N] scale of the factorisation
+  int prec_kind = -1, prec_prec = 0, prec_ready = 0;
+  // GMRES
+  double* Vb = nullptr;       // basis [(MAX_RESTART+1) * N]
+  int vb_cols = 0;
+  GmresScalars* gs = nullptr; // device
+  GmresScalars* gs_host = nullptr;
+  Reduce red;
+  double* host_scalars = nullptr;   // pinned [64]
+};
+
+void set_error(const std::string& msg);
+int check(hipError_t e, const char* what);
+
+// residual kernels (vtsa_kernels.hip)
+void launch_props(Handle& h, const double* y, double* pb, double* f, double* qstar, hipStream_t s);
+void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s);
+void launch_acc(Handle& h, double t, const double* y, const double* pb, double* acc, hipStream_t s);
+void launch_diag(Handle& h, double t, const double* y, const double* pb, double* cell, double* uz, double* ur,
+                 double* uth, double* bnd, double* wall, hipStream_t s);
+void launch_fdjac(Handle& h, double t, const double* y, const double* f0, hipStream_t s);
+void launch_equilibrium(Handle& h, int count, const double* pc, const double* pw, const double* T,
+                        const double* qw, double* qc, double* qwe, hipStream_t s);
+// Perturbed matvec residual: props and fluxes at z + sigma*(u*s), epilogue
+//   w = ((u*s) - dh (fp - f)/sigma)/s   (sigma, zero flag read from device scalars)
+void launch_matvec_residual(Handle& h, double t, const double* z, const double* u, const double* s,
+                            const double* f, double dh, double* w, bool respect_done, hipStream_t st);
+
+// linear algebra (vtsa_linalg.hip)
+void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, hipStream_t s);
+// out = post^-1 * M^-1 (pre * in * (in_scale ? *in_scale : 1)); optionally stores the scaled
+// input to `store_in`; optionally accumulates sum(out^2) and sets sigma in gs (matvec use).
+void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,
+                       const double* post, double* out, bool sigma_for_matvec, double zs_norm, hipStream_t s);
+int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
+                const double* b, double* w, double rtol, double atol, int restart, vtsa_gmres_info* info,
+                hipStream_t st);
+// reductions returning host values (synchronising)
+int reduce_rms(Handle& h, const double* a, const double* s, int len, double* out, hipStream_t st);
+int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mode, double* out, hipStream_t st);
+void launch_residual_combine(Handle& h, const double* z, const double* f, double dh, const double* rhs_known,
+                             double* R, hipStream_t st);
+int newton_eval_impl(Handle& h, double t, const double* z, double dh, const double* rk, const double* s, double* f,
+                     double* R, double* rn, double* pn, hipStream_t st);
+int newton_try_impl(Handle& h, double t, const double* z, const double* dz, double lam, double dh, const double* rk,
+                    const double* s, double* zt, double* ft, double* Rt, double* rn, double* pn, double* step,
+                    hipStream_t st);
+void scale_impl(Handle& h, const double* z, double rtol, double* s, hipStream_t st);
+int error_norm_impl(Handle& h, const double* y, const double* z2, const double* y3, double rtol, double* en,
+                    hipStream_t st);
+void vec_op_impl(Handle& h, int op, int npos, double a, double b, double c, double e, const double* x,
+                 const double* y, const double* z, const double* w, double* out, hipStream_t st);
+size_t partial_size(int ncol, int nw);
+
+}  // namespace vtsa

@@ -1,0 +1,507 @@
+// Residual, diagnostics, accumulator and finite-difference Jacobian kernels.
+//
+// f(t, y) is evaluated in two cell-parallel passes (one thread per cell, z fastest so that
+// a wavefront reads 64 consecutive cells of a z column with unit stride):
+//   k_props : thermodynamic state, gas properties, equilibrium and LDF rates of each cell,
+//             cached as 12 SoA property arrays (96 B/cell) and the loading rows of f;
+//   k_flux  : the 6 face fluxes of each cell from its stencil's cached properties, the
+//             telescoping divergence, sinks and wall/exchanger sources (species + energy rows),
+//             plus the wall rows (threads n .. n+nw-1).
+// The finite-difference Jacobian perturbs one column per thread and recomputes only the
+// affected row cell (k_fd_cells / k_fd_wall): the values equal the upstream coloured forward
+// differences (core/jacobian.py:71-86) because a row reads only the columns of its stencil.
+#include "vtsa_internal.h"
+
+namespace vtsa {
+
+__device__ inline void cell_ijk(const vtsa_model_desc& d, int cell, int& i, int& j, int& k) {
+  k = cell % d.Nz;
+  int col = cell / d.Nz;
+  j = col % d.Nth;
+  i = col / d.Nth;
+}
+
+__device__ inline void load_state(const double* __restrict__ y, int n, int cell, double c[NG], double q[NA], double& e) {
+  c[0] = y[cell]; c[1] = y[n + cell]; c[2] = y[2 * n + cell];
+  q[0] = y[3 * n + cell]; q[1] = y[4 * n + cell];
+  e = y[5 * n + cell];
+}
+
+// ------------------------------------------------------------------ residual passes
+__global__ void __launch_bounds__(BLK) k_props(Model m, const double* __restrict__ y, double* __restrict__ pb,
+                                               double* __restrict__ f, double* __restrict__ qstar) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  if (cell >= n) return;
+  double c[NG], q[NA], e;
+  load_state(y, n, cell, c, q, e);
+  Props p;
+  double rate[NA], qs[NA];
+  cell_props(m.d, c, q, e, p, rate, qs);
+  store_props(pb, n, cell, p);
+  if (f) { f[3 * n + cell] = rate[0]; f[4 * n + cell] = rate[1]; }
+  if (qstar) { qstar[cell] = qs[0]; qstar[n + cell] = qs[1]; }
+}
+
+__device__ inline double wall_row(const Model& m, const double* __restrict__ Tw_all, const Props& o, int j, int k,
+                                  double t, int swap = -1, double swap_val = 0.0) {
+  const vtsa_model_desc& d = m.d;
+  const int w = j * d.Nz + k;
+  double Tw = (w == swap) ? swap_val : Tw_all[w];
+  WallFlows wf = wall_flows(m, o, Tw, m.cov[w], t);
+  double cond = wall_conduction(d, Tw_all, j, k, swap, swap_val);
+  return (((wf.Q_jac + wf.Q_amb) - wf.Q_wb) + cond) / d.C_wall;
+}
+
+__global__ void __launch_bounds__(BLK) k_flux(Model m, double t, const double* __restrict__ y,
+                                              const double* __restrict__ pb, double* __restrict__ f) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  const vtsa_model_desc& d = m.d;
+  if (tid < n) {
+    int i, j, k;
+    cell_ijk(d, tid, i, j, k);
+    BaseAccess acc{y, pb, n};
+    double rate[NA] = {f[3 * n + tid], f[4 * n + tid]};
+    double Tw = (d.wall_on && i == d.Nr - 1) ? y[6 * n + j * d.Nz + k] : 0.0;
+    double out[4];
+    cell_flux(m, acc, Tw, i, j, k, t, rate, out);
+    f[tid] = out[0]; f[n + tid] = out[1]; f[2 * n + tid] = out[2]; f[5 * n + tid] = out[3];
+  } else if (tid < n + m.nw) {
+    const int w = tid - n;
+    const int j = w / d.Nz, k = w % d.Nz;
+    Props o;
+    load_props(pb, n, ((d.Nr - 1) * d.Nth + j) * d.Nz + k, o);
+    f[6 * n + w] = wall_row(m, y + 6 * n, o, j, k, t);
+  }
+}
+
+// ------------------------------------------------------------------ matvec residual
+// Finite-difference Jacobian-vector product of the Newton operator (core/solver.py:196-203):
+//   v = u*s, fp = f(t, z + sigma v), w = (v - dh (fp - f)/sigma)/s.
+// Pass 1 forms the perturbed state, its properties and the loading rows of w.
+__global__ void __launch_bounds__(BLK) k_mv_props(Model m, const double* __restrict__ z, const double* __restrict__ u,
+                                                  const double* __restrict__ s, const double* __restrict__ f,
+                                                  const GmresScalars* __restrict__ gs, int respect_done, double dh,
+                                                  double* __restrict__ yp, double* __restrict__ pb,
+                                                  double* __restrict__ fp, double* __restrict__ w) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  if (respect_done && gs->done) return;
+  const double sig = gs->sigma;
+  const bool zero = gs->zero_u != 0;
+  if (tid < n) {
+    double yl[NV];
+    for (int v = 0; v < NV; ++v) {
+      int idx = v * n + tid;
+      yl[v] = z[idx] + sig * (u[idx] * s[idx]);
+      yp[idx] = yl[v];
+    }
+    Props p;
+    double rate[NA], qs[NA];
+    cell_props(m.d, yl, yl + 3, yl[5], p, rate, qs);
+    store_props(pb, n, tid, p);
+    for (int a = 0; a < NA; ++a) {
+      int idx = (3 + a) * n + tid;
+      fp[idx] = rate[a];
+      double vv = u[idx] * s[idx];
+      w[idx] = zero ? 0.0 : (vv - (dh * (rate[a] - f[idx])) / sig) / s[idx];
+    }
+  } else if (tid < n + m.nw) {
+    int idx = 6 * n + (tid - n);
+    yp[idx] = z[idx] + sig * (u[idx] * s[idx]);
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_mv_flux(Model m, double t, const double* __restrict__ yp,
+                                                 const double* __restrict__ pb, const double* __restrict__ fp,
+                                                 const double* __restrict__ u, const double* __restrict__ s,
+                                                 const double* __restrict__ f, const GmresScalars* __restrict__ gs,
+                                                 int respect_done, double dh, double* __restrict__ w) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  const vtsa_model_desc& d = m.d;
+  if (respect_done && gs->done) return;
+  const double sig = gs->sigma;
+  const bool zero = gs->zero_u != 0;
+  if (tid < n) {
+    int i, j, k;
+    cell_ijk(d, tid, i, j, k);
+    BaseAccess acc{yp, pb, n};
+    double rate[NA] = {fp[3 * n + tid], fp[4 * n + tid]};
+    double Tw = (d.wall_on && i == d.Nr - 1) ? yp[6 * n + j * d.Nz + k] : 0.0;
+    double out[4];
+    cell_flux(m, acc, Tw, i, j, k, t, rate, out);
+    const int rows[4] = {0, 1, 2, 5};
+    for (int r = 0; r < 4; ++r) {
+      int idx = rows[r] * n + tid;
+      double vv = u[idx] * s[idx];
+      w[idx] = zero ? 0.0 : (vv - (dh * (out[r] - f[idx])) / sig) / s[idx];
+    }
+  } else if (tid < n + m.nw) {
+    const int wi = tid - n;
+    const int j = wi / d.Nz, k = wi % d.Nz;
+    Props o;
+    load_props(pb, n, ((d.Nr - 1) * d.Nth + j) * d.Nz + k, o);
+    double fw = wall_row(m, yp + 6 * n, o, j, k, t);
+    int idx = 6 * n + wi;
+    double vv = u[idx] * s[idx];
+    w[idx] = zero ? 0.0 : (vv - (dh * (fw - f[idx])) / sig) / s[idx];
+  }
+}
+
+// ------------------------------------------------------------------ accumulator rates
+// 36 accumulator rates (core/residual.py:371-417): boundary molar and enthalpy flows split by
+// direction per valve, pump and fan work, jacket / ambient / wall / exchanger heat flows.
+// One block; sums in a fixed order (deterministic).
+__device__ inline double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (t < off) sh[t] += sh[t + off];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(BLK) k_acc(Model m, double t, const double* __restrict__ y,
+                                             const double* __restrict__ pb, double* __restrict__ acc) {
+  __shared__ double sh[BLK];
+  const vtsa_model_desc& d = m.d;
+  const int n = m.n, Nth = d.Nth, Nz = d.Nz, Nr = d.Nr;
+  const int nf = Nr * Nth;
+  double out[NACC];
+  for (int a = 0; a < NACC; ++a) out[a] = 0.0;
+  const double ex = (d.gam - 1.0) / d.gam;
+  const double pre = (d.gam / (d.gam - 1.0)) * d.R;
+  for (int end = 0; end < 2; ++end) {
+    const int v = d.end_valve[end];
+    if (v < 0) continue;
+    double loc[10];
+    for (int a = 0; a < 10; ++a) loc[a] = 0.0;
+    const int kl = end == 0 ? 0 : Nz - 1;
+    for (int fidx = threadIdx.x; fidx < nf; fidx += blockDim.x) {
+      const int i = fidx / Nth, j = fidx % Nth;
+      const int cell = (i * Nth + j) * Nz + kl;
+      Props p;
+      load_props(pb, n, cell, p);
+      double c[NG] = {y[cell], y[n + cell], y[2 * n + cell]};
+      EndFlux b;
+      end_flux(d, end, t, m.Az[i], m.feed_flux[fidx], c, p, b);
+      const bool inflow = b.u_out < 0.0;
+      const bool split = d.end_kind[end] == END_VALVE;
+      for (int s = 0; s < NG; ++s) {
+        loc[s] += split ? b.F_inp[s] : (inflow ? b.F_in[s] : 0.0);
+        loc[3 + s] += split ? b.F_outp[s] : (inflow ? 0.0 : -b.F_in[s]);
+      }
+      loc[6] += split ? b.H_inp : (inflow ? b.H_in : 0.0);
+      loc[7] += split ? b.H_outp : (inflow ? 0.0 : -b.H_in);
+      if (d.end_is_vacuum[end]) {
+        double Fo[NG];
+        for (int s = 0; s < NG; ++s) Fo[s] = inflow ? 0.0 : -b.F_in[s];
+        double Ftot = d.pump_water ? (Fo[0] + Fo[1]) + Fo[2] : Fo[0] + Fo[2];
+        double ratio = npmax(d.P_amb / npmax(b.P_face, 1.0), 1.0);
+        double eta = d.eta_vac;
+        if (d.vac_krish) {
+          double pbar = b.P_face / 1e5;
+          eta = npmax(((0.8 * 19.55) * pbar) / (1.0 + 19.55 * pbar), 1e-3);
+        }
+        loc[8] += (((pre * p.T) * Ftot) * (pow(ratio, ex) - 1.0)) / eta;
+      }
+      if (d.end_is_feed[end]) {
+        double Ftot = split ? (b.F_inp[0] + b.F_inp[1]) + b.F_inp[2]
+                            : (inflow ? (b.F_in[0] + b.F_in[1]) + b.F_in[2] : 0.0);
+        double ratio = npmax(b.P_face / d.P_amb, 1.0);
+        loc[9] += (((pre * d.end_T_ext[end]) * Ftot) * (pow(ratio, ex) - 1.0)) / d.eta_bl;
+      }
+    }
+    for (int a = 0; a < 10; ++a) out[v * 10 + a] = block_sum(loc[a], sh);
+  }
+  // heat flows through the wall / jacket (per wall cell, or per outer bed cell without a wall)
+  double qj_pos = 0.0, qj_neg = 0.0, q_amb = 0.0, q_wb = 0.0, qh_pos = 0.0, qh_neg = 0.0;
+  for (int w = threadIdx.x; w < Nth * Nz; w += blockDim.x) {
+    const int j = w / Nz, k = w % Nz;
+    Props o;
+    load_props(pb, n, ((Nr - 1) * Nth + j) * Nz + k, o);
+    double Tw = d.wall_on ? y[6 * n + w] : 0.0;
+    WallFlows wf = wall_flows(m, o, Tw, m.cov[w], t);
+    qj_pos += npmax(wf.Q_jac, 0.0);
+    qj_neg += npmin(wf.Q_jac, 0.0);
+    q_amb += wf.Q_amb;
+    q_wb += wf.Q_wb;
+  }
+  if (d.hx_on) {
+    const double Tj = T_jacket(d, t);
+    for (int cell = threadIdx.x; cell < n; cell += blockDim.x) {
+      const int i = cell / (Nth * Nz);
+      double q = (d.hA_v * m.V[i]) * (Tj - pb[cell]);
+      qh_pos += npmax(q, 0.0);
+      qh_neg += npmin(q, 0.0);
+    }
+  }
+  out[30] = block_sum(qj_pos, sh);
+  out[31] = block_sum(qj_neg, sh);
+  out[32] = block_sum(q_amb, sh);
+  out[33] = block_sum(q_wb, sh);
+  out[34] = block_sum(qh_pos, sh);
+  out[35] = block_sum(qh_neg, sh);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < NACC; ++a) acc[a] = out[a];
+}
+
+// ------------------------------------------------------------------ diagnostics
+__global__ void __launch_bounds__(BLK) k_diag(Model m, double t, const double* __restrict__ y,
+                                              const double* __restrict__ pb, const double* __restrict__ qstar,
+                                              double* __restrict__ cellout, double* __restrict__ uz,
+                                              double* __restrict__ ur, double* __restrict__ uth,
+                                              double* __restrict__ bnd, double* __restrict__ wall) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  const vtsa_model_desc& d = m.d;
+  const int Nr = d.Nr, Nth = d.Nth, Nz = d.Nz;
+  if (tid < n) {
+    int i, j, k;
+    cell_ijk(d, tid, i, j, k);
+    Props p;
+    load_props(pb, n, tid, p);
+    cellout[tid] = p.T;
+    cellout[n + tid] = p.P;
+    cellout[2 * n + tid] = p.y[0];
+    cellout[3 * n + tid] = p.y[1];
+    cellout[4 * n + tid] = p.y[2];
+    cellout[5 * n + tid] = qstar[tid];
+    cellout[6 * n + tid] = qstar[n + tid];
+    cellout[7 * n + tid] = d.hx_on ? (d.hA_v * m.V[i]) * (T_jacket(d, t) - p.T) : 0.0;
+    BaseAccess acc{y, pb, n};
+    double F[NG], H, u;
+    if (k < Nz - 1) { z_face(m, acc, i, j, k, F, H, &u); uz[((i * Nth + j) * (Nz - 1)) + k] = u; }
+    if (i < Nr - 1) { r_face(m, acc, i, j, k, F, H, &u); ur[tid] = u; }
+    if (Nth > 1) { t_face(m, acc, i, j, k, F, H, &u); uth[tid] = u; }
+    if (k == 0 || k == Nz - 1) {
+      double c[NG] = {y[tid], y[n + tid], y[2 * n + tid]};
+      const int nf = Nr * Nth, fidx = i * Nth + j;
+      for (int end = 0; end < 2; ++end) {
+        if ((end == 0 && k != 0) || (end == 1 && k != Nz - 1)) continue;
+        EndFlux b;
+        end_flux(d, end, t, m.Az[i], m.feed_flux[fidx], c, p, b);
+        double* o = bnd + (size_t)end * 14 * nf;
+        for (int s = 0; s < NG; ++s) {
+          o[s * nf + fidx] = b.F_in[s];
+          o[(6 + s) * nf + fidx] = b.F_inp[s];
+          o[(9 + s) * nf + fidx] = b.F_outp[s];
+        }
+        o[3 * nf + fidx] = b.H_in;
+        o[4 * nf + fidx] = b.u_out;
+        o[5 * nf + fidx] = b.P_face;
+        o[12 * nf + fidx] = b.H_inp;
+        o[13 * nf + fidx] = b.H_outp;
+      }
+    }
+  } else if (tid < n + Nth * Nz) {
+    const int w = tid - n;
+    const int j = w / Nz, k = w % Nz;
+    Props o;
+    load_props(pb, n, ((Nr - 1) * Nth + j) * Nz + k, o);
+    double Tw = d.wall_on ? y[6 * n + w] : 0.0;
+    WallFlows wf = wall_flows(m, o, Tw, m.cov[w], t);
+    const int nw2 = Nth * Nz;
+    wall[w] = wf.Q_jac;
+    wall[nw2 + w] = wf.Q_wb;
+    wall[2 * nw2 + w] = wf.Q_amb;
+    wall[3 * nw2 + w] = r_bed(d, d.half_dr_outer, o);
+  }
+}
+
+// ------------------------------------------------------------------ equilibrium service
+__global__ void k_equilibrium(Model m, int count, const double* __restrict__ pc, const double* __restrict__ pw,
+                              const double* __restrict__ T, const double* __restrict__ qw, double* __restrict__ qc,
+                              double* __restrict__ qwe) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const vtsa_model_desc& d = m.d;
+  double psat = arden_buck(T[i]);
+  double we = q_h2o(pw[i], T[i], psat, d);
+  qwe[i] = we;
+  qc[i] = q_co2(pc[i], T[i], d.water_basis_eq ? we : qw[i], d);
+}
+
+// ------------------------------------------------------------------ finite-difference Jacobian
+// FD step of one column (core/jacobian.py:75-76): h = (y + 1.5e-8 max(|y|, typ)) - y
+__device__ inline double fd_step(double yv, double typ) {
+  double h = 1.5e-8 * npmax(fabs(yv), typ);
+  return (yv + h) - yv;
+}
+
+// Stencil slot -> neighbour cell of (i, j, k), or -1 if it does not exist.
+__device__ inline int slot_cell(const vtsa_model_desc& d, int i, int j, int k, int slot) {
+  const int Nr = d.Nr, Nth = d.Nth, Nz = d.Nz;
+  int ii = i, jj = j, kk = k;
+  switch (slot) {
+    case S_SELF: break;
+    case S_ZM1: kk = k - 1; break;
+    case S_ZP1: kk = k + 1; break;
+    case S_ZM2: kk = k - 2; break;
+    case S_ZP2: kk = k + 2; break;
+    case S_RM1: ii = i - 1; break;
+    case S_RP1: ii = i + 1; break;
+    case S_TM1: if (Nth == 1) return -1; jj = (j == 0) ? Nth - 1 : j - 1; break;
+    case S_TP1: if (Nth == 1) return -1; jj = (j + 1 == Nth) ? 0 : j + 1; break;
+  }
+  if (kk < 0 || kk >= Nz || ii < 0 || ii >= Nr) return -1;
+  return (ii * Nth + jj) * Nz + kk;
+}
+
+// Thread (X, slot, v): column v of stencil cell A = slot(X); writes Jc[X][slot][0..5][v].
+__global__ void __launch_bounds__(BLK) k_fd_cells(Model m, double t, const double* __restrict__ y,
+                                                  const double* __restrict__ pb, const double* __restrict__ f0,
+                                                  double* __restrict__ Jc) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  if (tid >= n * NSLOT * NV) return;
+  const vtsa_model_desc& d = m.d;
+  const int v = tid % NV;
+  const int slot = (tid / NV) % NSLOT;
+  const int X = tid / (NV * NSLOT);
+  int i, j, k;
+  cell_ijk(d, X, i, j, k);
+  double* out = Jc + ((size_t)X * NSLOT + slot) * 36;
+  const int A = slot_cell(d, i, j, k, slot);
+  if (A < 0) {
+    for (int r = 0; r < NV; ++r) out[r * NV + v] = 0.0;
+    return;
+  }
+  double ya[NV];
+  for (int vv = 0; vv < NV; ++vv) ya[vv] = y[vv * n + A];
+  const double h = fd_step(ya[v], d.typ[v]);
+  ya[v] = ya[v] + h;
+  Props pa;
+  double rate_a[NA], qs[NA];
+  cell_props(d, ya, ya + 3, ya[5], pa, rate_a, qs);
+  SwapAccess acc{y, pb, n, A, &pa, ya};
+  double rate[NA];
+  if (A == X) { rate[0] = rate_a[0]; rate[1] = rate_a[1]; }
+  else { rate[0] = f0[3 * n + X]; rate[1] = f0[4 * n + X]; }
+  double Tw = (d.wall_on && i == d.Nr - 1) ? y[6 * n + j * d.Nz + k] : 0.0;
+  double fl[4];
+  cell_flux(m, acc, Tw, i, j, k, t, rate, fl);
+  out[0 * NV + v] = (fl[0] - f0[X]) / h;
+  out[1 * NV + v] = (fl[1] - f0[n + X]) / h;
+  out[2 * NV + v] = (fl[2] - f0[2 * n + X]) / h;
+  out[5 * NV + v] = (fl[3] - f0[5 * n + X]) / h;
+  if (A == X) {
+    out[3 * NV + v] = (rate_a[0] - f0[3 * n + X]) / h;
+    out[4 * NV + v] = (rate_a[1] - f0[4 * n + X]) / h;
+  } else {
+    out[3 * NV + v] = 0.0;
+    out[4 * NV + v] = 0.0;
+  }
+}
+
+// Thread (W, col): col 0..4 = wall columns (self, z-1, z+1, t-1, t+1), col 5..10 = outer-cell
+// variables.  Writes Jw[W][col], Jwc[W][col-5] and, for col 0, Jcw[W][0..3].
+__global__ void __launch_bounds__(BLK) k_fd_wall(Model m, double t, const double* __restrict__ y,
+                                                 const double* __restrict__ pb, const double* __restrict__ f0,
+                                                 double* __restrict__ Jcw, double* __restrict__ Jw,
+                                                 double* __restrict__ Jwc) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n, nw = m.nw;
+  if (tid >= nw * 11) return;
+  const vtsa_model_desc& d = m.d;
+  const int col = tid % 11, W = tid / 11;
+  const int Nz = d.Nz, Nth = d.Nth;
+  const int j = W / Nz, k = W % Nz;
+  const int X = ((d.Nr - 1) * Nth + j) * Nz + k;
+  const double* Tw_all = y + 6 * n;
+  const double f0w = f0[6 * n + W];
+  if (col < 5) {
+    int jj = j, kk = k;
+    bool ok = true;
+    if (col == 1) { kk = k - 1; ok = kk >= 0; }
+    if (col == 2) { kk = k + 1; ok = kk < Nz; }
+    if (col == 3) { ok = Nth > 1; jj = (j == 0) ? Nth - 1 : j - 1; }
+    if (col == 4) { ok = Nth > 1; jj = (j + 1 == Nth) ? 0 : j + 1; }
+    if (!ok) { Jw[W * 5 + col] = 0.0; return; }
+    const int Wp = jj * Nz + kk;
+    const double h = fd_step(Tw_all[Wp], d.typ[6]);
+    const double tw_new = Tw_all[Wp] + h;
+    Props o;
+    load_props(pb, n, X, o);
+    double fp = wall_row(m, Tw_all, o, j, k, t, Wp, tw_new);
+    Jw[W * 5 + col] = (fp - f0w) / h;
+    if (col == 0) {
+      // outer cell rows wrt its wall temperature
+      BaseAccess acc{y, pb, n};
+      double rate[NA] = {f0[3 * n + X], f0[4 * n + X]};
+      double fl[4];
+      cell_flux(m, acc, tw_new, d.Nr - 1, j, k, t, rate, fl);
+      Jcw[W * 4 + 0] = (fl[0] - f0[X]) / h;
+      Jcw[W * 4 + 1] = (fl[1] - f0[n + X]) / h;
+      Jcw[W * 4 + 2] = (fl[2] - f0[2 * n + X]) / h;
+      Jcw[W * 4 + 3] = (fl[3] - f0[5 * n + X]) / h;
+    }
+  } else {
+    const int v = col - 5;
+    double ya[NV];
+    for (int vv = 0; vv < NV; ++vv) ya[vv] = y[vv * n + X];
+    const double h = fd_step(ya[v], d.typ[v]);
+    ya[v] = ya[v] + h;
+    Props po;
+    double rate[NA], qs[NA];
+    cell_props(d, ya, ya + 3, ya[5], po, rate, qs);
+    double fp = wall_row(m, Tw_all, po, j, k, t);
+    Jwc[W * 6 + v] = (fp - f0w) / h;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int nblk(long long work, int b = BLK) { return (int)((work + b - 1) / b); }
+
+void launch_props(Handle& h, const double* y, double* pb, double* f, double* qstar, hipStream_t s) {
+  hipLaunchKernelGGL(k_props, dim3(nblk(h.n)), dim3(BLK), 0, s, h.m, y, pb, f, qstar);
+}
+
+void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, s, h.m, t, y, pb, f);
+}
+
+void launch_acc(Handle& h, double t, const double* y, const double* pb, double* acc, hipStream_t s) {
+  hipLaunchKernelGGL(k_acc, dim3(1), dim3(BLK), 0, s, h.m, t, y, pb, acc);
+}
+
+void launch_diag(Handle& h, double t, const double* y, const double* pb, double* cell, double* uz, double* ur,
+                 double* uth, double* bnd, double* wall, hipStream_t s) {
+  // qstar of the current point: recompute the properties with the q* side output
+  double* qs = h.vtmp2;
+  hipLaunchKernelGGL(k_props, dim3(nblk(h.n)), dim3(BLK), 0, s, h.m, y, h.pb, nullptr, qs);
+  hipLaunchKernelGGL(k_diag, dim3(nblk(h.n + h.desc.Nth * h.desc.Nz)), dim3(BLK), 0, s, h.m, t, y, h.pb, qs,
+                     cell, uz, ur, uth, bnd, wall);
+}
+
+void launch_fdjac(Handle& h, double t, const double* y, const double* f0, hipStream_t s) {
+  // properties at y (the base point of every column)
+  launch_props(h, y, h.pb, nullptr, nullptr, s);
+  hipLaunchKernelGGL(k_fd_cells, dim3(nblk((long long)h.n * NSLOT * NV)), dim3(BLK), 0, s, h.m, t, y, h.pb, f0,
+                     h.Jc);
+  if (h.nw > 0)
+    hipLaunchKernelGGL(k_fd_wall, dim3(nblk((long long)h.nw * 11)), dim3(BLK), 0, s, h.m, t, y, h.pb, f0, h.Jcw,
+                       h.Jw, h.Jwc);
+}
+
+void launch_equilibrium(Handle& h, int count, const double* pc, const double* pw, const double* T,
+                        const double* qw, double* qc, double* qwe, hipStream_t s) {
+  hipLaunchKernelGGL(k_equilibrium, dim3(nblk(count)), dim3(BLK), 0, s, h.m, count, pc, pw, T, qw, qc, qwe);
+}
+
+void launch_matvec_residual(Handle& h, double t, const double* z, const double* u, const double* s,
+                            const double* f, double dh, double* w, bool respect_done, hipStream_t st) {
+  const int rd = respect_done ? 1 : 0;
+  hipLaunchKernelGGL(k_mv_props, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, z, u, s, f, h.gs, rd, dh, h.ytmp,
+                     h.pb2, h.ftmp, w);
+  hipLaunchKernelGGL(k_mv_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, t, h.ytmp, h.pb2, h.ftmp, u, s, f,
+                     h.gs, rd, dh, w);
+}
+
+}  // namespace vtsa

@@ -1,0 +1,160 @@
+"""Step runner (drop-in for upstream steps/_runner.py:26-188).
+
+``run_step`` builds the device model of the step, integrates it on the GPU, records one
+outlet row per accepted step from the device diagnostics, and returns ``(CycleState,
+StepResult)`` exactly as upstream; the state stays in HBM for the whole step.
+"""
+from __future__ import annotations
+
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from ..core.residual import TVSAModel
+from ..core.solver import TRBDF2
+from ..properties import R_GAS
+from ..valves import humid_air_composition
+from .result import StepResult
+
+
+def _end_summary(model, d, end, c_feed_co2):
+    """Outlet composition/flow summary of one end (steps/_runner.py:50-84)."""
+    b = d["bnd"][end]
+    Fout = -b["F_in"]
+    tot = Fout.sum(axis=0)
+    Ftot = float(tot.sum())
+    k_lay = 0 if end == "top" else -1
+    T_lay = d["T"][..., k_lay]
+    A = model.grid.Az * np.ones_like(T_lay)
+    P_face = float(np.sum(b["P_face"] * A) / np.sum(A))
+    out_faces = tot > 0.0
+    if "F_outpart" in b:
+        Fout = b["F_outpart"]
+        tot = Fout.sum(axis=0)
+        out_faces = tot > 0.0
+    F_leaving = float(tot[out_faces].sum()) if np.any(out_faces) else 0.0
+    if F_leaving > 1e-14:
+        ys = [float(Fout[k][out_faces].sum()) / F_leaving for k in range(model.ng)]
+        T_g = float(np.sum(T_lay[out_faces] * tot[out_faces]) / F_leaving)
+    else:
+        yl = d["y"][..., k_lay]
+        ys = [float(np.sum(yl[k] * A) / np.sum(A)) for k in range(model.ng)]
+        T_g = float(np.sum(T_lay * A) / np.sum(A))
+    out = {f"{end}_valve": model.conds[end].valve or "closed", f"{end}_valve_state": model.conds[end].state,
+           f"{end}_flow_out_mol_s": Ftot, f"{end}_P_Pa": P_face, f"{end}_T_K": T_g}
+    for k, s in enumerate(model.species):
+        out[f"{end}_y_{s}"] = ys[k]
+    c_co2 = ys[model.i_co2] * P_face / (R_GAS * T_g)
+    out[f"{end}_c_CO2_mol_m3"] = c_co2
+    out[f"{end}_C_over_Cfeed"] = c_co2 / c_feed_co2 if c_feed_co2 > 0 else np.nan
+    return out
+
+
+def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle0: float = 0.0,
+             T_jacket_prev: Optional[float] = None, logger=None, snapshots: Optional[int] = None,
+             record_outlet: bool = True, integrator_kw: Optional[Dict] = None):
+    """One cycle step on the GPU -> (state_out, StepResult) (steps/_runner.py:87-188).
+
+    ``record_outlet`` and ``integrator_kw`` (preconditioner kind/precision) have no upstream
+    counterpart; the defaults reproduce the upstream behaviour."""
+    log = logger or (lambda *a, **k: None)
+    model = TVSAModel(cfg, grid, props, step, state_in, T_jacket_prev)
+    y0 = model.pack(state_in)
+    dur = float(step["duration"])
+    n_snap = snapshots if snapshots is not None else cfg.run.snapshots_per_step
+    t_out = np.linspace(0.0, dur, max(2, n_snap))
+    yf = humid_air_composition(model.species, cfg.feed.P, cfg.feed.T, cfg.feed.RH, cfg.feed.y_co2_dry,
+                               float(props.p_sat(cfg.feed.T)))
+    c_feed_co2 = yf[model.i_co2] * cfg.feed.P / (R_GAS * cfg.feed.T)
+    outlet: List[Dict] = []
+    every = cfg.solver.outlet_record_every
+    counter = [0]
+    V = grid.volume_full()
+    Vt = float(V.sum())
+    Vr = np.ascontiguousarray(V)
+
+    def record(t, y, acc):
+        d = model.diag(t, y)
+        row = {"cycle": cycle_idx, "step": step["name"], "t_step_s": t, "t_cycle_s": t_cycle0 + t,
+               "t_total_s": state_in.t + t}
+        for end in ("top", "bottom"):
+            row.update(_end_summary(model, d, end, c_feed_co2))
+        row["bed_T_mean_K"] = float(np.sum(d["T"] * Vr) / Vt)
+        row["bed_P_mean_Pa"] = float(np.sum(d["P"] * Vr) / Vt)
+        q = model.inventories(y)
+        for s in model.adsorbed:
+            row[f"bed_q_{s}_mean"] = q[f"ads_{s}"] / model.rho_b / Vt
+        ai = model.acc_index
+        for v in ("V1", "V2", "V3"):
+            row[f"cum_CO2_out_{v}_mol"] = acc[ai[f"n_out_{v}_CO2"]]
+        row["cum_Q_jacket_J"] = acc[ai["Q_jac_pos"]] + acc[ai["Q_jac_neg"]]
+        outlet.append(row)
+
+    last_print = [time.time()]
+
+    def on_step(t, y, acc):
+        counter[0] += 1
+        if record_outlet and counter[0] % every == 0:
+            record(t, y, acc)
+        if time.time() - last_print[0] > 30.0:
+            last_print[0] = time.time()
+            log(f"      ... {step['name']}: t = {t:.0f}/{dur:.0f} s ({100 * t / dur:.0f} %), {counter[0]} steps")
+
+    on_step.wants_device = True
+    if record_outlet:
+        record(0.0, y0, np.zeros(model.n_acc))
+    stop = step.get("stop_on")
+    stop_fn = stop_val = None
+    direction = 1
+    if stop:
+        stop_fn, stop_val = model.stop_metric, stop["value"]
+        direction = -1 if stop["criterion"] == "min_temperature" else 1
+    wall0 = time.time()
+    if cfg.solver.method != "trbdf2":
+        raise RuntimeError(f"solver.method = {cfg.solver.method}: only the implicit trbdf2 path runs on the GPU")
+    integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **(integrator_kw or {}))
+    res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
+                          stop_direction=direction)
+    solver_stats = {"method": "trbdf2", "steps": res.n_steps, "rejected": res.n_reject,
+                    "newton_fail": res.n_newton_fail, "krylov": res.n_krylov, "jacobians": res.n_jac,
+                    "lu": res.n_lu, "nfev": res.nfev, "groups": "all-columns", **integ.stats}
+    if record_outlet and outlet[-1]["t_step_s"] < res.t:
+        record(res.t, res.y, res.acc)
+    wall = time.time() - wall0
+    state_out = model.to_state(res.y, state_in, state_in.t + res.t, step["name"])
+    state_out.cycle = cycle_idx
+    snaps = []
+    for (ts, ys, _a) in res.snapshots:
+        st = model.to_state(ys, state_in, state_in.t + ts, step["name"])
+        st.cycle = cycle_idx
+        snaps.append({"cycle": cycle_idx, "step": step["name"], "t_step": ts, "t_cycle": t_cycle0 + ts,
+                      "t_total": state_in.t + ts, "state": st})
+    inv0, inv1 = model.inventories(y0), model.inventories(res.y)
+    T_jac = model.T_jacket(res.t) if model.jacket_on else cfg.ambient.T
+    sr = StepResult(name=step["name"], type=step["type"], cycle=cycle_idx, t_start_cycle=t_cycle0,
+                    duration_planned=dur, duration=res.t, stopped_early=res.stopped_early,
+                    acc={nm: float(res.acc[i]) for i, nm in enumerate(model.acc_names)}, inv_start=inv0,
+                    inv_end=inv1, outlet=outlet, snapshots=snaps, solver=solver_stats, wall_time=wall,
+                    T_jacket=float(T_jac), jacket_on=model.jacket_on, product_port=step.get("product_port", "V3"),
+                    valves=dict(step["valves"]), ads_start={s: inv0[f"ads_{s}"] for s in model.adsorbed},
+                    ads_end={s: inv1[f"ads_{s}"] for s in model.adsorbed})
+    sr.steps_taken = res.steps_taken
+    log(f"    {step['name']:<14s} {res.t:9.1f} s  steps={res.n_steps:>5}  wall={wall:6.1f}s")
+    model.close()
+    return state_out, sr
+
+
+def _spec(cfg, step_type, name=None):
+    for s in cfg.cycle:
+        if s["type"] == step_type and (name is None or s["name"] == name):
+            return s
+    raise KeyError(f"no '{step_type}' step in cfg.cycle")
+
+
+def _make_runner(step_type):
+    def run(state_in, cfg, grid, props, step=None, **kw):
+        return run_step(state_in, cfg, grid, props, step or _spec(cfg, step_type), **kw)
+    run.__name__ = f"run_{step_type}"
+    run.__doc__ = f"Run the {step_type} step (spec from cfg.cycle unless given) -> (state_out, result)."
+    return run

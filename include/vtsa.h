@@ -1,0 +1,180 @@
+/* C ABI of the MI355X implicit step engine for the dac_vtsa TVSA bed model.
+ *
+ * One opaque handle per cycle step (valve states, jacket and grid fixed), as the upstream
+ * TVSAModel is built once per step (core/residual.py:41-123).  All vectors are device pointers
+ * (FP64) in the upstream state layout (core/residual.py:170-183):
+ *     y = [c_CO2 | c_H2O | c_N2 | q_CO2 | q_H2O | e | Tw]   cell = (i*Ntheta + j)*Nz + k
+ * Every call takes a HIP stream (NULL = default stream) and returns 0 on success, > 0 for a
+ * numerical failure (non-finite values) and < 0 for an API or HIP error; no exception crosses
+ * the ABI.  A handle is not thread-safe: calls on one handle must not overlap.
+ *
+ * The upstream simulator has no FFI of its own; its Python call sites are the boundary and the
+ * comment of each entry point names the upstream function it replaces (path:line).
+ */
+#ifndef VTSA_H
+#define VTSA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vtsa_handle vtsa_handle;
+typedef void* vtsa_stream;   /* hipStream_t */
+
+enum { VTSA_COADS_NONE = 0, VTSA_COADS_WADST = 1, VTSA_COADS_MECHANISTIC = 2, VTSA_COADS_SB = 3 };
+enum { VTSA_DISP_EDWARDS_RICHARDSON = 0, VTSA_DISP_YOUNG2021 = 1, VTSA_DISP_WAKAO_KAGEI = 2 };
+enum { VTSA_BED_ZS = 0, VTSA_BED_PARALLEL = 1, VTSA_BED_CONSTANT = 2 };
+enum { VTSA_WALLHTC_MN = 0, VTSA_WALLHTC_CONSTANT = 1 };
+enum { VTSA_PREC_BLOCK_JACOBI = 0, VTSA_PREC_ZLINE = 1 };
+enum { VTSA_FP64 = 0, VTSA_FP32 = 1 };
+
+/* Everything the residual of one cycle step depends on, as scalars.  Filled by the host from
+ * the validated configuration (upstream config/schema.py keys) and the step's boundary
+ * conditions (valves/valve_logic.py:71-148).  Ends: 0 = top (z = 0), 1 = bottom (z = L). */
+typedef struct vtsa_model_desc {
+  int32_t Nr, Nth, Nz;
+  int32_t wall_on, lumped, hx_on, jacket_on;
+  int32_t limiter, disp_model, coads_model, water_basis_eq, bed_model, wall_model;
+  int32_t arrhenius, mn_dynamic, vac_krish, pump_water;
+  int32_t end_kind[2], end_check[2], end_valve[2], end_is_vacuum[2], end_is_feed[2];
+  int32_t pad_;
+  double end_P_target[2], end_T_ext[2], end_speed[2], end_P_start[2], end_y_ext[2][3];
+  double dz, dth, A_wall_in, A_wall_out, A_wall_axial, R_wall_mid, wall_thickness, half_dr_outer, C_wall;
+  double R, eps_b, eps_t, rho_b, cp_s, T_ref;
+  double cpg[3], cpa[2], dHa[2], M[3];
+  double mu_f1[3], mu_a1[3], mu_b1[3], mu_f2[3], mu_a2[3], mu_b2[3];
+  double k_c1[3], k_n1[3], k_c2[3], k_n2[3];
+  int32_t mu_two[3], pad2_;
+  double wilke_mq[3][3], wilke_den[3][3];
+  double fuller_sqrtM_air, fuller_s_air, fuller_sqrtM_h2o, fuller_s_h2o;
+  double lam_const, lam_s, zs_B, zs_s;
+  double co2_q_inf0, co2_chi, co2_b0, co2_dH0, co2_tau0, co2_alpha, co2_T0, co2_p_reg;
+  double h2o_q_m, h2o_C, h2o_D, h2o_F, h2o_G, h2o_E10_slope, h2o_E10_intercept, h2o_rh_cap;
+  double wadst_b0, wadst_q_inf0, wadst_tau0, wadst_chi, wadst_alpha, wadst_dH, wadst_A;
+  double mech_f_blocked_max, mech_k, mech_phi_dry, mech_A, mech_dH_wet, mech_n;
+  double sb_gamma, sb_beta;
+  double k_ldf[2], Ea[2], k_T_ref;
+  double ka, kb, dp, er_g1, er_g2, er_beta, Pe_T, Pe_r, Pe_z, N_ratio, alpha_const, U_lumped;
+  double u_nom, tau_valve;
+  double h_jac, h_amb, T_amb, P_amb, T_jac_set, T_jac_prev, jac_tau, hA_v, lam_w;
+  double gam, eta_vac, eta_bl;
+  double typ[7];          /* FD perturbation floors per variable (core/jacobian.py:40-48) */
+  double atol[7];         /* absolute tolerances per variable (core/residual.py:135-144) */
+} vtsa_model_desc;
+
+/* GMRES outcome (scipy.sparse.linalg.gmres semantics as called at core/solver.py:223-224). */
+typedef struct vtsa_gmres_info {
+  int32_t info;          /* 0 converged, 1 not converged within one restart cycle */
+  int32_t iters;         /* Arnoldi iterations (the upstream callback count) */
+  int32_t breakdown;
+  int32_t pad_;
+  double bnorm, presid, rnorm, atol_eff;
+} vtsa_gmres_info;
+
+/* Build / free the device model of one step.  Replaces TVSAModel.__init__
+ * (core/residual.py:41-123).  Ring arrays have Nr entries (Ar, drc: Nr-1); feed_flux and N_nom
+ * have Nr*Nth entries; cov has Nth*Nz.  All are host pointers and are copied. */
+int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, const double* Ath,
+                const double* arc, const double* Ar, const double* drc, const double* feed_flux,
+                const double* N_nom, const double* cov, vtsa_handle** out);
+int vtsa_destroy(vtsa_handle* h);
+int vtsa_sizes(const vtsa_handle* h, int32_t* n_cells, int32_t* n_wall, int32_t* n_unknowns);
+
+/* f = dy/dt at (t, y); acc (36 accumulator rates, device) when non-NULL.
+ * Replaces TVSAModel.rhs (core/residual.py:211-417). */
+int vtsa_rhs(vtsa_handle* h, double t, const double* y, double* f, double* acc, vtsa_stream s);
+
+/* R = y - gamma_h f(t, y) - rhs_known.  Replaces TVSAModel.stage_residual (core/residual.py:420-423). */
+int vtsa_stage_residual(vtsa_handle* h, double t, const double* y, double gamma_h, const double* rhs_known,
+                        double* R, vtsa_stream s);
+
+/* Diagnostic fields of rhs(diag=True) (core/residual.py:366-368), device outputs:
+ *  cell[8*n]  = T, P, y_CO2, y_H2O, y_N2, q*_CO2, q*_H2O, Q_hx
+ *  uz[Nr*Nth*(Nz-1)], ur[(Nr-1)*Nth*Nz], uth[Nr*Nth*Nz]
+ *  bnd[2][14][Nr*Nth] = per end: F_in(3), H_in, u_out, P_face, F_inpart(3), F_outpart(3), H_inpart, H_outpart
+ *  wall[4][Nth*Nz]    = Q_jac, Q_wb, Q_amb, R_bed */
+int vtsa_diag(vtsa_handle* h, double t, const double* y, double* cell, double* uz, double* ur, double* uth,
+              double* bnd, double* wall, vtsa_stream s);
+
+/* Equilibrium loadings for arbitrary points (isotherm service; physics/isotherms.py:262-269).
+ * q_h2o_state is used as the CO2 water basis when the configuration asks for it. */
+int vtsa_equilibrium(vtsa_handle* h, int32_t count, const double* p_co2, const double* p_h2o, const double* T,
+                     const double* q_h2o_state, double* q_co2_eq, double* q_h2o_eq, vtsa_stream s);
+
+/* Finite-difference Jacobian of f at (t, y) given f0 = f(t, y), into handle-owned block storage:
+ *  Jc [n][9][6][6]  cell rows x (self, z-1, z+1, z-2, z+2, r-1, r+1, t-1, t+1) cell columns
+ *  Jcw[Nth*Nz][4]   outer-ring flux rows (c_CO2, c_H2O, c_N2, e) x own wall column
+ *  Jw [Nth*Nz][5]   wall rows x wall columns (self, z-1, z+1, t-1, t+1)
+ *  Jwc[Nth*Nz][6]   wall rows x outer-cell columns
+ * Entry values equal the upstream coloured forward differences (core/jacobian.py:71-86) with the
+ * same steps h = (y + 1.5e-8 max(|y|, typ)) - y; all columns are evaluated in one launch. */
+int vtsa_fd_jacobian(vtsa_handle* h, double t, const double* y, const double* f0, vtsa_stream s);
+/* Copy the block Jacobian to host buffers (any may be NULL). */
+int vtsa_jacobian_get(vtsa_handle* h, double* Jc, double* Jcw, double* Jw, double* Jwc);
+
+/* Factor the preconditioner M ~ I - dh J in the variables scaled by `scale` (device, length N).
+ * Replaces TRBDF2._factor / splu (core/solver.py:334-337).  kind: VTSA_PREC_*; precision:
+ * VTSA_FP64 or VTSA_FP32 (factors stored and applied in FP32, vectors stay FP64). */
+int vtsa_precond_setup(vtsa_handle* h, double dh, const double* scale, int32_t kind, int32_t precision,
+                       vtsa_stream s);
+/* out = M^-1 r (unscaled); replaces lu.solve (core/solver.py:182,209,460).  out may alias r. */
+int vtsa_precond_apply(vtsa_handle* h, const double* r, double* out, vtsa_stream s);
+
+/* Right-preconditioned GMRES(restart), one cycle, x0 = 0, on the scaled Newton operator
+ *   A u = (v - dh (f(t, z + sigma v) - f)/sigma)/s,  v = u*s,  sigma = sqrt(eps) zs_norm / rms(u)
+ *   P^-1 w = M^-1(w*s)/s
+ * solving (A P^-1) w = b with tolerance max(atol, rtol |b|) (core/solver.py:196-224).
+ * On return w holds the GMRES solution (in the preconditioned variable, as upstream). */
+int vtsa_gmres(vtsa_handle* h, double t, const double* z, const double* f, const double* s, double dh,
+               double zs_norm, const double* b, double* w, double rtol, double atol, int32_t restart,
+               vtsa_gmres_info* info, vtsa_stream s_);
+
+/* ---- Newton vector operations (core/solver.py:84-108,169-259); scalars returned on the host. */
+/* s = atol + rtol |z| */
+int vtsa_scale(vtsa_handle* h, const double* z, double rtol, double* s, vtsa_stream st);
+/* out[0] = sqrt(mean((a/s)^2)) (s NULL: plain rms) */
+int vtsa_rms(vtsa_handle* h, const double* a, const double* s, int32_t len, double* out, vtsa_stream st);
+/* f = f(t,z); R = z - dh f - rhs_known; rn = rms(R/s); pn = rms(M^-1 R / s) */
+int vtsa_newton_eval(vtsa_handle* h, double t, const double* z, double dh, const double* rhs_known,
+                     const double* s, double* f, double* R, double* rn, double* pn, vtsa_stream st);
+/* z_try = truncate(z + lam dz, z); f_try, R_try at z_try; rn, pn as above; step = rms((z_try-z)/s) */
+int vtsa_newton_try(vtsa_handle* h, double t, const double* z, const double* dz, double lam, double dh,
+                    const double* rhs_known, const double* s, double* z_try, double* f_try, double* R_try,
+                    double* rn, double* pn, double* step, vtsa_stream st);
+/* conservation test |W R| <= tol W s for every conserved quantity (core/solver.py:88-96);
+ * wr[4] = W R and ws[4] = W s returned on the host. */
+int vtsa_cons(vtsa_handle* h, const double* R, const double* s, double* wr, double* ws, vtsa_stream st);
+/* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall
+ * (core/residual.py:426-443) */
+int vtsa_inventories(vtsa_handle* h, const double* y, double* out, vtsa_stream st);
+
+/* Integrator vector formulas (core/solver.py:419-476), upstream operation order; npos entries
+ * (c and q) are truncated for ops 1 and 2 (core/solver.py:99-108):
+ *  0: out = x + a y            (y + dh f_n)
+ *  1: out = trunc(x + a y, x)  (stage predictor y + (c h) slope)
+ *  2: out = trunc(x + (y - x)/a, x)   (stage-2 predictor)
+ *  3: out = x + a ((y - x)/b)  (SDIRK2 stage-2 right-hand side)
+ *  4: out = a y - b x          (TR-BDF2 stage-2 right-hand side)
+ *  5: out = x + a ((b y + c z) + e w)  (third-order quadrature y3)
+ *  6: out = (y - x)/a          (slope)
+ *  7: out = y - x
+ *  8: out = (-x)/y             (GMRES right-hand side -R/s)
+ *  9: out = x y */
+int vtsa_vec_op(vtsa_handle* h, int32_t op, double a, double b, double c, double e, const double* x, const double* y,
+                const double* z, const double* w, double* out, vtsa_stream st);
+/* en = rms(M^-1 (z2 - y3) / (atol + rtol max(|y|, |z2|))), the filtered local error
+ * (core/solver.py:460-463). */
+int vtsa_error_norm(vtsa_handle* h, const double* y, const double* z2, const double* y3, double rtol, double* en,
+                    vtsa_stream st);
+
+/* Library version string and last error message (host memory owned by the library). */
+const char* vtsa_version(void);
+const char* vtsa_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,329 @@
+"""HIP path vs the CPU oracle and the upstream golden vectors (needs an MI355X).
+
+Tolerances: the residual restates the upstream FP64 arithmetic operation by operation, so it
+agrees with the oracle to round-off of the device libm (normwise 1e-12, looser only where a
+closed form of the device math library differs by ulps).  Finite-difference Jacobian entries
+amplify ulp differences of f by 1/h ~ 1e8 (h = 1.5e-8 |y|), so they are compared at 1e-6 of the
+column maximum, and checked bit-exactly against the device's own one-column differences.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dac_vtsa_amd.config import load_config
+from tests._helpers import CFG1, MID, SMALL, VARIANTS, golden, oracle_model, oracle_qstar, rel, state_from
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_model(over, step_index, state, T_jacket_prev=None):
+    from dac_vtsa_amd.core.grid import Grid
+    from dac_vtsa_amd.core.residual import TVSAModel
+    from dac_vtsa_amd.properties import get_backend
+    cfg = load_config(override_dict=over)
+    return cfg, TVSAModel(cfg, Grid.from_config(cfg), get_backend(cfg), cfg.cycle[step_index], state, T_jacket_prev)
+
+
+def test_library_loaded_from_tree():
+    from dac_vtsa_amd import _abi
+    L = _abi.lib()
+    assert b"gfx950" in L.vtsa_version()
+
+
+# ------------------------------------------------------------------ residual
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_rhs_matches_golden(variant):
+    """TVSAModel.rhs (core/residual.py:211-417): every step type and option variant on 3x4x6."""
+    z = golden("rhs_small.npz")
+    st = state_from(z, f"{variant}/")
+    for si in range(4):
+        cfg, m = gpu_model({**SMALL, **VARIANTS[variant]}, si, st, T_jacket_prev=300.0)
+        p = f"{variant}/s{si}/"
+        f, acc = m.rhs(12.5, z[p + "y"])
+        e = rel(f, z[p + "f"])
+        assert e < 1e-12, (variant, si, e)
+        np.testing.assert_allclose(acc, z[p + "acc"], rtol=1e-11, atol=1e-12 * np.abs(z[p + "acc"]).max())
+        m.close()
+
+
+def test_rhs_rows_componentwise_cfg1():
+    """Every row block, relative to its own scale, on the seeded 4x4x20 state."""
+    z = golden("rhs_cfg1.npz")
+    cfg, m = gpu_model(CFG1, 0, state_from(z))
+    f, acc = m.rhs(5.0, z["y"])
+    n = m.grid.ncell
+    for b in range(6):
+        sl = slice(b * n, (b + 1) * n)
+        assert rel(f[sl], z["f"][sl]) < 1e-11, b
+    assert rel(f[6 * n:], z["f"][6 * n:]) < 1e-12
+    np.testing.assert_allclose(acc, z["acc"], rtol=1e-11, atol=1e-12 * np.abs(z["acc"]).max())
+    R = m.stage_residual(5.0, z["y"], 0.37, z["rhs_known"])
+    assert rel(R, z["R"]) < 1e-12
+
+
+def test_rhs_midcycle_states():
+    z = golden("rhs_mid.npz")
+    st = state_from(z)
+    for si in range(4):
+        cfg, m = gpu_model(MID, si, st)
+        f, acc = m.rhs(3.0, z[f"s{si}/y"])
+        assert rel(f, z[f"s{si}/f"]) < 1e-12, si
+        np.testing.assert_allclose(acc, z[f"s{si}/acc"], rtol=1e-10, atol=1e-12 * np.abs(z[f"s{si}/acc"]).max())
+
+
+def test_diag_matches_golden():
+    z = golden("rhs_small.npz")
+    st = state_from(z, "base/")
+    for si in range(4):
+        cfg, m = gpu_model(SMALL, si, st, T_jacket_prev=300.0)
+        d = m.rhs(12.5, z[f"base/s{si}/y"], diag=True)
+        p = f"base/s{si}/d_"
+        for k in ("T", "P", "y", "qstar", "uz", "ur", "uth", "Q_jac", "Q_wb", "Q_amb", "R_bed"):
+            np.testing.assert_allclose(d[k], z[p + k], rtol=1e-11, atol=1e-14 * np.abs(z[p + k]).max(), err_msg=k)
+        for end in ("top", "bottom"):
+            for k, v in d["bnd"][end].items():
+                ref = z[p + f"bnd_{end}_{k}"]
+                np.testing.assert_allclose(v, ref, rtol=1e-11, atol=1e-14 * max(np.abs(ref).max(), 1e-300),
+                                           err_msg=f"{end}/{k}")
+
+
+@pytest.mark.parametrize("step_index", [0, 1, 2, 3])
+def test_device_rhs_conserves_species_and_energy(step_index):
+    """Invariant of the device residual + accumulators (upstream tests :79-104)."""
+    z = golden("rhs_small.npz")
+    st = state_from(z, "base/")
+    cfg, m = gpu_model(SMALL, step_index, st)
+    g = m.grid
+    f, acc = m.rhs(1.0, m.pack(st))
+    ai = m.acc_index
+    dc = f[:m.n_c].reshape((3,) + g.shape)
+    dq = f[m.off_q:m.off_e].reshape((2,) + g.shape)
+    for k, s in enumerate(m.species):
+        dinv = np.sum(m.eps_t * dc[k] * g.V) + (np.sum(m.rho_b * dq[k] * g.V) if k < 2 else 0.0)
+        net = sum(acc[ai[f"n_in_{v}_{s}"]] - acc[ai[f"n_out_{v}_{s}"]] for v in ("V1", "V2", "V3"))
+        assert abs(dinv - net) <= 1e-12 * max(1.0, abs(net)) + 1e-18
+    de = f[m.off_e:m.off_w].reshape(g.shape)
+    dTw = f[m.off_w:m.n_core].reshape(g.wall_shape)
+    dE = np.sum(de * g.V) + np.sum(m.wall.C * dTw)
+    netH = (sum(acc[ai[f"H_in_{v}"]] - acc[ai[f"H_out_{v}"]] for v in ("V1", "V2", "V3"))
+            + acc[ai["Q_jac_pos"]] + acc[ai["Q_jac_neg"]] + acc[ai["Q_amb"]] + acc[ai["Q_hx_pos"]]
+            + acc[ai["Q_hx_neg"]])
+    assert abs(dE - netH) <= 1e-9 * max(1.0, abs(netH))
+
+
+def test_inventories_and_equilibrium():
+    z = golden("rhs_cfg1.npz")
+    st = state_from(z)
+    cfg, m = gpu_model(CFG1, 0, st)
+    cfg, om = oracle_model(CFG1, 0, st)
+    inv, inv_ref = m.inventories(z["y"]), om.inventories(z["y"])
+    for k, v in inv_ref.items():
+        assert abs(inv[k] - v) <= 1e-13 * max(abs(v), 1e-30), k
+    z2 = golden("closures.npz")
+    qc, qw = m.equilibrium(z2["grid_p"], z2["grid_p"] * 1e-2, z2["grid_T"], z2["grid_qw"])
+    np.testing.assert_allclose(qc, z2["qco2_wadst"], rtol=1e-13)
+    np.testing.assert_allclose(qw, z2["qh2o"], rtol=1e-13)
+
+
+# ------------------------------------------------------------------ Jacobian
+@pytest.mark.parametrize("name,over", [("jac_small.npz", SMALL), ("jac_cfg1.npz", CFG1)])
+def test_fd_jacobian_matches_golden(name, over):
+    """FDJacobian.__call__ (core/jacobian.py:71-86) at the FD noise floor."""
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    z = golden(name)
+    cfg, m = gpu_model(over, 0, state_from(z))
+    jac = FDJacobian(m)
+    assert np.array_equal(jac.indptr, z["indptr"]) and np.array_equal(jac.indices, z["indices"])
+    J = jac(5.0, z["y"])
+    ref = z["data"]
+    colmax = np.maximum.reduceat(np.abs(ref), z["indptr"][:-1])
+    colmax = np.repeat(colmax, np.diff(z["indptr"]))
+    err = np.abs(J.data - ref) / np.maximum(colmax, 1e-300)
+    assert err.max() < 1e-6, err.max()
+    Jt = jac.theta_block(J)
+    assert np.array_equal(Jt.indptr, z["theta_indptr"])
+
+
+def test_fd_jacobian_bit_exact_to_device_columns():
+    """Batched all-columns FD == (f(y + h e_c) - f(y)) / h evaluated column by column on the device."""
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    z = golden("jac_small.npz")
+    cfg, m = gpu_model(SMALL, 0, state_from(z))
+    jac = FDJacobian(m)
+    y = z["y"]
+    J = jac(5.0, y).toarray()
+    f0, _ = m.rhs(5.0, y, want_acc=False)
+    h = 1.5e-8 * np.maximum(np.abs(y), jac.typ)
+    h = (y + h) - y
+    for j in np.random.default_rng(2).choice(len(y), 24, replace=False):
+        yp = y.copy()
+        yp[j] += h[j]
+        col = (m.rhs(5.0, yp, want_acc=False)[0] - f0) / h[j]
+        rows = np.nonzero(jac.P[:, j].toarray().ravel())[0]
+        assert np.array_equal(col[rows], J[rows, j]), j
+        assert np.all(col[np.setdiff1d(np.arange(len(y)), rows)] == 0.0)
+
+
+# ------------------------------------------------------------------ preconditioner, GMRES, Newton
+def _dense_zline(jac, m, dh, s0):
+    """Dense M~^-1 of the z-line block-pentadiagonal part (host reference)."""
+    g = m.grid
+    n, Nz = g.ncell, g.Nz
+    Jc, Jcw, Jw, Jwc = jac.blocks()
+    Jc = Jc.reshape(n, 9, 6, 6)
+    N = m.n_core
+    M = np.zeros((N, N))
+    for X in range(n):
+        k = X % Nz
+        for slot, dk in ((0, 0), (1, -1), (2, 1), (3, -2), (4, 2)):
+            if 0 <= k + dk < Nz:
+                A = X + dk
+                B = (np.eye(6) if slot == 0 else 0.0) - dh * Jc[X, slot]
+                for r in range(6):
+                    for c in range(6):
+                        M[r * n + X, c * n + A] = B[r, c]
+    for w in range(m.n_w):
+        M[6 * n + w, 6 * n + w] = 1.0 - dh * Jw.reshape(-1, 5)[w, 0]
+    return M
+
+
+@pytest.mark.parametrize("kind,prec,tol", [("zline", 0, 1e-9), ("block_jacobi", 0, 1e-9), ("zline", 1, 2e-4)])
+def test_preconditioner_matches_dense_solve(kind, prec, tol):
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    from dac_vtsa_amd.core.solver import DeviceOps, PRECONDITIONERS
+    z = golden("jac_small.npz")
+    cfg, m = gpu_model(SMALL, 0, state_from(z))
+    jac = FDJacobian(m)
+    y = z["y"]
+    jac(5.0, y)
+    ops = DeviceOps(m)
+    dh = 0.3
+    yd = dv.to_dev(y)
+    s0 = ops.scale(yd, 1e-4)
+    ops.precond_setup(dh, s0, PRECONDITIONERS[kind], prec)
+    M = _dense_zline(jac, m, dh, dv.to_host(s0))
+    if kind == "block_jacobi":
+        n, Nz = m.grid.ncell, m.grid.Nz
+        idx = np.arange(m.n_core)
+        cell = np.where(idx < 6 * n, idx % n, -1 - idx)
+        M = np.where(cell[:, None] == cell[None, :], M, 0.0)
+    r = np.random.default_rng(0).standard_normal(m.n_core) * np.abs(y)
+    out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
+    ref = np.linalg.solve(M, r)
+    sc = dv.to_host(s0)
+    assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
+
+
+def test_gmres_solves_newton_system():
+    """Device GMRES (scipy semantics) drives |b - A P^-1 w| below its tolerance; the solution
+    agrees with a dense solve of the linearised system."""
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    from dac_vtsa_amd.core.solver import DeviceOps
+    z = golden("jac_small.npz")
+    cfg, m = gpu_model(SMALL, 0, state_from(z))
+    jac = FDJacobian(m)
+    y = z["y"]
+    Jd = jac(5.0, y).toarray()
+    ops = DeviceOps(m)
+    dh = 0.05
+    yd = dv.to_dev(y)
+    s = ops.scale(yd, 1e-4)
+    ops.precond_setup(dh, s, 1, 0)
+    f = dv.empty(m.n_core)
+    m.rhs_dev(5.0, yd, f)
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(m.n_core)
+    w = dv.empty(m.n_core)
+    info = ops.gmres(5.0, yd, f, s, dh, 1.0, dv.to_dev(b), w, 1e-10, 0.0, 30)
+    assert info.info == 0 and info.iters >= 1, (info.info, info.iters, info.rnorm, info.atol_eff)
+    ws = dv.empty(m.n_core)
+    ops.vec(9, ws, x=w, y=s)
+    dz = dv.to_host(ops.prec_apply(ws, dv.empty(m.n_core)))
+    sh = dv.to_host(s)
+    A = (np.eye(m.n_core) - dh * Jd) * (sh[None, :] / sh[:, None])
+    ref = np.linalg.solve(A, b) * sh
+    assert np.max(np.abs(dz - ref) / sh) / np.max(np.abs(ref) / sh) < 1e-5
+
+
+def test_newton_stage_matches_oracle():
+    """One SDIRK2 stage solved to a tight Newton tolerance on both sides."""
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    from dac_vtsa_amd.core.solver import DeviceOps, newton_krylov
+    import oracle.solver as osv
+    from oracle.jacobian import OracleFDJacobian
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    z = golden("rhs_cfg1.npz")
+    over = {**CFG1, "solver.newton_kappa": 1e-4}
+    st = state_from(z)
+    cfg, m = gpu_model(over, 0, st)
+    cfg, om = oracle_model(over, 0, st)
+    y = z["y"]
+    h = 0.5
+    dh = osv.D * h
+    t = 0.0 + osv.G_SD * h
+    jac = FDJacobian(m)
+    ops = DeviceOps(m)
+    yd = dv.to_dev(y)
+    f0 = dv.empty(m.n_core)
+    m.rhs_dev(0.0, yd, f0)
+    jac.evaluate(0.0, yd, f0)
+    ops.precond_setup(dh, ops.scale(yd, 1e-4), 1, 0)
+    zg, ok, it, rate, _ = newton_krylov(ops, t, yd, yd, dh, None, 1e-4, 7, 1e-4, cons_tol=1e-3)
+    assert ok
+    oj = OracleFDJacobian(om)
+    J = oj(0.0, y)
+    lu = spla.splu((sp.identity(len(y), format="csc") - dh * oj.theta_block(J)).tocsc(), permc_spec="COLAMD")
+    zo, ok2, it2, rate2, _ = osv.newton_krylov(om, t, y, y, dh, lu, lambda v: om.atol + 1e-4 * np.abs(v), 7, 1e-4,
+                                               pos=np.arange(om.off_e), cons=om.conservation_matrix(), cons_tol=1e-3)
+    assert ok2
+    sc = om.atol + 1e-4 * np.abs(zo)
+    assert np.sqrt(np.mean(((dv.to_host(zg) - zo) / sc) ** 2)) < 1e-3
+
+
+# ------------------------------------------------------------------ integration
+def test_integrate_matches_golden_trajectory():
+    """20 s of adsorption from the seeded 4x4x20 state (core/solver.py:370-511): end state within the
+    controller tolerance of the upstream run, accumulators and balances consistent."""
+    from dac_vtsa_amd.core.solver import TRBDF2
+    z = golden("integ_cfg1.npz")
+    st = state_from(z)
+    cfg, m = gpu_model(CFG1, 0, st)
+    y0 = m.pack(st)
+    np.testing.assert_allclose(y0, z["sdirk2/y0"], rtol=1e-15)
+    res = TRBDF2(m, cfg, h_max=4.0).integrate(y0, 20.0, np.linspace(0, 20.0, 5))
+    sc = m.atol + 1e-4 * np.abs(z["sdirk2/y"])
+    e = (res.y - z["sdirk2/y"]) / sc
+    assert np.sqrt(np.mean(e * e)) < 1.0, np.sqrt(np.mean(e * e))
+    assert res.n_steps > 0 and np.all(np.isfinite(res.y))
+    np.testing.assert_allclose(res.acc, z["sdirk2/acc"], rtol=1e-3, atol=1e-6 * np.abs(z["sdirk2/acc"]).max())
+    inv0, inv1 = m.inventories(y0), m.inventories(res.y)
+    ai = m.acc_index
+    for s in m.species:
+        net = sum(res.acc[ai[f"n_in_{v}_{s}"]] - res.acc[ai[f"n_out_{v}_{s}"]] for v in ("V1", "V2", "V3"))
+        d = inv1[f"tot_{s}"] - inv0[f"tot_{s}"]
+        assert abs(d - net) <= 1e-5 * max(abs(inv0[f"tot_{s}"]), 1e-30), s
+
+
+@pytest.mark.parametrize("scheme", ["sdirk2", "trbdf2"])
+def test_integrate_tight_tolerance_matches_oracle(scheme):
+    """Both integrators at rtol 1e-7 approximate the same ODE solution: step-end fields agree to
+    1e-5 relative (well below either controller's tolerance at default settings)."""
+    from dac_vtsa_amd.core.solver import TRBDF2
+    from oracle.solver import OracleTRBDF2
+    z = golden("rhs_cfg1.npz")
+    over = {**CFG1, "solver.rtol": 1e-7, "solver.scheme": scheme}
+    st = state_from(z)
+    cfg, m = gpu_model(over, 0, st)
+    cfg, om = oracle_model(over, 0, st)
+    y0 = m.pack(st)
+    t_end = 2.0
+    rg = TRBDF2(m, cfg, h_max=0.1).integrate(y0, t_end, np.array([t_end]))
+    ro = OracleTRBDF2(om, cfg, h_max=0.1).integrate(y0, t_end, np.array([t_end]))
+    sc = np.abs(ro.y) + om.atol
+    assert np.max(np.abs(rg.y - ro.y) / sc) < 1e-4
+    assert np.sqrt(np.mean(((rg.y - ro.y) / sc) ** 2)) < 1e-5
