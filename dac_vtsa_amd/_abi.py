@@ -84,6 +84,15 @@ _SIGS = {
     "vtsa_inventories": [P, P, P, P],
     "vtsa_vec_op": [P, I32, D, D, D, D, P, P, P, P, P, P],
     "vtsa_error_norm": [P, P, P, P, D, C.POINTER(D), P],
+    "vtsa_device_count": [C.POINTER(I32)],
+    "vtsa_device_alloc": [C.c_size_t, C.POINTER(P)],
+    "vtsa_device_free": [P],
+    "vtsa_copy_h2d": [P, P, C.c_size_t, P],
+    "vtsa_copy_d2h": [P, P, C.c_size_t, P],
+    "vtsa_copy_d2d": [P, P, C.c_size_t, P],
+    "vtsa_fill": [P, D, C.c_size_t, P],
+    "vtsa_sync": [P],
+    "vtsa_time_rhs": [P, D, P, P, I32, C.POINTER(D), P],
     "vtsa_version": [],
     "vtsa_last_error": [],
 }

@@ -237,11 +237,25 @@ class TVSAModel:
             d.atol[v] = self.atol_var[v]
         return d
 
-    def _create(self):
+    def _device_arrays(self):
+        """Host arrays of vtsa_create, broadcast to the sizes the kernels index."""
         g = self.grid
-        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in
-                (g.V, g.Az, g.Ath, g.arc, g.Ar if g.Nr > 1 else np.zeros(1), g.drc if g.Nr > 1 else np.zeros(1),
-                 self.feed_flux, self.N_nom, self.cov)]
+        Nr, Nth, Nz = g.Nr, g.Nth, g.Nz
+        specs = [(g.V, (Nr,)), (g.Az, (Nr,)), (g.Ath, (Nr,)), (g.arc, (Nr,)),
+                 (g.Ar if Nr > 1 else np.zeros(1), (max(Nr - 1, 1),)),
+                 (g.drc if Nr > 1 else np.zeros(1), (max(Nr - 1, 1),)),
+                 (np.broadcast_to(self.feed_flux, (Nr, Nth)), (Nr, Nth)),
+                 (np.broadcast_to(self.N_nom.reshape(self.N_nom.shape[:2]), (Nr, Nth)), (Nr, Nth)),
+                 (np.broadcast_to(self.cov, (Nth, Nz)), (Nth, Nz))]
+        out = []
+        for a, shp in specs:
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shp)).ravel()
+            assert a.size == int(np.prod(shp))
+            out.append(a)
+        return out
+
+    def _create(self):
+        arrs = self._device_arrays()
         self._host_arrays = arrs
         h = C.c_void_p()
         ptrs = [a.ctypes.data_as(C.c_void_p) for a in arrs]
@@ -341,7 +355,7 @@ class TVSAModel:
         """Return (f, acc) -- or the diagnostics dict if diag=True (core/residual.py:211-417)."""
         if diag:
             return self.diag(t, y)
-        self._y.copy_(dv.torch().from_numpy(np.ascontiguousarray(y, dtype=np.float64)))
+        self._y.upload(y)
         self.rhs_dev(t, self._y, self._f, self._acc if want_acc else None)
         f = dv.to_host(self._f)
         return f, (dv.to_host(self._acc).copy() if want_acc else None)

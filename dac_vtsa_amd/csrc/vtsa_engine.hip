@@ -298,4 +298,67 @@ int vtsa_error_norm(vtsa_handle* H, const double* y, const double* z2, const dou
   return error_norm_impl(H->h, y, z2, y3, rtol, en, S(st));
 }
 
+// ---------------------------------------------------------------- plumbing
+__global__ void k_fill(double* p, double v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int vtsa_device_count(int32_t* count) {
+  int c = 0;
+  if (check(hipGetDeviceCount(&c), "hipGetDeviceCount")) { *count = 0; return -1; }
+  *count = c;
+  return 0;
+}
+
+int vtsa_device_alloc(size_t bytes, void** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  return check(hipMalloc(out, bytes ? bytes : 8), "hipMalloc");
+}
+
+int vtsa_device_free(void* p) { return p ? check(hipFree(p), "hipFree") : 0; }
+
+int vtsa_copy_h2d(void* dst, const void* src, size_t bytes, vtsa_stream st) {
+  if (check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(st)), "hipMemcpyAsync")) return -1;
+  return check(hipStreamSynchronize(S(st)), "hipStreamSynchronize");
+}
+
+int vtsa_copy_d2h(void* dst, const void* src, size_t bytes, vtsa_stream st) {
+  if (check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(st)), "hipMemcpyAsync")) return -1;
+  return check(hipStreamSynchronize(S(st)), "hipStreamSynchronize");
+}
+
+int vtsa_copy_d2d(void* dst, const void* src, size_t bytes, vtsa_stream st) {
+  return check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(st)), "hipMemcpyAsync");
+}
+
+int vtsa_fill(double* dst, double value, size_t count, vtsa_stream st) {
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, S(st), dst, value, count);
+  return finish(S(st));
+}
+
+int vtsa_sync(vtsa_stream st) { return check(hipStreamSynchronize(S(st)), "hipStreamSynchronize"); }
+
+int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st) {
+  if (!H || !y || !f || !ms || repeat < 1) return -1;
+  Handle& h = H->h;
+  hipEvent_t e0, e1;
+  if (check(hipEventCreate(&e0), "hipEventCreate") || check(hipEventCreate(&e1), "hipEventCreate")) return -1;
+  (void)hipEventRecord(e0, S(st));
+  for (int r = 0; r < repeat; ++r) {
+    launch_props(h, y, h.pb, f, nullptr, S(st));
+    launch_flux(h, t, y, h.pb, f, S(st));
+  }
+  (void)hipEventRecord(e1, S(st));
+  int rc = check(hipEventSynchronize(e1), "hipEventSynchronize");
+  float m = 0.f;
+  if (!rc) rc = check(hipEventElapsedTime(&m, e0, e1), "hipEventElapsedTime");
+  *ms = m;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc ? rc : finish(S(st));
+}
+
 }  // extern "C"

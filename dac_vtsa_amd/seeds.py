@@ -5,8 +5,13 @@ order T -> P -> y_CO2 -> y_H2O -> loading fractions (or loadings).  Workloads:
 
 * ``small``  : T ~ U(288, 293) K, P = 1.013e5 Pa, y_CO2 = 4e-4, y_H2O ~ U(0.005, 0.015),
                y_N2 the rest; q_j = U(0, 0.2) x local equilibrium (state-basis water for CO2).
-* ``midcycle``: T ~ U(288, 373) K, P ~ U(1.5e4, 1.013e5) Pa, y_CO2 ~ U(0, 0.9),
-               y_H2O ~ U(0, 0.05), y_N2 the rest; q_CO2 ~ U(0, 1.6), q_H2O ~ U(0, 6) mol/kg.
+* ``midcycle``: a spatially uniform bed whose values are drawn once per seed:
+               T ~ U(288, 373) K, P ~ U(1.5e4, 1.013e5) Pa, y_CO2 ~ U(0, 0.9), y_H2O ~ U(0, 0.05),
+               y_N2 the rest, q_CO2 ~ U(0, 1.6), q_H2O ~ U(0, 6) mol/kg.
+* ``midcycle_cells``: the same distributions drawn independently per cell.  The upstream Newton
+               controller cannot start from such a field (every stage fails at t = 0 down to
+               step-size underflow, with or without per-cell pressures), so it serves residual and
+               Jacobian parity tests only.
 The wall starts at the temperature of the outer bed ring.
 
 The equilibrium loading is supplied by the caller (``qstar_fn(p_co2, p_h2o, T, q_h2o)`` returning
@@ -27,11 +32,13 @@ def seeded_state(grid, kind: str, seed: int, qstar_fn=None) -> CycleState:
         P = np.full(shp, 1.013e5)
         y_co2 = np.full(shp, 4e-4)
         y_h2o = rng.uniform(0.005, 0.015, shp)
-    elif kind == "midcycle":
-        T = rng.uniform(288.0, 373.0, shp)
-        P = rng.uniform(1.5e4, 1.013e5, shp)
-        y_co2 = rng.uniform(0.0, 0.9, shp)
-        y_h2o = rng.uniform(0.0, 0.05, shp)
+    elif kind in ("midcycle", "midcycle_cells"):
+        draw = (lambda lo, hi: rng.uniform(lo, hi, shp)) if kind == "midcycle_cells" else \
+            (lambda lo, hi: np.full(shp, rng.uniform(lo, hi)))
+        T = draw(288.0, 373.0)
+        P = draw(1.5e4, 1.013e5)
+        y_co2 = draw(0.0, 0.9)
+        y_h2o = draw(0.0, 0.05)
     else:
         raise ValueError(kind)
     y_n2 = 1.0 - y_co2 - y_h2o
@@ -47,8 +54,8 @@ def seeded_state(grid, kind: str, seed: int, qstar_fn=None) -> CycleState:
         qc_eq, _ = qstar_fn(c[0] * R_GAS * T, c[1] * R_GAS * T, T, q_w)
         q_c = f_c * qc_eq
     else:
-        q_c = rng.uniform(0.0, 1.6, shp)
-        q_w = rng.uniform(0.0, 6.0, shp)
+        q_c = draw(0.0, 1.6)
+        q_w = draw(0.0, 6.0)
     q = np.stack([q_c, q_w])
     Tw = T[-1].copy()
     return CycleState(c=c, q=q, T=T, Tw=Tw, t=0.0, cycle=1, step="seeded")

@@ -53,7 +53,7 @@ def _end_summary(model, d, end, c_feed_co2):
 
 def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle0: float = 0.0,
              T_jacket_prev: Optional[float] = None, logger=None, snapshots: Optional[int] = None,
-             record_outlet: bool = True, integrator_kw: Optional[Dict] = None):
+             record_outlet: bool = True, integrator_kw: Optional[Dict] = None, progress_s: float = 30.0):
     """One cycle step on the GPU -> (state_out, StepResult) (steps/_runner.py:87-188).
 
     ``record_outlet`` and ``integrator_kw`` (preconditioner kind/precision) have no upstream
@@ -97,7 +97,7 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
         counter[0] += 1
         if record_outlet and counter[0] % every == 0:
             record(t, y, acc)
-        if time.time() - last_print[0] > 30.0:
+        if time.time() - last_print[0] > progress_s:
             last_print[0] = time.time()
             log(f"      ... {step['name']}: t = {t:.0f}/{dur:.0f} s ({100 * t / dur:.0f} %), {counter[0]} steps")
 

@@ -170,6 +170,20 @@ int vtsa_vec_op(vtsa_handle* h, int32_t op, double a, double b, double c, double
 int vtsa_error_norm(vtsa_handle* h, const double* y, const double* z2, const double* y3, double rtol, double* en,
                     vtsa_stream st);
 
+/* ---- Device memory, copies, streams and timing (plumbing for callers without a HIP runtime
+ * of their own; no upstream counterpart).  Copies to the host are synchronous. */
+int vtsa_device_count(int32_t* count);
+int vtsa_device_alloc(size_t bytes, void** out);
+int vtsa_device_free(void* ptr);
+int vtsa_copy_h2d(void* dst, const void* src, size_t bytes, vtsa_stream st);
+int vtsa_copy_d2h(void* dst, const void* src, size_t bytes, vtsa_stream st);
+int vtsa_copy_d2d(void* dst, const void* src, size_t bytes, vtsa_stream st);
+int vtsa_fill(double* dst, double value, size_t count, vtsa_stream st);
+int vtsa_sync(vtsa_stream st);
+/* Elapsed milliseconds of `repeat` back-to-back residual evaluations f = f(t, y) on stream st,
+ * measured with HIP events recorded on that stream (kernel timing for the roofline). */
+int vtsa_time_rhs(vtsa_handle* h, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st);
+
 /* Library version string and last error message (host memory owned by the library). */
 const char* vtsa_version(void);
 const char* vtsa_last_error(void);

@@ -119,7 +119,7 @@ def make_rhs_cfg1(path):
 
 def make_rhs_mid(path):
     cfg = load_config(override_dict={"grid.Nr": 4, "grid.Ntheta": 4, "grid.Nz": 10, "operation.P_vacuum": 1.5e4})
-    g, st = _seeded(cfg, "midcycle", 0)
+    g, st = _seeded(cfg, "midcycle_cells", 0)
     props = get_backend(cfg)
     out = {}
     _state_arrays("", st, out)

@@ -216,36 +216,60 @@ def test_preconditioner_matches_dense_solve(kind, prec, tol):
     assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
 
 
-def test_gmres_solves_newton_system():
-    """Device GMRES (scipy semantics) drives |b - A P^-1 w| below its tolerance; the solution
-    agrees with a dense solve of the linearised system."""
+@pytest.mark.parametrize("dh,rtol", [(0.05, 1e-2), (0.05, 1e-4), (0.3, 1e-2)])
+def test_gmres_matches_scipy_on_newton_system(dh, rtol):
+    """Device GMRES(30) against scipy.sparse.linalg.gmres (the upstream solver, core/solver.py:206-232)
+    on the same Newton system: the directional FD matvec of the oracle residual, right-preconditioned
+    by a dense solve with the device's z-line matrix.  The FD operator is only linear to ~1e-2 at
+    this state (limiter and upwind switches), so both solvers report info = 1 on some cases; they must
+    agree on info, the inner iteration count and the solution."""
+    import scipy.linalg as sl
+    import scipy.sparse.linalg as spla
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.core.jacobian import FDJacobian
     from dac_vtsa_amd.core.solver import DeviceOps
-    z = golden("jac_small.npz")
-    cfg, m = gpu_model(SMALL, 0, state_from(z))
+    z = golden("integ_cfg1.npz")
+    st = state_from(z)
+    y = z["sdirk2/snap_y"][2]
+    t = float(z["sdirk2/snap_t"][2])
+    cfg, m = gpu_model(CFG1, 0, st)
+    cfg, om = oracle_model(CFG1, 0, st)
     jac = FDJacobian(m)
-    y = z["y"]
-    Jd = jac(5.0, y).toarray()
     ops = DeviceOps(m)
-    dh = 0.05
     yd = dv.to_dev(y)
+    f = dv.empty(m.n_core)
+    m.rhs_dev(t, yd, f)
+    jac.evaluate(t, yd, f)
     s = ops.scale(yd, 1e-4)
     ops.precond_setup(dh, s, 1, 0)
-    f = dv.empty(m.n_core)
-    m.rhs_dev(5.0, yd, f)
-    rng = np.random.default_rng(1)
-    b = rng.standard_normal(m.n_core)
-    w = dv.empty(m.n_core)
-    info = ops.gmres(5.0, yd, f, s, dh, 1.0, dv.to_dev(b), w, 1e-10, 0.0, 30)
-    assert info.info == 0 and info.iters >= 1, (info.info, info.iters, info.rnorm, info.atol_eff)
-    ws = dv.empty(m.n_core)
-    ops.vec(9, ws, x=w, y=s)
-    dz = dv.to_host(ops.prec_apply(ws, dv.empty(m.n_core)))
     sh = dv.to_host(s)
-    A = (np.eye(m.n_core) - dh * Jd) * (sh[None, :] / sh[:, None])
-    ref = np.linalg.solve(A, b) * sh
-    assert np.max(np.abs(dz - ref) / sh) / np.max(np.abs(ref) / sh) < 1e-5
+    fh = dv.to_host(f)
+    zs_norm = max(1.0, ops.rms(yd, s))
+    b = dh * fh / sh                               # -R/s at z = y with rhs_known = y
+    floor = 0.1 * 1e-2 * np.sqrt(m.n_core)
+    w = dv.empty(m.n_core)
+    info = ops.gmres(t, yd, f, s, dh, zs_norm, dv.to_dev(b), w, rtol, floor, 30)
+
+    lu = sl.lu_factor(_dense_zline(jac, m, dh, sh))
+    fo, _ = om.rhs(t, y, want_acc=False)
+
+    def matvec(u):
+        un = np.sqrt(np.mean(u * u))
+        sig = np.sqrt(np.finfo(float).eps) * zs_norm / un
+        v = u * sh
+        fp, _ = om.rhs(t, y + sig * v, want_acc=False)
+        return (v - dh * (fp - fo) / sig) / sh
+
+    AP = spla.LinearOperator((m.n_core, m.n_core), dtype=float,
+                             matvec=lambda u: matvec(sl.lu_solve(lu, u * sh) / sh))
+    hist = []
+    wo, info_o = spla.gmres(AP, b, rtol=rtol, atol=floor, restart=30, maxiter=1,
+                            callback=hist.append, callback_type="pr_norm")
+    assert (info.info, info.iters) == (info_o, len(hist)), (info.info, info.iters, info_o, len(hist))
+    assert info.bnorm == pytest.approx(np.linalg.norm(b), rel=1e-12)
+    assert info.atol_eff == pytest.approx(max(floor, rtol * np.linalg.norm(b)), rel=1e-12)
+    wd = dv.to_host(w)
+    assert np.max(np.abs(wd - wo)) / np.max(np.abs(wo)) < 1e-5
 
 
 def test_newton_stage_matches_oracle():
