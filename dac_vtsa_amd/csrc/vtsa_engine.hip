@@ -43,6 +43,7 @@ static void free_handle(Handle* h) {
   if (h->red.counter) (void)hipFree(h->red.counter);
   if (h->gs_host) (void)hipHostFree(h->gs_host);
   if (h->host_scalars) (void)hipHostFree(h->host_scalars);
+  if (h->progress) (void)hipHostFree(h->progress);
   delete h;
 }
 
@@ -83,6 +84,7 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   if (!desc || !out) { set_error("vtsa_create: null argument"); return -1; }
   const int Nr = desc->Nr, Nth = desc->Nth, Nz = desc->Nz;
   if (Nr < 1 || Nth < 1 || Nz < 2 || Nth == 2) { set_error("vtsa_create: invalid grid"); return -2; }
+  if (apply_columns_per_block(Nz) < 1) { set_error("vtsa_create: Nz too large for the z-line preconditioner (LDS)"); return -2; }
   vtsa_handle* H = new vtsa_handle();
   Handle& h = H->h;
   h.desc = *desc;
@@ -118,6 +120,8 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   rc |= dalloc(&h.gs, 1);
   rc |= check(hipHostMalloc((void**)&h.gs_host, sizeof(GmresScalars)), "hipHostMalloc");
   rc |= check(hipHostMalloc((void**)&h.host_scalars, sizeof(double) * 64), "hipHostMalloc");
+  rc |= check(hipHostMalloc((void**)&h.progress, 64, hipHostMallocCoherent | hipHostMallocMapped), "hipHostMalloc");
+  if (rc == 0) rc |= check(hipHostGetDevicePointer((void**)&h.progress_dev, h.progress, 0), "hipHostGetDevicePointer");
   h.red.max_blocks = 512;
   rc |= dalloc(&h.red.partial, partial_size(h.ncol, h.nw));
   rc |= dalloc(&h.red.counter, 16);

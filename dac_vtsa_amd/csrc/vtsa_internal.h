@@ -63,6 +63,8 @@ struct Handle {
   GmresScalars* gs_host = nullptr;
   Reduce red;
   double* host_scalars = nullptr;   // pinned [64]
+  int* progress = nullptr;          // coherent pinned host word: GMRES columns completed (< 0: done)
+  int* progress_dev = nullptr;      // its device address
 };
 
 void set_error(const std::string& msg);
@@ -107,5 +109,6 @@ int error_norm_impl(Handle& h, const double* y, const double* z2, const double* 
 void vec_op_impl(Handle& h, int op, int npos, double a, double b, double c, double e, const double* x,
                  const double* y, const double* z, const double* w, double* out, hipStream_t st);
 size_t partial_size(int ncol, int nw);
+int apply_columns_per_block(int Nz);
 
 }  // namespace vtsa

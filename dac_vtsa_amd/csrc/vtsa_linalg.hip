@@ -8,12 +8,15 @@
 //    are left to GMRES.
 //  * block Jacobi: D_k^-1 of the diagonal blocks only.
 //  * wall rows: point Jacobi on the wall diagonal.
-// The apply kernel gives each column six lanes (one per variable) of a 64-lane wavefront
-// (10 columns per wavefront); the sweep along z exchanges vector components with wavefront
-// shuffles, so a column's forward and backward substitution needs no LDS or barriers.
+// The apply kernel stages the right-hand side of up to ten columns in LDS (coalesced loads and
+// stores); six lanes per column (one per variable) sweep along z through LDS, with the factor
+// rows of the next cell loaded while the current one is processed.
 //
 // Reductions use one launch: per-block partial sums, then the last block to finish (ticket
 // counter) adds the partials in block order, so results are deterministic.
+#include <chrono>
+#include <thread>
+
 #include "vtsa_internal.h"
 
 namespace vtsa {
@@ -22,20 +25,35 @@ constexpr int RBLK = 256;        // reduction block size
 constexpr int RMAXB = 512;       // max blocks of a reduction grid
 constexpr int PW = 64;           // preconditioner apply: threads per block (one wavefront)
 constexpr int PCOLS = 10;        // columns per wavefront (6 lanes each)
+constexpr int AC = 4;            // apply gather/scatter: elements per lane whose loads go together
 constexpr double EPS64 = 2.220446049250313e-16;
 
 // ------------------------------------------------------------------ reduction helpers
+// Butterfly sum over the 64 lanes of a wavefront.  a + b == b + a exactly, so every lane ends
+// with the same bits, and the order is fixed, so the result is deterministic.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum: wavefront sums, then the per-wave values in wave order.  sh needs one slot per wave.
 __device__ inline double wave_block_sum(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
+  const int nwv = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (t < off) sh[t] += sh[t + off];
-    __syncthreads();
-  }
-  double r = sh[0];
+  double r = 0.0;
+  for (int w = 0; w < nwv; ++w) r += sh[w];
   __syncthreads();
   return r;
+}
+
+// Sum of partial[0 .. cnt) by one wavefront (lane-strided, then butterfly): deterministic.
+__device__ inline double wave_sum_strided(const double* __restrict__ partial, int cnt) {
+  double acc = 0.0;
+  for (int b = threadIdx.x & 63; b < cnt; b += 64) acc += partial[b];
+  return wave_sum(acc);
 }
 
 // Returns true in the last block to finish; that block may then read every block's partials.
@@ -54,6 +72,41 @@ __device__ inline void ticket_reset(unsigned* counter) {
   __hip_atomic_store(counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Last block of a one-value reduction: its first wavefront sums the per-block partials; returns
+// true, with the total in *tot, in thread 0 of the last block only (counter already reset).
+__device__ inline bool last_block_total(unsigned* counter, bool* sflag, const double* partial, double* tot) {
+  if (!ticket_last(counter, sflag)) return false;
+  if (threadIdx.x >= 64) return false;
+  const double t = wave_sum_strided(partial, gridDim.x);
+  if (threadIdx.x != 0) return false;
+  ticket_reset(counter);
+  *tot = t;
+  return true;
+}
+
+// Last block of an nval-value reduction (partial rows of stride RMAXB): its wavefronts sum one
+// row each into out[]; returns true in every thread of the last block, after a barrier.
+__device__ inline bool last_block_totals(unsigned* counter, bool* sflag, const double* partial, int nval,
+                                         double* out) {
+  if (!ticket_last(counter, sflag)) return false;
+  const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  for (int q = wv; q < nval; q += nwv) {
+    const double t = wave_sum_strided(partial + (size_t)q * RMAXB, gridDim.x);
+    if ((threadIdx.x & 63) == 0) out[q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) ticket_reset(counter);
+  return true;
+}
+
+// Barrier for one-wavefront blocks that exchange data through LDS only: orders LDS accesses
+// (lgkmcnt) without waiting for outstanding global loads, so prefetches stay in flight.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 static inline int rgrid(long long work) {
   long long b = (work + RBLK - 1) / RBLK;
   if (b < 1) b = 1;
@@ -61,114 +114,178 @@ static inline int rgrid(long long work) {
   return (int)b;
 }
 
-// ------------------------------------------------------------------ 6x6 helpers
-template <typename T>
-__device__ inline void mat_inv6(T* A, T* Ainv) {
-  // Gauss-Jordan with partial pivoting, A is destroyed.
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 6; ++c) Ainv[r * 6 + c] = (r == c) ? T(1) : T(0);
-  for (int p = 0; p < 6; ++p) {
-    int piv = p;
-    T best = fabs(A[p * 6 + p]);
-    for (int r = p + 1; r < 6; ++r) {
-      T v = fabs(A[r * 6 + p]);
-      if (v > best) { best = v; piv = r; }
-    }
-    if (piv != p) {
-      for (int c = 0; c < 6; ++c) {
-        T a = A[p * 6 + c]; A[p * 6 + c] = A[piv * 6 + c]; A[piv * 6 + c] = a;
-        T b = Ainv[p * 6 + c]; Ainv[p * 6 + c] = Ainv[piv * 6 + c]; Ainv[piv * 6 + c] = b;
-      }
-    }
-    T inv = T(1) / A[p * 6 + p];
-    for (int c = 0; c < 6; ++c) { A[p * 6 + c] *= inv; Ainv[p * 6 + c] *= inv; }
-    for (int r = 0; r < 6; ++r) {
-      if (r == p) continue;
-      T fct = A[r * 6 + p];
-      if (fct == T(0)) continue;
-      for (int c = 0; c < 6; ++c) { A[r * 6 + c] -= fct * A[p * 6 + c]; Ainv[r * 6 + c] -= fct * Ainv[p * 6 + c]; }
-    }
-  }
-}
+// ------------------------------------------------------------------ preconditioner factor
+// Block LU of the z-line matrices (block Jacobi: the diagonal inverses only).  Six lanes per
+// column, lane r computing row r of every block product, PCOLS columns per wavefront; the
+// factors of the previous two cells stay in LDS.  The pivot blocks are inverted by Gauss-Jordan
+// with partial pivoting (the first row of largest |entry| wins), rows in parallel through LDS.
+// Block products accumulate over m in order in T, as C = A B and C -= A B of 6x6 matrices.
+// F blocks per cell, rows of 6: 0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2.
+constexpr int FSTR = 258;   // LDS doubles per column: Dinv/U1/U2 ring of two cells + pivot scratch
 
-// C = A * B
 template <typename T>
-__device__ inline void mat_mul6(const T* A, const T* B, T* C) {
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 6; ++c) {
-      T acc = T(0);
-      for (int m = 0; m < 6; ++m) acc += A[r * 6 + m] * B[m * 6 + c];
-      C[r * 6 + c] = acc;
-    }
-}
-// C -= A * B
-template <typename T>
-__device__ inline void mat_submul6(const T* A, const T* B, T* C) {
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 6; ++c) {
-      T acc = T(0);
-      for (int m = 0; m < 6; ++m) acc += A[r * 6 + m] * B[m * 6 + c];
-      C[r * 6 + c] -= acc;
-    }
-}
-
-// Scaled block of M~ = S0^-1 (I - dh J) S0 between row cell X and column cell A (slot).
-template <typename T>
-__device__ inline void load_mblock(const double* __restrict__ Jc, const double* __restrict__ s0, int n, int X, int A,
-                                   int slot, double dh, T* B) {
-  const double* J = Jc + ((size_t)X * NSLOT + slot) * 36;
-  for (int r = 0; r < 6; ++r) {
-    double sr = s0[r * n + X];
-    for (int c = 0; c < 6; ++c) {
-      double v = (slot == S_SELF && r == c ? 1.0 : 0.0) - dh * J[r * 6 + c];
-      B[r * 6 + c] = (T)(v * (s0[c * n + A] / sr));
-    }
-  }
-}
-
-// One thread per (r, theta) column: block LU along z.  F blocks per cell: 0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2.
-template <typename T>
-__global__ void k_prec_factor(int n, int Nz, int ncol, int kind, const double* __restrict__ Jc,
-                              const double* __restrict__ s0, double dh, T* __restrict__ F) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  T A[36], B[36], D[36];
+__global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int kind,
+                                                    const double* __restrict__ Jc, const double* __restrict__ s0,
+                                                    double dh, T* __restrict__ F) {
+  __shared__ double lds[PCOLS * FSTR];
+  const int lane = threadIdx.x;
+  const int grp = lane / 6, r = lane - grp * 6;
+  const int col0 = blockIdx.x * PCOLS;
+  const int nc = min(PCOLS, ncol - col0);
+  const bool active = grp < nc;
+  const int col = col0 + (active ? grp : 0);
+  double* Lc = lds + (active ? grp : 0) * FSTR;
+  double* pDi = Lc;          // [2][36] Dinv of cells k-2 / k-1 (ring slot k & 1)
+  double* pU1 = Lc + 72;     // [2][36]
+  double* pU2 = Lc + 144;    // [2][36]
+  double* colp = Lc + 216;   // [6]  |pivot column|
+  double* sw = Lc + 222;     // [24] row swap
+  double* prow = Lc + 246;   // [12] normalised pivot row of D and of its inverse
+  const bool zl = kind == VTSA_PREC_ZLINE;
   for (int k = 0; k < Nz; ++k) {
     const int X = col * Nz + k;
-    T* FX = F + (size_t)X * 5 * 36;
-    if (kind == VTSA_PREC_BLOCK_JACOBI) {
-      load_mblock<T>(Jc, s0, n, X, X, S_SELF, dh, D);
-      mat_inv6<T>(D, FX + 2 * 36);
-      continue;
+    const double sr = s0[r * n + X];
+    // row r of the block of M~ = S0^-1 (I - dh J) S0 between cell X and cell X + dk
+    auto mrow = [&](int slot, int dk, T* out) {
+      const double* J = Jc + ((size_t)X * NSLOT + slot) * 36 + r * 6;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double v = (slot == S_SELF && r == c ? 1.0 : 0.0) - dh * J[c];
+        out[c] = (T)(v * (s0[c * n + X + dk] / sr));
+      }
+    };
+    const int q = k & 1, q1 = (k + 1) & 1;   // ring slots of cells k-2 (and k) and k-1
+    T l2[6], l1[6], d[6], u1[6], u2[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { l2[c] = T(0); l1[c] = T(0); u1[c] = T(0); u2[c] = T(0); }
+    if (zl && k >= 2) {   // L2 = A_{k,k-2} Dinv_{k-2}
+      T a[6];
+      mrow(S_ZM2, -2, a);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += a[m] * (T)pDi[q * 36 + m * 6 + c];
+        l2[c] = acc;
+      }
     }
-    // L2 = A_{k,k-2} D_{k-2}^-1
-    if (k >= 2) {
-      load_mblock<T>(Jc, s0, n, X, X - 2, S_ZM2, dh, A);
-      mat_mul6<T>(A, F + (size_t)(X - 2) * 180 + 2 * 36, FX + 0);
+    if (zl && k >= 1) {   // L1 = (A_{k,k-1} - L2 U1_{k-2}) Dinv_{k-1}
+      T a[6];
+      mrow(S_ZM1, -1, a);
+      if (k >= 2) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          T acc = T(0);
+#pragma unroll
+          for (int m = 0; m < 6; ++m) acc += l2[m] * (T)pU1[q * 36 + m * 6 + c];
+          a[c] -= acc;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += a[m] * (T)pDi[q1 * 36 + m * 6 + c];
+        l1[c] = acc;
+      }
     }
-    // L1 = (A_{k,k-1} - L2 U1_{k-2}) D_{k-1}^-1
-    if (k >= 1) {
-      load_mblock<T>(Jc, s0, n, X, X - 1, S_ZM1, dh, A);
-      if (k >= 2) mat_submul6<T>(FX + 0, F + (size_t)(X - 2) * 180 + 3 * 36, A);
-      mat_mul6<T>(A, F + (size_t)(X - 1) * 180 + 2 * 36, FX + 36);
+    mrow(S_SELF, 0, d);   // D = A_kk - L1 U1_{k-1} - L2 U2_{k-2}
+    if (zl && k >= 1) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += l1[m] * (T)pU1[q1 * 36 + m * 6 + c];
+        d[c] -= acc;
+      }
     }
-    // D = A_kk - L1 U1_{k-1} - L2 U2_{k-2}
-    load_mblock<T>(Jc, s0, n, X, X, S_SELF, dh, D);
-    if (k >= 1) mat_submul6<T>(FX + 36, F + (size_t)(X - 1) * 180 + 3 * 36, D);
-    if (k >= 2) mat_submul6<T>(FX + 0, F + (size_t)(X - 2) * 180 + 4 * 36, D);
-    for (int e = 0; e < 36; ++e) B[e] = D[e];
-    mat_inv6<T>(B, FX + 2 * 36);
-    // U1 = A_{k,k+1} - L1 U2_{k-1}
-    if (k + 1 < Nz) {
-      load_mblock<T>(Jc, s0, n, X, X + 1, S_ZP1, dh, A);
-      if (k >= 1) mat_submul6<T>(FX + 36, F + (size_t)(X - 1) * 180 + 4 * 36, A);
-      for (int e = 0; e < 36; ++e) FX[3 * 36 + e] = A[e];
+    if (zl && k >= 2) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += l2[m] * (T)pU2[q * 36 + m * 6 + c];
+        d[c] -= acc;
+      }
     }
-    // U2 = A_{k,k+2}
-    if (k + 2 < Nz) {
-      load_mblock<T>(Jc, s0, n, X, X + 2, S_ZP2, dh, A);
-      for (int e = 0; e < 36; ++e) FX[4 * 36 + e] = A[e];
+    if (zl && k + 1 < Nz) {   // U1 = A_{k,k+1} - L1 U2_{k-1}
+      mrow(S_ZP1, 1, u1);
+      if (k >= 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          T acc = T(0);
+#pragma unroll
+          for (int m = 0; m < 6; ++m) acc += l1[m] * (T)pU2[q1 * 36 + m * 6 + c];
+          u1[c] -= acc;
+        }
+      }
     }
+    if (zl && k + 2 < Nz) mrow(S_ZP2, 2, u2);   // U2 = A_{k,k+2}
+    lds_barrier();   // every lane has read ring slot q (cell k-2) before cell k overwrites it
+    T* FX = F + (size_t)X * 180 + r * 6;
+    if (active && zl) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        FX[c] = l2[c];
+        FX[36 + c] = l1[c];
+        FX[108 + c] = u1[c];
+        FX[144 + c] = u2[c];
+        pU1[q * 36 + r * 6 + c] = (double)u1[c];
+        pU2[q * 36 + r * 6 + c] = (double)u2[c];
+      }
+    }
+    // Gauss-Jordan inverse of D, lane r holding row r of D and of the inverse
+    T iv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) iv[c] = (r == c) ? T(1) : T(0);
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      if (active) colp[r] = (double)fabs(d[p]);
+      lds_barrier();
+      int piv = p;
+      double best = colp[p];
+      for (int rr = p + 1; rr < 6; ++rr) {
+        const double v = colp[rr];
+        if (v > best) { best = v; piv = rr; }
+      }
+      if (active && (r == p || r == piv)) {
+        double* dst = sw + (r == p ? 0 : 12);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { dst[c] = (double)d[c]; dst[6 + c] = (double)iv[c]; }
+      }
+      lds_barrier();
+      if (piv != p && (r == p || r == piv)) {
+        const double* src = sw + (r == p ? 12 : 0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { d[c] = (T)src[c]; iv[c] = (T)src[6 + c]; }
+      }
+      if (r == p) {
+        const T inv = T(1) / d[p];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { d[c] *= inv; iv[c] *= inv; }
+        if (active) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) { prow[c] = (double)d[c]; prow[6 + c] = (double)iv[c]; }
+        }
+      }
+      lds_barrier();
+      if (r != p) {
+        const T f = d[p];
+        if (f != T(0)) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) { d[c] -= f * (T)prow[c]; iv[c] -= f * (T)prow[6 + c]; }
+        }
+      }
+      lds_barrier();
+    }
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        FX[72 + c] = iv[c];
+        pDi[q * 36 + r * 6 + c] = (double)iv[c];
+      }
+    }
+    lds_barrier();
   }
 }
 
@@ -187,102 +304,201 @@ struct ApplyArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int nw, int kind, const T* __restrict__ F,
-                                                   const double* __restrict__ Fw, const double* __restrict__ s0,
-                                                   ApplyArgs a, double* __restrict__ partial, unsigned* counter) {
-  __shared__ double sh[PW];
+__global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int nw, int kind, int cpb,
+                                                   const T* __restrict__ F, const double* __restrict__ Fw,
+                                                   const double* __restrict__ s0, ApplyArgs a,
+                                                   double* __restrict__ partial, unsigned* counter) {
+  // LDS: xs[cpb][Nz][6] (right-hand side, then z, then x, in place) and tb[cpb][6] (exchange)
+  extern __shared__ double xs[];
+  __shared__ double sh[1];
   __shared__ bool slast;
   if (a.respect_done && a.gs->done) return;
-  const int ncb = (ncol + PCOLS - 1) / PCOLS;
+  const int ncb = (ncol + cpb - 1) / cpb;
   const double sc = a.in_scale ? *a.in_scale : 1.0;
   double local = 0.0;
   const int lane = threadIdx.x;
   if ((int)blockIdx.x < ncb) {
-    const int grp = lane / 6, r = lane % 6;
-    const int col = blockIdx.x * PCOLS + grp;
-    const bool active = (grp < PCOLS) && (col < ncol);
-    const int base = (grp < PCOLS ? grp : 0) * 6;
-    // forward: z_k = b_k - L1 z_{k-1} - L2 z_{k-2}
-    double zm1 = 0.0, zm2 = 0.0;
-    for (int k = 0; k < Nz; ++k) {
-      const int X = (active ? col : 0) * Nz + k;
-      const int idx = r * n + X;
-      double b = 0.0;
-      if (active) {
-        double x = a.in[idx] * sc;
-        if (a.store_in) a.store_in[idx] = x;
-        if (a.pre) x = x * a.pre[idx];
-        b = x / s0[idx];
-      }
-      double zk = b;
-      if (kind == VTSA_PREC_ZLINE) {
-        const T* L2 = F + (size_t)X * 180 + 0 * 36 + r * 6;
-        const T* L1 = F + (size_t)X * 180 + 1 * 36 + r * 6;
-        for (int c = 0; c < 6; ++c) {
-          double v1 = __shfl(zm1, base + c);
-          double v2 = __shfl(zm2, base + c);
-          if (active && k >= 1) zk -= (double)L1[c] * v1;
-          if (active && k >= 2) zk -= (double)L2[c] * v2;
+    const int col0 = blockIdx.x * cpb;
+    const int nc = min(cpb, ncol - col0);
+    const int X0 = col0 * Nz;          // the block's cells are X0 .. X0 + nc*Nz - 1
+    const int cnt = nc * Nz;
+    double* tb = xs + (size_t)cpb * Nz * 6;
+    // 1. b = (pre * scale * in) / s0, coalesced per variable.  store_in may alias in, so every
+    //    load of a chunk of AC elements is issued before the chunk's stores.
+    for (int r = 0; r < 6; ++r)
+      for (int e0 = lane; e0 < cnt; e0 += PW * AC) {
+        double xin[AC], xpre[AC], xs0[AC];
+#pragma unroll
+        for (int j = 0; j < AC; ++j) {
+          const int e = e0 + j * PW;
+          const int idx = r * n + X0 + (e < cnt ? e : 0);
+          xin[j] = a.in[idx];
+          xpre[j] = a.pre ? a.pre[idx] : 1.0;
+          xs0[j] = s0[idx];
+        }
+#pragma unroll
+        for (int j = 0; j < AC; ++j) {
+          const int e = e0 + j * PW;
+          if (e < cnt) {
+            double x = xin[j] * sc;
+            if (a.store_in) a.store_in[r * n + X0 + e] = x;
+            if (a.pre) x = x * xpre[j];
+            xs[e * 6 + r] = x / xs0[j];
+          }
         }
       }
-      if (active) a.out[idx] = zk;
-      zm2 = zm1;
-      zm1 = zk;
+    lds_barrier();
+    if (kind == VTSA_PREC_ZLINE) {
+      // 2. six lanes per column (lane r owns row r); factor rows of the next cell are loaded
+      //    while the current cell is processed
+      const int grp = lane / 6, r = lane - grp * 6;
+      const bool active = grp < nc;
+      const int g = active ? grp : 0;
+      double* xc = xs + (size_t)g * Nz * 6;
+      double* tg = tb + g * 6;
+      const T* Fc = F + (size_t)(X0 + g * Nz) * 180 + r * 6;
+      T c1[6], c2[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) { c1[q] = Fc[36 + q]; c2[q] = Fc[q]; }
+      for (int k = 0; k < Nz; ++k) {          // forward: z_k = b_k - L1 z_{k-1} - L2 z_{k-2}
+        T n1[6], n2[6];
+        const T* Fn = Fc + (size_t)(k + 1 < Nz ? k + 1 : k) * 180;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { n1[q] = Fn[36 + q]; n2[q] = Fn[q]; }
+        double zk = xc[k * 6 + r];
+        if (k >= 1) {
+          double s1 = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s1 += (double)c1[c] * xc[(k - 1) * 6 + c];
+          zk -= s1;
+        }
+        if (k >= 2) {
+          double s2 = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s2 += (double)c2[c] * xc[(k - 2) * 6 + c];
+          zk -= s2;
+        }
+        if (active) xc[k * 6 + r] = zk;
+        lds_barrier();
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { c1[q] = n1[q]; c2[q] = n2[q]; }
+      }
+      T u1[6], u2[6], di[6];
+      {
+        const T* Fk = Fc + (size_t)(Nz - 1) * 180;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { u1[q] = Fk[108 + q]; u2[q] = Fk[144 + q]; di[q] = Fk[72 + q]; }
+      }
+      for (int k = Nz - 1; k >= 0; --k) {     // backward: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2})
+        T m1[6], m2[6], md[6];
+        const T* Fn = Fc + (size_t)(k > 0 ? k - 1 : 0) * 180;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { m1[q] = Fn[108 + q]; m2[q] = Fn[144 + q]; md[q] = Fn[72 + q]; }
+        double tk = xc[k * 6 + r];
+        if (k + 1 < Nz) {
+          double s1 = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s1 += (double)u1[c] * xc[(k + 1) * 6 + c];
+          tk -= s1;
+        }
+        if (k + 2 < Nz) {
+          double s2 = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s2 += (double)u2[c] * xc[(k + 2) * 6 + c];
+          tk -= s2;
+        }
+        if (active) tg[r] = tk;
+        lds_barrier();
+        double xk = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) xk += (double)di[c] * tg[c];
+        if (active) xc[k * 6 + r] = xk;
+        lds_barrier();
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { u1[q] = m1[q]; u2[q] = m2[q]; di[q] = md[q]; }
+      }
+    } else {
+      // block Jacobi: every cell on its own, one lane per cell
+      for (int e = lane; e < cnt; e += PW) {
+        const T* Di = F + (size_t)(X0 + e) * 180 + 72;
+        double bv[6], xv[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) bv[c] = xs[e * 6 + c];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          double acc = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc += (double)Di[r * 6 + c] * bv[c];
+          xv[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) xs[e * 6 + r] = xv[r];
+      }
+      lds_barrier();
     }
-    // backward: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2})
-    double xp1 = 0.0, xp2 = 0.0;
-    for (int k = Nz - 1; k >= 0; --k) {
-      const int X = (active ? col : 0) * Nz + k;
-      const int idx = r * n + X;
-      double tk = active ? a.out[idx] : 0.0;
-      if (kind == VTSA_PREC_ZLINE) {
-        const T* U1 = F + (size_t)X * 180 + 3 * 36 + r * 6;
-        const T* U2 = F + (size_t)X * 180 + 4 * 36 + r * 6;
-        for (int c = 0; c < 6; ++c) {
-          double v1 = __shfl(xp1, base + c);
-          double v2 = __shfl(xp2, base + c);
-          if (active && k + 1 < Nz) tk -= (double)U1[c] * v1;
-          if (active && k + 2 < Nz) tk -= (double)U2[c] * v2;
+    // 3. out = x s0 / post, coalesced per variable, and the norm of out / red_s (loads of a
+    //    chunk before its stores, as above)
+    for (int r = 0; r < 6; ++r)
+      for (int e0 = lane; e0 < cnt; e0 += PW * AC) {
+        double xs0[AC], xpost[AC], xred[AC];
+#pragma unroll
+        for (int j = 0; j < AC; ++j) {
+          const int e = e0 + j * PW;
+          const int idx = r * n + X0 + (e < cnt ? e : 0);
+          xs0[j] = s0[idx];
+          xpost[j] = a.post ? a.post[idx] : 1.0;
+          xred[j] = a.red_s ? a.red_s[idx] : 1.0;
+        }
+#pragma unroll
+        for (int j = 0; j < AC; ++j) {
+          const int e = e0 + j * PW;
+          if (e < cnt) {
+            double o = xs[e * 6 + r] * xs0[j];
+            if (a.post) o = o / xpost[j];
+            a.out[r * n + X0 + e] = o;
+            double q = a.red_s ? o / xred[j] : o;
+            local += q * q;
+          }
         }
       }
-      const T* Di = F + (size_t)X * 180 + 2 * 36 + r * 6;
-      double xk = 0.0;
-      for (int c = 0; c < 6; ++c) {
-        double tc = __shfl(tk, base + c);
-        if (active) xk += (double)Di[c] * tc;
-      }
-      if (active) {
-        double o = xk * s0[idx];
-        if (a.post) o = o / a.post[idx];
-        a.out[idx] = o;
-        double q = a.red_s ? o / a.red_s[idx] : o;
-        local += q * q;
-      }
-      xp2 = xp1;
-      xp1 = xk;
-    }
   } else {
     // wall rows: point Jacobi
     const int wb = blockIdx.x - ncb;
     const int nwb = gridDim.x - ncb;
-    for (int w = wb * PW + lane; w < nw; w += nwb * PW) {
-      const int idx = 6 * n + w;
-      double x = a.in[idx] * sc;
-      if (a.store_in) a.store_in[idx] = x;
-      if (a.pre) x = x * a.pre[idx];
-      double o = x / Fw[w];
-      if (a.post) o = o / a.post[idx];
-      a.out[idx] = o;
-      double q = a.red_s ? o / a.red_s[idx] : o;
-      local += q * q;
+    for (int w0 = wb * PW + lane; w0 < nw; w0 += nwb * PW * AC) {
+      double xin[AC], xpre[AC], xfw[AC], xpost[AC], xred[AC];
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int w = w0 + j * nwb * PW;
+        const int wi = w < nw ? w : 0;
+        const int idx = 6 * n + wi;
+        xin[j] = a.in[idx];
+        xpre[j] = a.pre ? a.pre[idx] : 1.0;
+        xfw[j] = Fw[wi];
+        xpost[j] = a.post ? a.post[idx] : 1.0;
+        xred[j] = a.red_s ? a.red_s[idx] : 1.0;
+      }
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int w = w0 + j * nwb * PW;
+        if (w < nw) {
+          const int idx = 6 * n + w;
+          double x = xin[j] * sc;
+          if (a.store_in) a.store_in[idx] = x;
+          if (a.pre) x = x * xpre[j];
+          double o = x / xfw[j];
+          if (a.post) o = o / xpost[j];
+          a.out[idx] = o;
+          double q = a.red_s ? o / xred[j] : o;
+          local += q * q;
+        }
+      }
     }
   }
   double bs = wave_block_sum(local, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) tot += partial[b];
-    ticket_reset(counter);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
     const int N = 6 * n + nw;
     double rms = sqrt(tot / (double)N);
     if (a.red_out) *a.red_out = rms;
@@ -296,13 +512,13 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
 
 // ------------------------------------------------------------------ host launchers (preconditioner)
 void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, hipStream_t s) {
-  hipMemcpyAsync(h.s0, scale, sizeof(double) * h.N, hipMemcpyDeviceToDevice, s);
-  const int nb = (h.ncol + 63) / 64;
+  (void)hipMemcpyAsync(h.s0, scale, sizeof(double) * h.N, hipMemcpyDeviceToDevice, s);
+  const int nb = (h.ncol + PCOLS - 1) / PCOLS;
   if (prec == VTSA_FP32)
-    hipLaunchKernelGGL(k_prec_factor<float>, dim3(nb), dim3(64), 0, s, h.n, h.desc.Nz, h.ncol, kind, h.Jc, h.s0, dh,
+    hipLaunchKernelGGL(k_prec_factor<float>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, kind, h.Jc, h.s0, dh,
                        (float*)h.F);
   else
-    hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb), dim3(64), 0, s, h.n, h.desc.Nz, h.ncol, kind, h.Jc, h.s0, dh,
+    hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, kind, h.Jc, h.s0, dh,
                        (double*)h.F);
   if (h.nw > 0) hipLaunchKernelGGL(k_prec_wall, dim3((h.nw + 255) / 256), dim3(256), 0, s, h.nw, h.Jw, dh, h.Fw);
   h.prec_kind = kind;
@@ -310,22 +526,31 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
   h.prec_ready = 1;
 }
 
+// Columns per apply block: up to PCOLS, as many as fit the 64 KiB of LDS (cpb (6 Nz + 6) doubles).
+int apply_columns_per_block(int Nz) {
+  const int fit = (int)(65536 / (8 * (6 * (long long)Nz + 6)));
+  return fit < PCOLS ? fit : PCOLS;
+}
+
 size_t partial_size(int ncol, int nw) {
   size_t a = (size_t)RMAXB * (MAX_RESTART + 2);
-  size_t pg = (size_t)(ncol + PCOLS - 1) / PCOLS + (nw + PW * 4 - 1) / (PW * 4) + 1;
+  size_t pg = (size_t)ncol + (nw + PW * 4 - 1) / (PW * 4) + 1;   // apply grid with one column per block
   size_t b = (size_t)RMAXB * 8 + pg;
   return a > b ? a : b;
 }
 
 static void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
-  const int ncb = (h.ncol + PCOLS - 1) / PCOLS;
+  const int Nz = h.desc.Nz;
+  const int cpb = apply_columns_per_block(Nz);
+  const int ncb = (h.ncol + cpb - 1) / cpb;
   int nwb = h.nw > 0 ? (h.nw + PW * 4 - 1) / (PW * 4) : 0;
   const int grid = ncb + nwb;
+  const size_t lds = sizeof(double) * ((size_t)cpb * Nz * 6 + (size_t)cpb * 6);
   if (h.prec_prec == VTSA_FP32)
-    hipLaunchKernelGGL(k_prec_apply<float>, dim3(grid), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, h.nw, h.prec_kind,
+    hipLaunchKernelGGL(k_prec_apply<float>, dim3(grid), dim3(PW), lds, s, h.n, Nz, h.ncol, h.nw, h.prec_kind, cpb,
                        (const float*)h.F, h.Fw, h.s0, a, h.red.partial + RMAXB * 8, h.red.counter + 1);
   else
-    hipLaunchKernelGGL(k_prec_apply<double>, dim3(grid), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, h.nw, h.prec_kind,
+    hipLaunchKernelGGL(k_prec_apply<double>, dim3(grid), dim3(PW), lds, s, h.n, Nz, h.ncol, h.nw, h.prec_kind, cpb,
                        (const double*)h.F, h.Fw, h.s0, a, h.red.partial + RMAXB * 8, h.red.counter + 1);
 }
 
@@ -353,10 +578,8 @@ __global__ void __launch_bounds__(RBLK) k_residual_combine(int N, const double* 
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) tot += partial[b];
-    ticket_reset(counter);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
     *result = sqrt(tot / (double)N);
   }
 }
@@ -373,10 +596,8 @@ __global__ void __launch_bounds__(RBLK) k_rms(int N, const double* __restrict__ 
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) tot += partial[b];
-    ticket_reset(counter);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
     *result = sqrt(tot / (double)N);
   }
 }
@@ -404,10 +625,8 @@ __global__ void __launch_bounds__(RBLK) k_newton_update(int N, int npos, const d
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) tot += partial[b];
-    ticket_reset(counter);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
     *result = sqrt(tot / (double)N);
   }
 }
@@ -475,18 +694,15 @@ __global__ void __launch_bounds__(RBLK) k_weighted(Model m, int mode, const doub
     }
   }
   const int nout = mode == 0 ? 8 : 7;
-  for (int q = 0; q < nout; ++q) {
-    double bs = wave_block_sum(loc[q], sh);
-    if (threadIdx.x == 0) partial[q * RMAXB + blockIdx.x] = bs;
-  }
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    for (int q = 0; q < nout; ++q) {
-      double tot = 0.0;
-      for (int bb = 0; bb < (int)gridDim.x; ++bb) tot += partial[q * RMAXB + bb];
-      result[q] = tot;
+  __shared__ double tot[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {   // constant trip count keeps loc[] in registers
+    if (q < nout) {
+      double bs = wave_block_sum(loc[q], sh);
+      if (threadIdx.x == 0) partial[q * RMAXB + blockIdx.x] = bs;
     }
-    ticket_reset(counter);
   }
+  if (last_block_totals(counter, &slast, partial, nout, tot) && threadIdx.x < nout) result[threadIdx.x] = tot[threadIdx.x];
 }
 
 // Integrator vector formulas (core/solver.py:419-476), exact upstream operation order.
@@ -521,17 +737,15 @@ __global__ void k_vec_op(int op, int N, int npos, double a, double b, double c, 
 // Initialisation: |b|, tolerances (scipy gmres with x0 = 0, M = I), V[0] unnormalised = b.
 __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restrict__ b, double rtol, double atol,
                                                   GmresScalars* __restrict__ gs, double* __restrict__ partial,
-                                                  unsigned* counter) {
+                                                  unsigned* counter, int* __restrict__ progress) {
   __shared__ double sh[RBLK];
   __shared__ bool slast;
   double loc = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) loc += b[i] * b[i];
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int bb = 0; bb < (int)gridDim.x; ++bb) tot += partial[bb];
-    ticket_reset(counter);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
     double bn = sqrt(tot);
     gs->bnorm = bn;
     double ae = atol > rtol * bn ? atol : rtol * bn;
@@ -544,36 +758,90 @@ __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restric
     gs->ptol = bn > 0.0 ? bn * (1.0 < ae / bn ? 1.0 : ae / bn) : 0.0;
     gs->S[0] = bn;
     gs->inv_h1 = bn > 0.0 ? 1.0 / bn : 0.0;
+    __hip_atomic_store(progress, gs->done ? -1 : 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
-// Pass 1 of classical Gram-Schmidt: h_k = V_k . w (k <= col), |w|^2.
+// Classical Gram-Schmidt with one reorthogonalisation (CGS2), three passes per Arnoldi step.
+// Every pass reads the K = col + 1 basis vectors once per element; per-vector block sums go
+// through wavefront butterflies (no per-thread arrays indexed by K), and the last block sums the
+// per-block partials with one wavefront per basis vector.
+constexpr int GMW = MAX_RESTART + 2;   // partial rows per reduction (K dots + |w|^2)
+constexpr int GMC = 8;                 // basis vectors whose loads are in flight together
+
+// Wavefront sums of V_k . w over this thread's elements for k < nk (w is stored as the column
+// after the last basis vector, so k = col + 1 gives |w|^2); lane 0 writes them to out[k].
+__device__ inline void block_dots(const double* __restrict__ Vb, size_t ld, const double* __restrict__ w, int nk,
+                                  int N, int i0, int stride, double* out) {
+  for (int k0 = 0; k0 < nk; k0 += GMC) {
+    double p[GMC];
+#pragma unroll
+    for (int j = 0; j < GMC; ++j) p[j] = 0.0;
+    for (int i = i0; i < N; i += stride) {
+      const double wi = w[i];
+      double v[GMC];
+#pragma unroll
+      for (int j = 0; j < GMC; ++j) v[j] = (k0 + j < nk) ? Vb[(size_t)(k0 + j) * ld + i] : 0.0;
+#pragma unroll
+      for (int j = 0; j < GMC; ++j) p[j] += v[j] * wi;
+    }
+    // GMC butterflies side by side: each step issues all its lane exchanges before waiting
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double o[GMC];
+#pragma unroll
+      for (int j = 0; j < GMC; ++j) o[j] = __shfl_xor(p[j], off, 64);
+#pragma unroll
+      for (int j = 0; j < GMC; ++j) p[j] += o[j];
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int j = 0; j < GMC; ++j)
+        if (k0 + j < nk) out[k0 + j] = p[j];
+    }
+  }
+}
+
+// w_i - sum_k h_k V_k[i], subtracting in k order, with GMC loads in flight.
+__device__ inline double project_out(const double* __restrict__ Vb, size_t ld, const double* hk, int K, int i,
+                                     double wi) {
+  for (int k0 = 0; k0 < K; k0 += GMC) {
+    double v[GMC];
+#pragma unroll
+    for (int j = 0; j < GMC; ++j) v[j] = (k0 + j < K) ? Vb[(size_t)(k0 + j) * ld + i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < GMC; ++j)
+      if (k0 + j < K) wi -= hk[k0 + j] * v[j];
+  }
+  return wi;
+}
+
+// Pass 1: h_k = V_k . w (k <= col), |w|^2.
 __global__ void __launch_bounds__(RBLK) k_gm_dots(int N, int col, const double* __restrict__ Vb, size_t ld,
                                                   const double* __restrict__ w, GmresScalars* __restrict__ gs,
                                                   double* __restrict__ partial, unsigned* counter) {
-  __shared__ double sh[RBLK];
+  __shared__ double sh[(RBLK / 64) * GMW];
+  __shared__ double tot[GMW];
   __shared__ bool slast;
   if (gs->done) return;
-  double loc[MAX_RESTART + 1];
   const int K = col + 1;
-  for (int k = 0; k <= K; ++k) loc[k] = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    double wi = w[i];
-    for (int k = 0; k < K; ++k) loc[k] += Vb[k * ld + i] * wi;
-    loc[K] += wi * wi;
+  const int wv = threadIdx.x >> 6;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  block_dots(Vb, ld, w, K + 1, N, i0, stride, sh + wv * GMW);
+  __syncthreads();
+  if ((int)threadIdx.x <= K) {
+    double s = 0.0;
+    for (int q = 0; q < RBLK / 64; ++q) s += sh[q * GMW + threadIdx.x];
+    partial[threadIdx.x * RMAXB + blockIdx.x] = s;
   }
-  for (int k = 0; k <= K; ++k) {
-    double bs = wave_block_sum(loc[k], sh);
-    if (threadIdx.x == 0) partial[k * RMAXB + blockIdx.x] = bs;
-  }
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    for (int k = 0; k <= K; ++k) {
-      double tot = 0.0;
-      for (int bb = 0; bb < (int)gridDim.x; ++bb) tot += partial[k * RMAXB + bb];
-      if (k < K) gs->h_tmp[k] = tot; else gs->h0sq = tot;
+  if (last_block_totals(counter, &slast, partial, K + 1, tot) && (int)threadIdx.x <= K) {
+    const int k = threadIdx.x;
+    if (k < K) {
+      gs->h_tmp[k] = tot[k];
+      gs->H[k][col] = tot[k];
+    } else {
+      gs->h0sq = tot[K];
     }
-    for (int k = 0; k < K; ++k) gs->H[k][col] = gs->h_tmp[k];
-    ticket_reset(counter);
   }
 }
 
@@ -581,32 +849,28 @@ __global__ void __launch_bounds__(RBLK) k_gm_dots(int N, int col, const double* 
 __global__ void __launch_bounds__(RBLK) k_gm_orth(int N, int col, const double* __restrict__ Vb, size_t ld,
                                                   double* __restrict__ w, GmresScalars* __restrict__ gs,
                                                   double* __restrict__ partial, unsigned* counter) {
-  __shared__ double sh[RBLK];
+  __shared__ double sh[(RBLK / 64) * GMW];
+  __shared__ double hk[GMW];
+  __shared__ double tot[GMW];
   __shared__ bool slast;
   if (gs->done) return;
   const int K = col + 1;
-  double hk[MAX_RESTART];
-  for (int k = 0; k < K; ++k) hk[k] = gs->h_tmp[k];
-  double loc[MAX_RESTART];
-  for (int k = 0; k < K; ++k) loc[k] = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    double wi = w[i];
-    for (int k = 0; k < K; ++k) wi -= hk[k] * Vb[k * ld + i];
-    w[i] = wi;
-    for (int k = 0; k < K; ++k) loc[k] += Vb[k * ld + i] * wi;
+  const int wv = threadIdx.x >> 6;
+  if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
+  __syncthreads();
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (int i = i0; i < N; i += stride) w[i] = project_out(Vb, ld, hk, K, i, w[i]);
+  block_dots(Vb, ld, w, K, N, i0, stride, sh + wv * GMW);
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    double s = 0.0;
+    for (int q = 0; q < RBLK / 64; ++q) s += sh[q * GMW + threadIdx.x];
+    partial[threadIdx.x * RMAXB + blockIdx.x] = s;
   }
-  for (int k = 0; k < K; ++k) {
-    double bs = wave_block_sum(loc[k], sh);
-    if (threadIdx.x == 0) partial[k * RMAXB + blockIdx.x] = bs;
-  }
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    for (int k = 0; k < K; ++k) {
-      double tot = 0.0;
-      for (int bb = 0; bb < (int)gridDim.x; ++bb) tot += partial[k * RMAXB + bb];
-      gs->h_tmp[k] = tot;
-      gs->H[k][col] += tot;
-    }
-    ticket_reset(counter);
+  if (last_block_totals(counter, &slast, partial, K, tot) && (int)threadIdx.x < K) {
+    const int k = threadIdx.x;
+    gs->h_tmp[k] = tot[k];
+    gs->H[k][col] += tot[k];
   }
 }
 
@@ -624,69 +888,95 @@ __device__ inline void lartg(double f, double g, double& c, double& s, double& r
 // (scipy gmres inner loop).
 __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double* __restrict__ Vb, size_t ld,
                                                     double* __restrict__ w, GmresScalars* __restrict__ gs,
-                                                    double* __restrict__ partial, unsigned* counter) {
-  __shared__ double sh[RBLK];
+                                                    double* __restrict__ partial, unsigned* counter,
+                                                    int* __restrict__ progress) {
+  __shared__ double sh[RBLK / 64];
+  __shared__ double hk[GMW];
+  __shared__ double hc[GMW], csl[GMW], snl[GMW];
   __shared__ bool slast;
   if (gs->done) return;
   const int K = col + 1;
-  double hk[MAX_RESTART];
-  for (int k = 0; k < K; ++k) hk[k] = gs->h_tmp[k];
+  if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
+  __syncthreads();
   double loc = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    double wi = w[i];
-    for (int k = 0; k < K; ++k) wi -= hk[k] * Vb[k * ld + i];
+    const double wi = project_out(Vb, ld, hk, K, i, w[i]);
     w[i] = wi;
     loc += wi * wi;
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int bb = 0; bb < (int)gridDim.x; ++bb) tot += partial[bb];
+  if (!ticket_last(counter, &slast)) return;
+  // last block: |w|, then the Givens sweep of column col on LDS copies
+  const int t = threadIdx.x;
+  double tot = 0.0;
+  if (t < 64) tot = wave_sum_strided(partial, gridDim.x);
+  for (int k = t; k <= col; k += blockDim.x) hc[k] = gs->H[k][col];
+  for (int k = t; k < col; k += blockDim.x) { csl[k] = gs->cs[k]; snl[k] = gs->sn[k]; }
+  __syncthreads();
+  if (t == 0) {
     ticket_reset(counter);
     double h1 = sqrt(tot);
     double h0 = sqrt(gs->h0sq);
     bool brk = h1 <= EPS64 * h0;
-    gs->H[col + 1][col] = brk ? 0.0 : h1;
+    hc[col + 1] = brk ? 0.0 : h1;
     gs->inv_h1 = brk ? 0.0 : 1.0 / h1;
     for (int k = 0; k < col; ++k) {
-      double c = gs->cs[k], s = gs->sn[k];
-      double n0 = gs->H[k][col], n1 = gs->H[k + 1][col];
-      gs->H[k][col] = c * n0 + s * n1;
-      gs->H[k + 1][col] = -s * n0 + c * n1;
+      double c = csl[k], s = snl[k];
+      double n0 = hc[k], n1 = hc[k + 1];
+      hc[k] = c * n0 + s * n1;
+      hc[k + 1] = -s * n0 + c * n1;
     }
     double c, s, mag;
-    lartg(gs->H[col][col], gs->H[col + 1][col], c, s, mag);
+    lartg(hc[col], hc[col + 1], c, s, mag);
     gs->cs[col] = c;
     gs->sn[col] = s;
-    gs->H[col][col] = mag;
-    gs->H[col + 1][col] = 0.0;
+    hc[col] = mag;
+    hc[col + 1] = 0.0;
     double tmp = -s * gs->S[col];
     gs->S[col] = c * gs->S[col];
     gs->S[col + 1] = tmp;
     gs->presid = fabs(tmp);
     gs->iters = col + 1;
     if (brk) gs->breakdown = 1;
-    if (gs->presid <= gs->ptol || brk) gs->done = 1;
+    const bool fin = gs->presid <= gs->ptol || brk;
+    if (fin) gs->done = 1;
+    // host-visible progress: completed columns, negative once the cycle is done
+    __hip_atomic_store(progress, fin ? -(col + 1) : (col + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+  __syncthreads();
+  for (int k = t; k <= col + 1; k += blockDim.x) gs->H[k][col] = hc[k];
 }
 
 // Back substitution of the least-squares problem (scipy's pseudo-solve of the triangular system).
-__global__ void k_gm_solve(GmresScalars* gs) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Back substitution H y = S (scipy's column-oriented loop, same operation order), one wavefront:
+// the triangle is staged in LDS, lane i owns y_i.  Launch with one block of 64 threads.
+__global__ void __launch_bounds__(64) k_gm_solve(GmresScalars* gs) {
+  __shared__ double Hs[MAX_RESTART][MAX_RESTART + 1];
+  __shared__ double ys[MAX_RESTART];
   if (gs->done == 2) return;
   const int col = gs->iters - 1;
   if (col < 0) return;
-  if (gs->H[col][col] == 0.0) gs->S[col] = 0.0;
-  for (int k = 0; k <= col; ++k) gs->yv[k] = gs->S[k];
-  for (int k = col; k > 0; --k) {
-    if (gs->yv[k] != 0.0) {
-      gs->yv[k] /= gs->H[k][k];
-      double tmp = gs->yv[k];
-      for (int i = 0; i < k; ++i) gs->yv[i] -= tmp * gs->H[i][k];
-    }
+  const int lane = threadIdx.x;
+  for (int e = lane; e < (col + 1) * (col + 1); e += 64) {
+    const int i = e / (col + 1), k = e - i * (col + 1);
+    Hs[i][k] = gs->H[i][k];
   }
-  if (gs->yv[0] != 0.0) gs->yv[0] /= gs->H[0][0];
+  if (lane <= col) ys[lane] = (lane == col && gs->H[col][col] == 0.0) ? 0.0 : gs->S[lane];
+  __syncthreads();
+  for (int k = col; k > 0; --k) {
+    const double yk = ys[k];
+    if (yk != 0.0) {
+      const double q = yk / Hs[k][k];
+      if (lane < k) ys[lane] -= q * Hs[lane][k];
+      __syncthreads();
+      if (lane == 0) ys[k] = q;
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && ys[0] != 0.0) ys[0] /= Hs[0][0];
+  __syncthreads();
+  if (lane <= col) gs->yv[lane] = ys[lane];
 }
 
 // x = sum_k y_k V_k
@@ -715,10 +1005,8 @@ __global__ void __launch_bounds__(RBLK) k_gm_resid(int N, const double* __restri
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (ticket_last(counter, &slast) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int bb = 0; bb < (int)gridDim.x; ++bb) tot += partial[bb];
-    ticket_reset(counter);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
     gs->rnorm = sqrt(tot);
   }
 }
@@ -823,8 +1111,30 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   double* Vb = h.Vb;
   double* U = h.vtmp2;
   const int g = rgrid(N);
-  hipLaunchKernelGGL(k_gm_init, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial, h.red.counter + 5);
+  // The host queues at most GM_LOOKAHEAD Arnoldi columns beyond the last one the device reports
+  // complete (a coherent host word written by k_gm_init / k_gm_finish), and stops queueing once
+  // the device reports convergence; queued columns past convergence return at once.
+  constexpr int GM_LOOKAHEAD = 3;
+  constexpr int NOT_STARTED = -0x7fffffff;
+  __atomic_store_n(h.progress, NOT_STARTED, __ATOMIC_RELEASE);
+  hipLaunchKernelGGL(k_gm_init, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial, h.red.counter + 5,
+                     h.progress_dev);
+  if (check(hipGetLastError(), "k_gm_init")) return -1;
   for (int col = 0; col < restart; ++col) {
+    if (col >= GM_LOOKAHEAD) {
+      const auto t0 = std::chrono::steady_clock::now();
+      int v;
+      for (;;) {
+        v = __atomic_load_n(h.progress, __ATOMIC_ACQUIRE);
+        if (v != NOT_STARTED && (v < 0 || v >= col - GM_LOOKAHEAD + 1)) break;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
+          set_error("vtsa_gmres: no device progress for 120 s");
+          return -1;
+        }
+        std::this_thread::yield();
+      }
+      if (v < 0) break;
+    }
     // u = P^-1 v_col with v_col = w_prev / |w_prev| normalised in place
     const double* vin = col == 0 ? b : Vb + (size_t)col * ld;
     ApplyArgs a{vin, &h.gs->inv_h1, Vb + (size_t)col * ld, s, s, U, nullptr, nullptr, h.gs, zs_norm, 1};
@@ -836,9 +1146,9 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
     hipLaunchKernelGGL(k_gm_orth, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
                        h.red.counter + 5);
     hipLaunchKernelGGL(k_gm_finish, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
-                       h.red.counter + 5);
+                       h.red.counter + 5, h.progress_dev);
   }
-  hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(1), 0, st, h.gs);
+  hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, h.gs);
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);
   // true residual r = b - A P^-1 w (one more matvec, as scipy does after the cycle)
   {
