@@ -49,19 +49,31 @@ __device__ inline double wave_block_sum(double v, double* sh) {
   return r;
 }
 
+// Cross-block reductions without cache-wide fences.  Every block writes its partial sums with
+// agent-scope stores (coherent across the XCDs' L2 caches), waits for their completion, and
+// only then takes a ticket; the last block to take one reads the partials with agent-scope
+// loads.  A device-scope release/acquire fence would write back and invalidate the whole L2 of
+// the block's XCD instead, once per block.
+__device__ inline void store_partial(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_partial(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Sum of partial[0 .. cnt) by one wavefront (lane-strided, then butterfly): deterministic.
 __device__ inline double wave_sum_strided(const double* __restrict__ partial, int cnt) {
   double acc = 0.0;
-  for (int b = threadIdx.x & 63; b < cnt; b += 64) acc += partial[b];
+  for (int b = threadIdx.x & 63; b < cnt; b += 64) acc += load_partial(partial + b);
   return wave_sum(acc);
 }
 
 // Returns true in the last block to finish; that block may then read every block's partials.
 __device__ inline bool ticket_last(unsigned* counter, bool* sflag) {
-  __syncthreads();
+  __builtin_amdgcn_s_waitcnt(0);   // this wavefront's partial stores have completed
+  __syncthreads();                 // ... and every other wavefront's
   if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned tk = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned tk = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *sflag = (tk == gridDim.x - 1);
   }
   __syncthreads();
@@ -69,7 +81,7 @@ __device__ inline bool ticket_last(unsigned* counter, bool* sflag) {
 }
 
 __device__ inline void ticket_reset(unsigned* counter) {
-  __hip_atomic_store(counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Last block of a one-value reduction: its first wavefront sums the per-block partials; returns
@@ -496,7 +508,7 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
     }
   }
   double bs = wave_block_sum(local, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
     const int N = 6 * n + nw;
@@ -577,7 +589,7 @@ __global__ void __launch_bounds__(RBLK) k_residual_combine(int N, const double* 
     if (s) { double q = r / s[i]; loc += q * q; }
   }
   double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
     *result = sqrt(tot / (double)N);
@@ -595,7 +607,7 @@ __global__ void __launch_bounds__(RBLK) k_rms(int N, const double* __restrict__ 
     loc += q * q;
   }
   double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
     *result = sqrt(tot / (double)N);
@@ -624,7 +636,7 @@ __global__ void __launch_bounds__(RBLK) k_newton_update(int N, int npos, const d
     loc += q * q;
   }
   double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
     *result = sqrt(tot / (double)N);
@@ -699,7 +711,7 @@ __global__ void __launch_bounds__(RBLK) k_weighted(Model m, int mode, const doub
   for (int q = 0; q < 8; ++q) {   // constant trip count keeps loc[] in registers
     if (q < nout) {
       double bs = wave_block_sum(loc[q], sh);
-      if (threadIdx.x == 0) partial[q * RMAXB + blockIdx.x] = bs;
+      if (threadIdx.x == 0) store_partial(partial + q * RMAXB + blockIdx.x, bs);
     }
   }
   if (last_block_totals(counter, &slast, partial, nout, tot) && threadIdx.x < nout) result[threadIdx.x] = tot[threadIdx.x];
@@ -743,7 +755,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restric
   double loc = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) loc += b[i] * b[i];
   double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
     double bn = sqrt(tot);
@@ -758,7 +770,8 @@ __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restric
     gs->ptol = bn > 0.0 ? bn * (1.0 < ae / bn ? 1.0 : ae / bn) : 0.0;
     gs->S[0] = bn;
     gs->inv_h1 = bn > 0.0 ? 1.0 / bn : 0.0;
-    __hip_atomic_store(progress, gs->done ? -1 : 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    // pacing only: the host reads no data on the strength of this word, so relaxed suffices
+    __hip_atomic_store(progress, gs->done ? -1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -832,7 +845,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_dots(int N, int col, const double* 
   if ((int)threadIdx.x <= K) {
     double s = 0.0;
     for (int q = 0; q < RBLK / 64; ++q) s += sh[q * GMW + threadIdx.x];
-    partial[threadIdx.x * RMAXB + blockIdx.x] = s;
+    store_partial(partial + threadIdx.x * RMAXB + blockIdx.x, s);
   }
   if (last_block_totals(counter, &slast, partial, K + 1, tot) && (int)threadIdx.x <= K) {
     const int k = threadIdx.x;
@@ -865,7 +878,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_orth(int N, int col, const double* 
   if ((int)threadIdx.x < K) {
     double s = 0.0;
     for (int q = 0; q < RBLK / 64; ++q) s += sh[q * GMW + threadIdx.x];
-    partial[threadIdx.x * RMAXB + blockIdx.x] = s;
+    store_partial(partial + threadIdx.x * RMAXB + blockIdx.x, s);
   }
   if (last_block_totals(counter, &slast, partial, K, tot) && (int)threadIdx.x < K) {
     const int k = threadIdx.x;
@@ -905,7 +918,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
     loc += wi * wi;
   }
   double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   if (!ticket_last(counter, &slast)) return;
   // last block: |w|, then the Givens sweep of column col on LDS copies
   const int t = threadIdx.x;
@@ -942,7 +955,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
     const bool fin = gs->presid <= gs->ptol || brk;
     if (fin) gs->done = 1;
     // host-visible progress: completed columns, negative once the cycle is done
-    __hip_atomic_store(progress, fin ? -(col + 1) : (col + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(progress, fin ? -(col + 1) : (col + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   __syncthreads();
   for (int k = t; k <= col + 1; k += blockDim.x) gs->H[k][col] = hc[k];
@@ -1004,7 +1017,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_resid(int N, const double* __restri
     loc += r * r;
   }
   double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
     gs->rnorm = sqrt(tot);
