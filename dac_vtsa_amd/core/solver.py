@@ -36,7 +36,7 @@ SW1 = (3.0 * G_SD - 1.0) / (6.0 * G_SD)
 SW2 = 1.0 / (6.0 * G_SD * (1.0 - G_SD))
 SW3 = (2.0 - 3.0 * G_SD) / (6.0 * (1.0 - G_SD))
 
-PRECONDITIONERS = {"block_jacobi": 0, "zline": 1}
+PRECONDITIONERS = {"block_jacobi": 0, "zline": 1, "zline_gs": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1}
 
 
@@ -124,8 +124,9 @@ class DeviceOps:
         _abi.call("vtsa_error_norm", self.h, dv.ptr(y), dv.ptr(z2), dv.ptr(y3), float(rtol), C.byref(en), self.st())
         return en.value
 
-    def precond_setup(self, dh, scale, kind, precision):
-        _abi.call("vtsa_precond_setup", self.h, float(dh), dv.ptr(scale), int(kind), int(precision), self.st())
+    def precond_setup(self, dh, scale, kind, precision, sweeps=1):
+        _abi.call("vtsa_precond_setup", self.h, float(dh), dv.ptr(scale), int(kind), int(precision), int(sweeps),
+                  self.st())
 
 
 def _finite(x):
@@ -239,11 +240,15 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
 class TRBDF2:
     """Two-stage L-stable implicit integrator (core/solver.py:297-511) on the GPU.
 
-    Extra keyword arguments (no upstream counterpart): ``precond`` ("zline" | "block_jacobi"),
-    ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``."""
+    Extra keyword arguments (no upstream counterpart): ``precond`` ("zline_gs" | "zline" |
+    "block_jacobi"), ``gs_sweeps`` (symmetric line Gauss-Seidel sweeps of "zline_gs"),
+    ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``.  The
+    preconditioner stands in for the upstream SuperLU factor of the theta-block matrix wherever
+    that factor is used: GMRES, the Newton residual norm and the error-estimate filter."""
 
     def __init__(self, model, cfg, h_max: float, h_init: Optional[float] = None, logger=None,
-                 precond: str = "zline", precision: str = "fp64", krylov_dim: int = 30):
+                 precond: str = "zline_gs", precision: str = "fp64", krylov_dim: int = 30,
+                 gs_sweeps: int = 1):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -271,6 +276,7 @@ class TRBDF2:
         self.precond_kind = PRECONDITIONERS[precond]
         self.precision = PRECISIONS[precision]
         self.krylov_dim = krylov_dim
+        self.gs_sweeps = int(gs_sweeps)
         self.cap_after_fail = 0.5
         self.cap_relax = 1.15
         self.ops = DeviceOps(model)
@@ -283,7 +289,8 @@ class TRBDF2:
         return self.ops.scale(y, self.rtol)
 
     def _factor(self, y_ref, h):
-        self.ops.precond_setup(D * h, self.ops.scale(y_ref, self.rtol), self.precond_kind, self.precision)
+        self.ops.precond_setup(D * h, self.ops.scale(y_ref, self.rtol), self.precond_kind, self.precision,
+                               self.gs_sweeps)
         return Factorisation(h)
 
     def _solve_stage(self, t, z0, rhs_known, dh, lu, res, y_ref):

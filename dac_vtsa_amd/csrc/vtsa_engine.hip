@@ -44,6 +44,9 @@ static void free_handle(Handle* h) {
   if (h->gs_host) (void)hipHostFree(h->gs_host);
   if (h->host_scalars) (void)hipHostFree(h->host_scalars);
   if (h->progress) (void)hipHostFree(h->progress);
+  if (h->gs_order) (void)hipFree(h->gs_order);
+  if (h->gsb) (void)hipFree(h->gsb);
+  if (h->gsu) (void)hipFree(h->gsu);
   delete h;
 }
 
@@ -84,7 +87,7 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   if (!desc || !out) { set_error("vtsa_create: null argument"); return -1; }
   const int Nr = desc->Nr, Nth = desc->Nth, Nz = desc->Nz;
   if (Nr < 1 || Nth < 1 || Nz < 2 || Nth == 2) { set_error("vtsa_create: invalid grid"); return -2; }
-  if (apply_columns_per_block(Nz) < 1) { set_error("vtsa_create: Nz too large for the z-line preconditioner (LDS)"); return -2; }
+  if (!zline_fits(Nz)) { set_error("vtsa_create: Nz too large for the z-line preconditioner (LDS)"); return -2; }
   vtsa_handle* H = new vtsa_handle();
   Handle& h = H->h;
   h.desc = *desc;
@@ -117,6 +120,15 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   rc |= check(hipMalloc(&h.F, sizeof(double) * (size_t)h.n * 180), "hipMalloc");
   rc |= dalloc(&h.Fw, nwall);
   rc |= dalloc(&h.s0, h.N);
+  rc |= dalloc(&h.gsb, h.N);
+  rc |= dalloc(&h.gsu, h.N);
+  {
+    std::vector<int> order;
+    gs_colour_lines(Nr, Nth, h.nw > 0 ? Nth : 0, order, h.gs_cstart);
+    rc |= check(hipMalloc((void**)&h.gs_order, sizeof(int) * order.size()), "hipMalloc");
+    if (rc == 0)
+      rc |= check(hipMemcpy(h.gs_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  }
   rc |= dalloc(&h.gs, 1);
   rc |= check(hipHostMalloc((void**)&h.gs_host, sizeof(GmresScalars)), "hipHostMalloc");
   rc |= check(hipHostMalloc((void**)&h.host_scalars, sizeof(double) * 64), "hipHostMalloc");
@@ -221,11 +233,15 @@ int vtsa_jacobian_get(vtsa_handle* H, double* Jc, double* Jcw, double* Jw, doubl
 }
 
 int vtsa_precond_setup(vtsa_handle* H, double dh, const double* scale, int32_t kind, int32_t precision,
-                       vtsa_stream s) {
+                       int32_t sweeps, vtsa_stream s) {
   if (!H || !scale) { set_error("vtsa_precond_setup: null argument"); return -1; }
-  if (kind != VTSA_PREC_BLOCK_JACOBI && kind != VTSA_PREC_ZLINE) { set_error("unknown preconditioner"); return -2; }
+  if (kind != VTSA_PREC_BLOCK_JACOBI && kind != VTSA_PREC_ZLINE && kind != VTSA_PREC_ZLINE_GS) {
+    set_error("unknown preconditioner");
+    return -2;
+  }
   if (precision != VTSA_FP64 && precision != VTSA_FP32) { set_error("unknown precision"); return -2; }
-  launch_prec_setup(H->h, dh, scale, kind, precision, S(s));
+  if (sweeps < 1 || sweeps > 16) { set_error("vtsa_precond_setup: sweeps must be in 1..16"); return -2; }
+  launch_prec_setup(H->h, dh, scale, kind, precision, sweeps, S(s));
   return finish(S(s));
 }
 

@@ -56,6 +56,13 @@ struct Handle {
   double* Fw = nullptr;       // [nw] wall diagonal of M
   double* s0 = nullptr;       // [N] scale of the factorisation
   int prec_kind = -1, prec_prec = 0, prec_ready = 0;
+  double prec_dh = 0.0;
+  // line Gauss-Seidel: lines grouped by colour (cell z-lines 0..ncol-1, wall lines after)
+  int* gs_order = nullptr;           // device [ncol + wall lines]
+  std::vector<int> gs_cstart;        // colour c owns gs_order[gs_cstart[c] .. gs_cstart[c + 1])
+  int gs_sweeps = 1;
+  double* gsb = nullptr;             // [N] scaled right-hand side
+  double* gsu = nullptr;             // [N] unscaled iterate
   // GMRES
   double* Vb = nullptr;       // basis [(MAX_RESTART+1) * N]
   int vb_cols = 0;
@@ -85,7 +92,8 @@ void launch_matvec_residual(Handle& h, double t, const double* z, const double* 
                             const double* f, double dh, double* w, bool respect_done, hipStream_t st);
 
 // linear algebra (vtsa_linalg.hip)
-void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, hipStream_t s);
+void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, int sweeps, hipStream_t s);
+void gs_colour_lines(int Nr, int Nth, int nwall_lines, std::vector<int>& order, std::vector<int>& cstart);
 // out = post^-1 * M^-1 (pre * in * (in_scale ? *in_scale : 1)); optionally stores the scaled
 // input to `store_in`; optionally accumulates sum(out^2) and sets sigma in gs (matvec use).
 void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,
@@ -109,6 +117,17 @@ int error_norm_impl(Handle& h, const double* y, const double* z2, const double* 
 void vec_op_impl(Handle& h, int op, int npos, double a, double b, double c, double e, const double* x,
                  const double* y, const double* z, const double* w, double* out, hipStream_t st);
 size_t partial_size(int ncol, int nw);
-int apply_columns_per_block(int Nz);
+bool zline_fits(int Nz);
+
+// Preconditioner apply: out = post^-1 * S0 M~^-1 S0^-1 (pre * scale * in); optional side outputs:
+//   store_in  <- scale * in (normalises a GMRES basis vector in place; may alias in)
+//   red_out   <- rms(out / red_s) (or rms(out))
+//   gs        <- unorm, sigma = sqrt(eps) zs_norm / unorm, zero_u (matvec finite-difference step)
+// respect_done: return at once when the GMRES cycle in gs has finished.
+struct ApplyArgs {
+  const double* in; const double* in_scale; double* store_in; const double* pre; const double* post; double* out;
+  const double* red_s; double* red_out; GmresScalars* gs; double zs_norm; int respect_done;
+};
+void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s);
 
 }  // namespace vtsa

@@ -1,0 +1,799 @@
+// Preconditioner of the Newton-Krylov solve: M ~ I - dh J in variables scaled by s0
+// (M~ = S0^-1 M S0).
+//  * z-line: for every (r, theta) column the block-pentadiagonal matrix of the 6x6 cell blocks
+//    coupling k with k-2..k+2 (MUSCL stencil) is factored by block LU without inter-block
+//    pivoting, M~ = L U, storing per cell L_{k,k-2}, L_{k,k-1}, D_k^-1 (Gauss-Jordan with partial
+//    pivoting), U_{k,k+1}, U_{k,k+2}.  The r and theta couplings are left to GMRES.
+//  * block Jacobi: D_k^-1 of the diagonal blocks only.
+//  * wall rows: point Jacobi on the wall diagonal.
+//  * line Gauss-Seidel: the z-line solves above, swept over coloured lines (cell z-lines and wall
+//    lines) so that the r and theta couplings are taken into account as well.
+#include "vtsa_internal.h"
+#include "vtsa_reduce.h"
+
+namespace vtsa {
+
+constexpr int PW = 64;           // threads per block (one wavefront)
+constexpr int PCOLS = 10;        // factor: columns per wavefront (6 lanes each)
+constexpr int AC = 4;            // apply gather/scatter: elements per lane whose loads go together
+
+// ------------------------------------------------------------------ preconditioner factor
+// Block LU of the z-line matrices (block Jacobi: the diagonal inverses only).  Six lanes per
+// column, lane r computing row r of every block product, PCOLS columns per wavefront; the
+// factors of the previous two cells stay in LDS.  The pivot blocks are inverted by Gauss-Jordan
+// with partial pivoting (the first row of largest |entry| wins), rows in parallel through LDS.
+// Block products accumulate over m in order in T, as C = A B and C -= A B of 6x6 matrices.
+// F blocks per cell, rows of 6: 0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2.
+constexpr int FSTR = 258;   // LDS doubles per column: Dinv/U1/U2 ring of two cells + pivot scratch
+
+template <typename T>
+__global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int kind,
+                                                    const double* __restrict__ Jc, const double* __restrict__ s0,
+                                                    double dh, T* __restrict__ F) {
+  __shared__ double lds[PCOLS * FSTR];
+  const int lane = threadIdx.x;
+  const int grp = lane / 6, r = lane - grp * 6;
+  const int col0 = blockIdx.x * PCOLS;
+  const int nc = min(PCOLS, ncol - col0);
+  const bool active = grp < nc;
+  const int col = col0 + (active ? grp : 0);
+  double* Lc = lds + (active ? grp : 0) * FSTR;
+  double* pDi = Lc;          // [2][36] Dinv of cells k-2 / k-1 (ring slot k & 1)
+  double* pU1 = Lc + 72;     // [2][36]
+  double* pU2 = Lc + 144;    // [2][36]
+  double* colp = Lc + 216;   // [6]  |pivot column|
+  double* sw = Lc + 222;     // [24] row swap
+  double* prow = Lc + 246;   // [12] normalised pivot row of D and of its inverse
+  const bool zl = kind == VTSA_PREC_ZLINE;
+  for (int k = 0; k < Nz; ++k) {
+    const int X = col * Nz + k;
+    const double sr = s0[r * n + X];
+    // row r of the block of M~ = S0^-1 (I - dh J) S0 between cell X and cell X + dk
+    auto mrow = [&](int slot, int dk, T* out) {
+      const double* J = Jc + ((size_t)X * NSLOT + slot) * 36 + r * 6;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double v = (slot == S_SELF && r == c ? 1.0 : 0.0) - dh * J[c];
+        out[c] = (T)(v * (s0[c * n + X + dk] / sr));
+      }
+    };
+    const int q = k & 1, q1 = (k + 1) & 1;   // ring slots of cells k-2 (and k) and k-1
+    T l2[6], l1[6], d[6], u1[6], u2[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { l2[c] = T(0); l1[c] = T(0); u1[c] = T(0); u2[c] = T(0); }
+    if (zl && k >= 2) {   // L2 = A_{k,k-2} Dinv_{k-2}
+      T a[6];
+      mrow(S_ZM2, -2, a);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += a[m] * (T)pDi[q * 36 + m * 6 + c];
+        l2[c] = acc;
+      }
+    }
+    if (zl && k >= 1) {   // L1 = (A_{k,k-1} - L2 U1_{k-2}) Dinv_{k-1}
+      T a[6];
+      mrow(S_ZM1, -1, a);
+      if (k >= 2) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          T acc = T(0);
+#pragma unroll
+          for (int m = 0; m < 6; ++m) acc += l2[m] * (T)pU1[q * 36 + m * 6 + c];
+          a[c] -= acc;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += a[m] * (T)pDi[q1 * 36 + m * 6 + c];
+        l1[c] = acc;
+      }
+    }
+    mrow(S_SELF, 0, d);   // D = A_kk - L1 U1_{k-1} - L2 U2_{k-2}
+    if (zl && k >= 1) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += l1[m] * (T)pU1[q1 * 36 + m * 6 + c];
+        d[c] -= acc;
+      }
+    }
+    if (zl && k >= 2) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += l2[m] * (T)pU2[q * 36 + m * 6 + c];
+        d[c] -= acc;
+      }
+    }
+    if (zl && k + 1 < Nz) {   // U1 = A_{k,k+1} - L1 U2_{k-1}
+      mrow(S_ZP1, 1, u1);
+      if (k >= 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          T acc = T(0);
+#pragma unroll
+          for (int m = 0; m < 6; ++m) acc += l1[m] * (T)pU2[q1 * 36 + m * 6 + c];
+          u1[c] -= acc;
+        }
+      }
+    }
+    if (zl && k + 2 < Nz) mrow(S_ZP2, 2, u2);   // U2 = A_{k,k+2}
+    lds_barrier();   // every lane has read ring slot q (cell k-2) before cell k overwrites it
+    T* FX = F + (size_t)X * 180 + r * 6;
+    if (active && zl) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        FX[c] = l2[c];
+        FX[36 + c] = l1[c];
+        FX[108 + c] = u1[c];
+        FX[144 + c] = u2[c];
+        pU1[q * 36 + r * 6 + c] = (double)u1[c];
+        pU2[q * 36 + r * 6 + c] = (double)u2[c];
+      }
+    }
+    // Gauss-Jordan inverse of D, lane r holding row r of D and of the inverse
+    T iv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) iv[c] = (r == c) ? T(1) : T(0);
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      if (active) colp[r] = (double)fabs(d[p]);
+      lds_barrier();
+      int piv = p;
+      double best = colp[p];
+      for (int rr = p + 1; rr < 6; ++rr) {
+        const double v = colp[rr];
+        if (v > best) { best = v; piv = rr; }
+      }
+      if (active && (r == p || r == piv)) {
+        double* dst = sw + (r == p ? 0 : 12);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { dst[c] = (double)d[c]; dst[6 + c] = (double)iv[c]; }
+      }
+      lds_barrier();
+      if (piv != p && (r == p || r == piv)) {
+        const double* src = sw + (r == p ? 12 : 0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { d[c] = (T)src[c]; iv[c] = (T)src[6 + c]; }
+      }
+      if (r == p) {
+        const T inv = T(1) / d[p];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { d[c] *= inv; iv[c] *= inv; }
+        if (active) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) { prow[c] = (double)d[c]; prow[6 + c] = (double)iv[c]; }
+        }
+      }
+      lds_barrier();
+      if (r != p) {
+        const T f = d[p];
+        if (f != T(0)) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) { d[c] -= f * (T)prow[c]; iv[c] -= f * (T)prow[6 + c]; }
+        }
+      }
+      lds_barrier();
+    }
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        FX[72 + c] = iv[c];
+        pDi[q * 36 + r * 6 + c] = (double)iv[c];
+      }
+    }
+    lds_barrier();
+  }
+}
+
+
+__global__ void k_prec_wall(int nw, const double* __restrict__ Jw, double dh, double* __restrict__ Fw) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nw) Fw[w] = 1.0 - dh * Jw[w * 5 + 0];
+}
+
+// ------------------------------------------------------------------ preconditioner apply
+// out = post^-1 * S0 M~^-1 S0^-1 (pre * scale * in), side outputs as described at ApplyArgs.
+//
+// One z-column per 64-lane block.  The column's right-hand side is gathered into LDS
+// (xs[k][6]) with batched coalesced loads; the factor rows are staged into LDS by direct
+// global-to-LDS copies in segments of ZSEG cells, double-buffered so the next segment is in
+// flight while the current one is swept.  Lanes 0..5 sweep (lane r owns row r); each new
+// 6-vector is broadcast to every lane with v_readlane, so the chain of cells along z carries
+// no LDS round trip.  Wall rows: point Jacobi in blocks of their own.
+constexpr int ZSEG = 16;   // cells per staged segment
+
+template <typename T>
+__host__ __device__ constexpr size_t zline_buf_bytes() {   // one staging buffer, whole 1 KiB copies
+  return ((size_t)ZSEG * 108 * sizeof(T) + 1023) / 1024 * 1024;
+}
+template <typename T>
+__host__ __device__ constexpr size_t zline_lds_bytes(int Nz) {
+  return sizeof(double) * ((size_t)Nz * 6 + 8) + 2 * zline_buf_bytes<T>();
+}
+
+// Copy `per` factor elements (starting at element `off` of each cell's 180) of cells
+// [kbeg, kbeg + kc) of the column starting at cell X0 into dst, packed per cell: 16-byte
+// chunks, 64 per instruction, straight into LDS.  Chunks past the end re-read the first one.
+template <typename T>
+__device__ inline void stage_rows(const T* __restrict__ F, int X0, int kbeg, int kc, int off, int per, T* dst) {
+  const int cpc = per * (int)sizeof(T) / 16;          // chunks per cell
+  const int total = kc * cpc;
+  const char* base = reinterpret_cast<const char*>(F + (size_t)(X0 + kbeg) * 180 + off);
+  char* d = reinterpret_cast<char*>(dst);
+  for (int j = 0; j * 64 < total; ++j) {
+    const int c = j * 64 + (int)threadIdx.x;
+    const int cc = c < total ? c : 0;
+    const int cell = cc / cpc, q = cc - cell * cpc;
+    const char* src = base + (size_t)cell * 180 * sizeof(T) + (size_t)q * 16;
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(d + j * 1024), 16, 0, 0);
+  }
+}
+
+// The apply's blocks are one wavefront, whose LDS instructions execute in issue order, so LDS
+// traffic between its lanes needs only a compiler ordering point (no s_barrier, and no wait for
+// the staged global-to-LDS copies still in flight).
+static_assert(PW == 64, "the apply kernel assumes one-wavefront blocks");
+__device__ inline void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+// Lane `src`'s value in every lane (v_readlane into scalar registers, no LDS round trip).
+__device__ inline double lane_value(double v, int src) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Wait for this wavefront's staged copies, then barrier (one-wavefront blocks).
+__device__ inline void stage_wait() {
+  __builtin_amdgcn_s_waitcnt(0);
+  lds_barrier();
+}
+
+// ------------------------------------------------------------------ z-line column solve
+// Shared by the z-line apply and the line Gauss-Seidel half-sweeps.  xs[k][6] holds the column's
+// right-hand side in LDS and is overwritten with M~_line^-1 xs.  All 64 lanes call; lanes 0..5
+// own the rows.  zline_stage_first starts moving the first forward segment and may be issued
+// before the caller fills xs.
+
+template <typename T>
+__device__ inline void zline_stage_first(const T* __restrict__ F, int X0, int Nz, T* fb0) {
+  stage_rows<T>(F, X0, 0, min(ZSEG, Nz), 0, 72, fb0);
+}
+
+template <typename T>
+__device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, double* xs, T* fb0, T* fb1) {
+  const int lane = threadIdx.x;
+  const bool active = lane < 6;
+  const int r = active ? lane : 0;
+  const int nseg = (Nz + ZSEG - 1) / ZSEG;
+  // the last two cells' vectors, wave-uniform (every lane holds all six components)
+  double zm1[6], zm2[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) { zm1[c] = 0.0; zm2[c] = 0.0; }
+  // forward: z_k = b_k - L1 z_{k-1} - L2 z_{k-2}
+  for (int sg = 0; sg < nseg; ++sg) {
+    const int k0 = sg * ZSEG, kc = min(ZSEG, Nz - k0);
+    T* cur = (sg & 1) ? fb1 : fb0;
+    stage_wait();
+    if (sg + 1 < nseg) stage_rows<T>(F, X0, k0 + ZSEG, min(ZSEG, Nz - k0 - ZSEG), 0, 72, (sg & 1) ? fb0 : fb1);
+    // row r of L2 at [0, 36), of L1 at [36, 72) of each cell; the next cell's rows and right-hand
+    // side are read while the current cell is computed
+    T l1[6], l2[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { l1[c] = cur[r * 6 + 36 + c]; l2[c] = cur[r * 6 + c]; }
+    double bk = xs[k0 * 6 + r];
+    for (int kk = 0; kk < kc; ++kk) {
+      const int k = k0 + kk;
+      const int kn = kk + 1 < kc ? kk + 1 : kk;
+      T n1[6], n2[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) { n1[c] = cur[kn * 72 + r * 6 + 36 + c]; n2[c] = cur[kn * 72 + r * 6 + c]; }
+      const double bn = xs[(k0 + kn) * 6 + r];
+      double s1 = 0.0, s2 = 0.0;   // the factor holds zero blocks where k < 2
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        s1 = fma((double)l1[c], zm1[c], s1);
+        s2 = fma((double)l2[c], zm2[c], s2);
+      }
+      const double zk = (bk - s1) - s2;
+      if (active) xs[k * 6 + r] = zk;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        zm2[c] = zm1[c];
+        zm1[c] = lane_value(zk, c);
+        l1[c] = n1[c];
+        l2[c] = n2[c];
+      }
+      bk = bn;
+    }
+  }
+  // backward: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2}); rows Dinv | U1 | U2 per cell
+  {
+    const int k0 = (nseg - 1) * ZSEG;
+    stage_wait();   // the forward sweep is done with both buffers
+    stage_rows<T>(F, X0, k0, Nz - k0, 72, 108, fb0);
+  }
+  double xp1[6], xp2[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) { xp1[c] = 0.0; xp2[c] = 0.0; }
+  for (int sg = nseg - 1, b = 0; sg >= 0; --sg, b ^= 1) {
+    const int k0 = sg * ZSEG, kc = min(ZSEG, Nz - k0);
+    T* cur = b ? fb1 : fb0;
+    stage_wait();
+    if (sg > 0) stage_rows<T>(F, X0, k0 - ZSEG, ZSEG, 72, 108, b ? fb0 : fb1);
+    // row r of Dinv at [0, 36), U1 at [36, 72), U2 at [72, 108) of each cell; the next (lower)
+    // cell's rows and z are read while the current cell is computed
+    T dr[6], u1[6], u2[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const T* R = cur + (kc - 1) * 108 + r * 6;
+      dr[c] = R[c]; u1[c] = R[36 + c]; u2[c] = R[72 + c];
+    }
+    double zk = xs[(k0 + kc - 1) * 6 + r];
+    for (int kk = kc - 1; kk >= 0; --kk) {
+      const int k = k0 + kk;
+      const int kn = kk > 0 ? kk - 1 : kk;
+      const T* Rn = cur + kn * 108 + r * 6;
+      T nd[6], n1[6], n2[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) { nd[c] = Rn[c]; n1[c] = Rn[36 + c]; n2[c] = Rn[72 + c]; }
+      const double zn = xs[(k0 + kn) * 6 + r];
+      double s1 = 0.0, s2 = 0.0;   // zero blocks where k + 2 > Nz - 1
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        s1 = fma((double)u1[c], xp1[c], s1);
+        s2 = fma((double)u2[c], xp2[c], s2);
+      }
+      const double tk = (zk - s1) - s2;
+      double tt[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) tt[c] = lane_value(tk, c);
+      double xk = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) xk = fma((double)dr[c], tt[c], xk);
+      if (active) xs[k * 6 + r] = xk;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        xp2[c] = xp1[c];
+        xp1[c] = lane_value(xk, c);
+        dr[c] = nd[c];
+        u1[c] = n1[c];
+        u2[c] = n2[c];
+      }
+      zk = zn;
+    }
+  }
+  lds_barrier();
+}
+
+// ------------------------------------------------------------------ z-line / block-Jacobi apply
+// One z-column per 64-lane block; wall rows (point Jacobi) in blocks of their own.
+template <typename T>
+__global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int nw, int kind,
+                                                   const T* __restrict__ F, const double* __restrict__ Fw,
+                                                   const double* __restrict__ s0, ApplyArgs a,
+                                                   double* __restrict__ partial, unsigned* counter) {
+  extern __shared__ __align__(16) double lds[];    // xs[Nz][6] | 8 spare | two staging buffers
+  __shared__ double sh[1];
+  __shared__ bool slast;
+  if (a.respect_done && a.gs->done) return;
+  const double sc = a.in_scale ? *a.in_scale : 1.0;
+  double local = 0.0;
+  const int lane = threadIdx.x;
+  if ((int)blockIdx.x < ncol) {
+    const int X0 = blockIdx.x * Nz;
+    double* xs = lds;
+    T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);
+    T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
+    const int cnt = 6 * Nz;
+    const bool zl = kind == VTSA_PREC_ZLINE;
+    if (zl) zline_stage_first<T>(F, X0, Nz, fb0);
+    // 1. b = (pre * scale * in) / s0.  store_in may alias in, so every load of a chunk of AC
+    //    elements is issued before the chunk's stores.
+    for (int e0 = lane; e0 < cnt; e0 += PW * AC) {
+      double xin[AC], xpre[AC], xs0[AC];
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int e = e0 + j * PW < cnt ? e0 + j * PW : 0;
+        const int r = e / Nz, k = e - r * Nz;
+        const int idx = r * n + X0 + k;
+        xin[j] = a.in[idx];
+        xpre[j] = a.pre ? a.pre[idx] : 1.0;
+        xs0[j] = s0[idx];
+      }
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int e = e0 + j * PW;
+        if (e < cnt) {
+          const int r = e / Nz, k = e - r * Nz;
+          double x = xin[j] * sc;
+          if (a.store_in) a.store_in[r * n + X0 + k] = x;
+          if (a.pre) x = x * xpre[j];
+          xs[k * 6 + r] = x / xs0[j];
+        }
+      }
+    }
+    if (zl) {
+      zline_sweeps<T>(F, X0, Nz, xs, fb0, fb1);
+    } else {
+      // block Jacobi: one lane per cell
+      lds_barrier();
+      for (int k = lane; k < Nz; k += PW) {
+        const T* Di = F + (size_t)(X0 + k) * 180 + 72;
+        double bv[6], xv[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) bv[c] = xs[k * 6 + c];
+#pragma unroll
+        for (int rr = 0; rr < 6; ++rr) {
+          double acc = 0.0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc += (double)Di[rr * 6 + c] * bv[c];
+          xv[rr] = acc;
+        }
+#pragma unroll
+        for (int rr = 0; rr < 6; ++rr) xs[k * 6 + rr] = xv[rr];
+      }
+      lds_barrier();
+    }
+    // 2. out = x s0 / post and the norm of out / red_s (loads of a chunk before its stores)
+    for (int e0 = lane; e0 < cnt; e0 += PW * AC) {
+      double xs0[AC], xpost[AC], xred[AC];
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int e = e0 + j * PW < cnt ? e0 + j * PW : 0;
+        const int r = e / Nz, k = e - r * Nz;
+        const int idx = r * n + X0 + k;
+        xs0[j] = s0[idx];
+        xpost[j] = a.post ? a.post[idx] : 1.0;
+        xred[j] = a.red_s ? a.red_s[idx] : 1.0;
+      }
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int e = e0 + j * PW;
+        if (e < cnt) {
+          const int r = e / Nz, k = e - r * Nz;
+          double o = xs[k * 6 + r] * xs0[j];
+          if (a.post) o = o / xpost[j];
+          a.out[r * n + X0 + k] = o;
+          double q = a.red_s ? o / xred[j] : o;
+          local += q * q;
+        }
+      }
+    }
+  } else {
+    // wall rows: point Jacobi
+    const int wb = blockIdx.x - ncol;
+    const int nwb = gridDim.x - ncol;
+    for (int w0 = wb * PW + lane; w0 < nw; w0 += nwb * PW * AC) {
+      double xin[AC], xpre[AC], xfw[AC], xpost[AC], xred[AC];
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int w = w0 + j * nwb * PW;
+        const int wi = w < nw ? w : 0;
+        const int idx = 6 * n + wi;
+        xin[j] = a.in[idx];
+        xpre[j] = a.pre ? a.pre[idx] : 1.0;
+        xfw[j] = Fw[wi];
+        xpost[j] = a.post ? a.post[idx] : 1.0;
+        xred[j] = a.red_s ? a.red_s[idx] : 1.0;
+      }
+#pragma unroll
+      for (int j = 0; j < AC; ++j) {
+        const int w = w0 + j * nwb * PW;
+        if (w < nw) {
+          const int idx = 6 * n + w;
+          double x = xin[j] * sc;
+          if (a.store_in) a.store_in[idx] = x;
+          if (a.pre) x = x * xpre[j];
+          double o = x / xfw[j];
+          if (a.post) o = o / xpost[j];
+          a.out[idx] = o;
+          double q = a.red_s ? o / xred[j] : o;
+          local += q * q;
+        }
+      }
+    }
+  }
+  double bs = wave_block_sum(local, sh);
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
+    const int N = 6 * n + nw;
+    double rms = sqrt(tot / (double)N);
+    if (a.red_out) *a.red_out = rms;
+    if (a.gs) {
+      a.gs->unorm = rms;
+      a.gs->zero_u = (rms == 0.0) ? 1 : 0;
+      a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * a.zs_norm) / rms;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ line Gauss-Seidel apply
+// Red/black (in general: greedy-coloured) Gauss-Seidel over lines: the z-lines of the cells
+// (solved exactly by the z-line factor) and the wall lines (tridiagonal along z, solved by the
+// Thomas algorithm).  Lines of one colour do not couple, so each colour is one launch.  The
+// sequence of colours for S sweeps is 0, 1, .., C-1, C-2, .., 0, then 1, .., 0 again per extra
+// sweep (a colour repeated back to back would reproduce its own result).  The iterate is kept
+// unscaled, U = S0 x~, so the couplings to other lines need only J and U:
+//     (M~ x~)_R = x~_R - (dh / s0_R) sum_C J_RC U_C      (R, C in different lines)
+struct GsArgs {
+  const double* Jc; const double* Jcw; const double* Jw; const double* Jwc; const double* s0;
+  double* bs; double* U; const int* order; double dh; int first;
+};
+
+// b~ = (pre * scale * in) / s0 and U = 0 (store_in as in the z-line apply; may alias in).
+__global__ void k_gs_gather(int N, ApplyArgs a, const double* __restrict__ s0, double* __restrict__ bs,
+                            double* __restrict__ U) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.respect_done && a.gs->done) return;
+  if (i >= N) return;
+  const double sc = a.in_scale ? *a.in_scale : 1.0;
+  double x = a.in[i] * sc;
+  if (a.store_in) a.store_in[i] = x;
+  if (a.pre) x = x * a.pre[i];
+  bs[i] = x / s0[i];
+  U[i] = 0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw,
+                                                const T* __restrict__ F, GsArgs g, int cbeg,
+                                                const GmresScalars* __restrict__ gs, int respect_done) {
+  extern __shared__ __align__(16) double lds[];
+  if (respect_done && gs->done) return;
+  const int lane = threadIdx.x;
+  const int L = g.order[cbeg + blockIdx.x];
+  double* xs = lds;
+  const double dh = g.dh;
+  if (L < ncol) {
+    // cell z-line (i, j)
+    const int i = L / Nth, j = L - (L / Nth) * Nth;
+    const int X0 = L * Nz;
+    T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);
+    T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
+    zline_stage_first<T>(F, X0, Nz, fb0);
+    const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
+    // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
+    const int nb[4] = {i > 0 ? ((i - 1) * Nth + j) * Nz : -1, i + 1 < Nr ? ((i + 1) * Nth + j) * Nz : -1,
+                       Nth > 1 ? (i * Nth + jm) * Nz : -1, Nth > 1 ? (i * Nth + jp) * Nz : -1};
+    const bool wall = nw > 0 && i == Nr - 1;
+    for (int e = lane; e < 6 * Nz; e += PW) {
+      const int r = e / Nz, k = e - r * Nz;
+      const int X = X0 + k;
+      double acc = 0.0;
+      if (!g.first) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (nb[q] < 0) continue;
+          const double* Jr = g.Jc + ((size_t)X * NSLOT + S_RM1 + q) * 36 + r * 6;
+          const int Y = nb[q] + k;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc += Jr[c] * g.U[c * n + Y];
+        }
+        if (wall && (r <= 2 || r == 5)) {
+          const int W = j * Nz + k;
+          acc += g.Jcw[W * 4 + (r <= 2 ? r : 3)] * g.U[6 * n + W];
+        }
+      }
+      const int idx = r * n + X;
+      xs[k * 6 + r] = g.bs[idx] + (dh / g.s0[idx]) * acc;
+    }
+    zline_sweeps<T>(F, X0, Nz, xs, fb0, fb1);
+    for (int e = lane; e < 6 * Nz; e += PW) {
+      const int r = e / Nz, k = e - r * Nz;
+      const int idx = r * n + X0 + k;
+      g.U[idx] = g.s0[idx] * xs[k * 6 + r];
+    }
+  } else {
+    // wall line j: tridiagonal along z (axial conduction), couplings to the outer cells and to
+    // the neighbouring wall lines moved to the right-hand side
+    const int j = L - ncol;
+    const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
+    double* rhs = xs;          // [Nz]
+    double* cp = xs + Nz;      // [Nz] Thomas coefficients
+    for (int k = lane; k < Nz; k += PW) {
+      const int W = j * Nz + k;
+      const int X = ((Nr - 1) * Nth + j) * Nz + k;
+      double acc = 0.0;
+      if (!g.first) {
+        if (Nth > 1) acc += g.Jw[W * 5 + 3] * g.U[6 * n + jm * Nz + k] + g.Jw[W * 5 + 4] * g.U[6 * n + jp * Nz + k];
+#pragma unroll
+        for (int v = 0; v < 6; ++v) acc += g.Jwc[W * 6 + v] * g.U[v * n + X];
+      }
+      rhs[k] = g.bs[6 * n + W] + (dh / g.s0[6 * n + W]) * acc;
+    }
+    lds_barrier();
+    if (lane == 0) {
+      const double* s0w = g.s0 + 6 * n + j * Nz;
+      const double* Jwl = g.Jw + (size_t)j * Nz * 5;
+      double cprev = 0.0, dprev = 0.0;
+      for (int k = 0; k < Nz; ++k) {
+        const double diag = 1.0 - dh * Jwl[k * 5 + 0];
+        const double lo = k > 0 ? (-dh * Jwl[k * 5 + 1]) * (s0w[k - 1] / s0w[k]) : 0.0;
+        const double up = k + 1 < Nz ? (-dh * Jwl[k * 5 + 2]) * (s0w[k + 1] / s0w[k]) : 0.0;
+        const double den = diag - lo * cprev;
+        cprev = up / den;
+        dprev = (rhs[k] - lo * dprev) / den;
+        cp[k] = cprev;
+        rhs[k] = dprev;
+      }
+      double x = 0.0;
+      for (int k = Nz - 1; k >= 0; --k) {
+        x = rhs[k] - cp[k] * x;
+        rhs[k] = x;
+      }
+    }
+    lds_barrier();
+    for (int k = lane; k < Nz; k += PW) {
+      const int idx = 6 * n + j * Nz + k;
+      g.U[idx] = g.s0[idx] * rhs[k];
+    }
+  }
+}
+
+// out = U / post and the norm of out / red_s (the apply's side outputs).
+__global__ void __launch_bounds__(RBLK) k_gs_final(int N, ApplyArgs a, const double* __restrict__ U,
+                                                   double* __restrict__ partial, unsigned* counter) {
+  __shared__ double sh[RBLK / 64];
+  __shared__ bool slast;
+  if (a.respect_done && a.gs->done) return;
+  double local = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    double o = U[i];
+    if (a.post) o = o / a.post[i];
+    a.out[i] = o;
+    double q = a.red_s ? o / a.red_s[i] : o;
+    local += q * q;
+  }
+  double bs = wave_block_sum(local, sh);
+  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
+  double tot;
+  if (last_block_total(counter, &slast, partial, &tot)) {
+    double rms = sqrt(tot / (double)N);
+    if (a.red_out) *a.red_out = rms;
+    if (a.gs) {
+      a.gs->unorm = rms;
+      a.gs->zero_u = (rms == 0.0) ? 1 : 0;
+      a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * a.zs_norm) / rms;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host side (preconditioner)
+// Greedy colouring of the line graph (cell z-lines (i, j) couple to (i +- 1, j) and (i, j +- 1);
+// wall line j couples to cell line (Nr - 1, j) and wall lines j +- 1).  Lines are visited in
+// index order, so an even Ntheta gives the red/black pattern (i + j) mod 2.
+void gs_colour_lines(int Nr, int Nth, int nwall_lines, std::vector<int>& order, std::vector<int>& cstart) {
+  const int ncol = Nr * Nth;
+  const int nl = ncol + nwall_lines;
+  std::vector<int> colour(nl, -1);
+  auto neighbours = [&](int L, std::vector<int>& out) {
+    out.clear();
+    if (L < ncol) {
+      const int i = L / Nth, j = L % Nth;
+      if (i > 0) out.push_back(L - Nth);
+      if (i + 1 < Nr) out.push_back(L + Nth);
+      if (Nth > 1) {
+        out.push_back(i * Nth + (j == 0 ? Nth - 1 : j - 1));
+        out.push_back(i * Nth + (j + 1 == Nth ? 0 : j + 1));
+      }
+      if (nwall_lines > 0 && i == Nr - 1) out.push_back(ncol + j);
+    } else {
+      const int j = L - ncol;
+      out.push_back((Nr - 1) * Nth + j);
+      if (Nth > 1) {
+        out.push_back(ncol + (j == 0 ? Nth - 1 : j - 1));
+        out.push_back(ncol + (j + 1 == Nth ? 0 : j + 1));
+      }
+    }
+  };
+  std::vector<int> nbr;
+  int ncolour = 0;
+  for (int L = 0; L < nl; ++L) {
+    neighbours(L, nbr);
+    for (int c = 0;; ++c) {
+      bool used = false;
+      for (int M : nbr)
+        if (M != L && colour[M] == c) used = true;
+      if (!used) { colour[L] = c; if (c + 1 > ncolour) ncolour = c + 1; break; }
+    }
+  }
+  order.clear();
+  cstart.assign(ncolour + 1, 0);
+  for (int c = 0; c < ncolour; ++c) {
+    cstart[c] = (int)order.size();
+    for (int L = 0; L < nl; ++L)
+      if (colour[L] == c) order.push_back(L);
+  }
+  cstart[ncolour] = (int)order.size();
+}
+
+void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, int sweeps, hipStream_t s) {
+  (void)hipMemcpyAsync(h.s0, scale, sizeof(double) * h.N, hipMemcpyDeviceToDevice, s);
+  const int fkind = kind == VTSA_PREC_BLOCK_JACOBI ? VTSA_PREC_BLOCK_JACOBI : VTSA_PREC_ZLINE;
+  const int nb = (h.ncol + PCOLS - 1) / PCOLS;
+  if (prec == VTSA_FP32)
+    hipLaunchKernelGGL(k_prec_factor<float>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, fkind, h.Jc, h.s0, dh,
+                       (float*)h.F);
+  else
+    hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, fkind, h.Jc, h.s0,
+                       dh, (double*)h.F);
+  if (h.nw > 0) hipLaunchKernelGGL(k_prec_wall, dim3((h.nw + 255) / 256), dim3(256), 0, s, h.nw, h.Jw, dh, h.Fw);
+  h.prec_kind = kind;
+  h.prec_prec = prec;
+  h.prec_dh = dh;
+  h.gs_sweeps = sweeps;
+  h.prec_ready = 1;
+}
+
+// The apply's LDS (FP64 factors, the larger case) must fit the 64 KiB a block may use.
+bool zline_fits(int Nz) { return zline_lds_bytes<double>(Nz) <= 65536; }
+
+size_t partial_size(int ncol, int nw) {
+  size_t a = (size_t)RMAXB * (MAX_RESTART + 2);
+  size_t pg = (size_t)ncol + (nw + PW * 4 - 1) / (PW * 4) + 1;   // apply grid: one block per column
+  if (pg < (size_t)RMAXB) pg = RMAXB;                            // line Gauss-Seidel final reduction
+  size_t b = (size_t)RMAXB * 8 + pg;
+  return a > b ? a : b;
+}
+
+template <typename T>
+static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
+  const int N = h.N, Nz = h.desc.Nz;
+  hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
+  GsArgs g{h.Jc, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1};
+  const int C = (int)h.gs_cstart.size() - 1;
+  std::vector<int> seq;
+  for (int c = 0; c < C; ++c) seq.push_back(c);
+  for (int c = C - 2; c >= 0; --c) seq.push_back(c);
+  for (int sw = 1; sw < h.gs_sweeps; ++sw) {
+    for (int c = 1; c < C; ++c) seq.push_back(c);
+    for (int c = C - 2; c >= 0; --c) seq.push_back(c);
+  }
+  for (size_t q = 0; q < seq.size(); ++q) {
+    const int c = seq[q];
+    g.first = q == 0 ? 1 : 0;
+    const int cnt = h.gs_cstart[c + 1] - h.gs_cstart[c];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_gs_half<T>, dim3(cnt), dim3(PW), zline_lds_bytes<T>(Nz), s, h.n, h.desc.Nr, h.desc.Nth, Nz,
+                       h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+  }
+  hipLaunchKernelGGL(k_gs_final, dim3(rgrid(N)), dim3(RBLK), 0, s, N, a, h.gsu, h.red.partial + RMAXB * 8,
+                     h.red.counter + 1);
+}
+
+void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
+  const int Nz = h.desc.Nz;
+  if (h.prec_kind == VTSA_PREC_ZLINE_GS) {
+    if (h.prec_prec == VTSA_FP32) gs_launch<float>(h, a, s);
+    else gs_launch<double>(h, a, s);
+    return;
+  }
+  const int nwb = h.nw > 0 ? (h.nw + PW * 4 - 1) / (PW * 4) : 0;
+  const int grid = h.ncol + nwb;
+  if (h.prec_prec == VTSA_FP32)
+    hipLaunchKernelGGL(k_prec_apply<float>, dim3(grid), dim3(PW), zline_lds_bytes<float>(Nz), s, h.n, Nz, h.ncol,
+                       h.nw, h.prec_kind, (const float*)h.F, h.Fw, h.s0, a, h.red.partial + RMAXB * 8,
+                       h.red.counter + 1);
+  else
+    hipLaunchKernelGGL(k_prec_apply<double>, dim3(grid), dim3(PW), zline_lds_bytes<double>(Nz), s, h.n, Nz, h.ncol,
+                       h.nw, h.prec_kind, (const double*)h.F, h.Fw, h.s0, a, h.red.partial + RMAXB * 8,
+                       h.red.counter + 1);
+}
+
+void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,
+                       const double* post, double* out, bool sigma_for_matvec, double zs_norm, hipStream_t s) {
+  ApplyArgs a{in, in_scale, store_in, pre, post, out, nullptr, nullptr, sigma_for_matvec ? h.gs : nullptr, zs_norm,
+              sigma_for_matvec ? 1 : 0};
+  apply_launch(h, a, s);
+}
+
+}  // namespace vtsa

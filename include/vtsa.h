@@ -28,7 +28,7 @@ enum { VTSA_COADS_NONE = 0, VTSA_COADS_WADST = 1, VTSA_COADS_MECHANISTIC = 2, VT
 enum { VTSA_DISP_EDWARDS_RICHARDSON = 0, VTSA_DISP_YOUNG2021 = 1, VTSA_DISP_WAKAO_KAGEI = 2 };
 enum { VTSA_BED_ZS = 0, VTSA_BED_PARALLEL = 1, VTSA_BED_CONSTANT = 2 };
 enum { VTSA_WALLHTC_MN = 0, VTSA_WALLHTC_CONSTANT = 1 };
-enum { VTSA_PREC_BLOCK_JACOBI = 0, VTSA_PREC_ZLINE = 1 };
+enum { VTSA_PREC_BLOCK_JACOBI = 0, VTSA_PREC_ZLINE = 1, VTSA_PREC_ZLINE_GS = 2 };
 enum { VTSA_FP64 = 0, VTSA_FP32 = 1 };
 
 /* Everything the residual of one cycle step depends on, as scalars.  Filled by the host from
@@ -116,10 +116,13 @@ int vtsa_fd_jacobian(vtsa_handle* h, double t, const double* y, const double* f0
 int vtsa_jacobian_get(vtsa_handle* h, double* Jc, double* Jcw, double* Jw, double* Jwc);
 
 /* Factor the preconditioner M ~ I - dh J in the variables scaled by `scale` (device, length N).
- * Replaces TRBDF2._factor / splu (core/solver.py:334-337).  kind: VTSA_PREC_*; precision:
- * VTSA_FP64 or VTSA_FP32 (factors stored and applied in FP32, vectors stay FP64). */
+ * Replaces TRBDF2._factor / splu (core/solver.py:334-337).  kind: VTSA_PREC_BLOCK_JACOBI (6x6
+ * cell blocks), VTSA_PREC_ZLINE (block LU of every z-line) or VTSA_PREC_ZLINE_GS (line
+ * Gauss-Seidel over the z-lines of the cells and the wall lines, coloured so that lines of one
+ * colour do not couple; `sweeps` symmetric sweeps, 1..16, ignored by the other kinds);
+ * precision: VTSA_FP64 or VTSA_FP32 (factors stored and applied in FP32, vectors stay FP64). */
 int vtsa_precond_setup(vtsa_handle* h, double dh, const double* scale, int32_t kind, int32_t precision,
-                       vtsa_stream s);
+                       int32_t sweeps, vtsa_stream s);
 /* out = M^-1 r (unscaled); replaces lu.solve (core/solver.py:182,209,460).  out may alias r. */
 int vtsa_precond_apply(vtsa_handle* h, const double* r, double* out, vtsa_stream s);
 
