@@ -4,16 +4,18 @@ Workload (the metric's single-GPU configuration): grid Nr=10, Ntheta=16, Nz=50 (
 48,800 unknowns), seeded mid-cycle state (seed 0: T~U(288,373) K, P~U(1.5e4,1.013e5) Pa,
 y_CO2~U(0,0.9), y_H2O~U(0,0.05), q_CO2~U(0,1.6), q_H2O~U(0,6) mol/kg), one full heating step of
 22,400 s (jacket 373.15 K, product vacuum pump at 1.5e4 Pa), FP64 residual / Jacobian / Newton /
-GMRES with the z-line preconditioner (``--precision fp32`` stores the preconditioner in FP32).
+GMRES with the line Gauss-Seidel preconditioner (``--precision fp32`` stores its factors in FP32).
 
 One bench "step" = one complete heating step through the drop-in ``run_step`` (outlet recording
 and balances included, as upstream), from the same seeded state.  value = steps x 22,400 s /
 wall seconds of the timed region.
 
-Also reported: the per-step mass/energy balance errors, the roofline of the dominant kernel
-(the finite-difference matvec residual, timed with HIP events on the library's stream), and a
-CPU baseline: the NumPy/SciPy oracle (same algorithms as upstream, 1 thread) on the same state
-for a bounded wall-clock sample, with the GPU time over the same simulated window.
+Also reported: the per-step mass/energy balance errors; the roofline of the dominant kernel, the
+line Gauss-Seidel half-sweep, from every 8th of its launches in the timed region bracketed by HIP
+events on its stream (algorithmic bytes per launch / average launch time); the residual kernel
+pair timed the same way as a secondary figure; and a CPU baseline: the NumPy/SciPy oracle (same
+algorithms as upstream, 1 thread) on the same state for a bounded wall-clock sample, with the GPU
+wall time over the same simulated window.
 """
 from __future__ import annotations
 
@@ -35,12 +37,12 @@ STEP_INDEX = 2          # heating in the default cycle
 SEED = 0
 
 
-def setup(precision):
+def setup(precision, overrides=None):
     from dac_vtsa_amd.config import load_config
     from dac_vtsa_amd.core.grid import Grid
     from dac_vtsa_amd.properties import get_backend
     from dac_vtsa_amd.seeds import seeded_state
-    cfg = load_config(override_dict=WORKLOAD)
+    cfg = load_config(override_dict={**WORKLOAD, **(overrides or {})})
     grid = Grid.from_config(cfg)
     props = get_backend(cfg)
     state = seeded_state(grid, "midcycle", SEED)
@@ -51,16 +53,28 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def one_step(cfg, grid, props, state, precision):
+def one_step(cfg, grid, props, state, precision, sweeps=2, conserve="polish"):
     from dac_vtsa_amd.steps import run_step
     return run_step(state, cfg, grid, props, cfg.cycle[STEP_INDEX], logger=_log, progress_s=20.0,
-                    integrator_kw={"precision": precision})
+                    integrator_kw={"precision": precision, "gs_sweeps": sweeps, "conserve": conserve})
 
 
-def kernel_roofline(cfg, grid, props, state, n_launch=200):
-    """Average duration of one residual evaluation (k_props + k_flux, the kernel pair every
-    GMRES matvec, line-search point and error estimate launches), timed with HIP events
-    recorded by libvtsa on the stream it launches on, and its algorithmic traffic."""
+def copy_probe(mib=64, repeat=5):
+    """Calibration for the HBM traffic counters: `repeat` device-to-device copies of `mib` MiB
+    (the runtime's copy kernel reads and writes exactly that many bytes per dispatch)."""
+    from dac_vtsa_amd import device as dv
+    n = mib * (1 << 20) // 8
+    a, b = dv.zeros(n), dv.empty(n)
+    for _ in range(repeat):
+        b.copy_from(a)
+    dv.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_dispatch": n * 8, "dispatches": repeat}), flush=True)
+
+
+def residual_roofline(cfg, grid, props, state, n_launch=200):
+    """Average duration of one residual evaluation (k_props + k_flux, the kernel pair of every
+    line-search point and error estimate), timed with HIP events recorded by libvtsa on the
+    stream it launches on, and its algorithmic traffic (read y, write f)."""
     import ctypes as C
     from dac_vtsa_amd import _abi
     from dac_vtsa_amd import device as dv
@@ -79,21 +93,47 @@ def kernel_roofline(cfg, grid, props, state, n_launch=200):
     return avg_s, algo_bytes
 
 
-def cpu_baseline(cfg, state, budget_s):
-    """Oracle integration of the same step on 1 host thread for a bounded wall-clock budget."""
+def gpu_window(cfg, grid, props, state, t_end, precision, sweeps=2, conserve="polish"):
+    """GPU wall seconds for the first t_end simulated seconds of the step (same integrator)."""
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.residual import TVSAModel
+    from dac_vtsa_amd.core.solver import TRBDF2
+    step = cfg.cycle[STEP_INDEX]
+    m = TVSAModel(cfg, grid, props, step, state)
+    integ = TRBDF2(m, cfg, h_max=cfg.solver.h_max_fraction * float(step["duration"]), precision=precision,
+                   gs_sweeps=sweeps, conserve=conserve)
+    dv.synchronize()
+    t0 = time.time()
+    integ.integrate(m.pack(state), t_end, np.array([t_end]))
+    dv.synchronize()
+    wall = time.time() - t0
+    m.close()
+    return wall
+
+
+def cpu_baseline(cfg, state, budget_s, sr):
+    """The oracle (NumPy/SciPy, the upstream algorithms) on 1 host thread, two windows of
+    budget_s / 2 wall seconds each: from the start of the heating step, and from the GPU
+    trajectory's snapshot nearest mid-step (with the GPU's step size there as the first step).
+    Returns [(t_start, simulated seconds, wall seconds, accepted steps)] per window."""
     from oracle.model import OracleModel
     from oracle.runner import make
     from oracle.solver import OracleTRBDF2
     g, gas = make(cfg)
     step = cfg.cycle[STEP_INDEX]
-    m = OracleModel(cfg, g, gas, step, state)
-    y0 = m.pack(state)
     dur = float(step["duration"])
-    integ = OracleTRBDF2(m, cfg, h_max=cfg.solver.h_max_fraction * dur)
-    t0 = time.time()
-    res = integ.integrate(y0, dur, np.array([dur]), stop_fn=lambda t, y: time.time() - t0, stop_value=budget_s)
-    wall = time.time() - t0
-    return res.t, wall, res.n_steps
+    mid = min(sr.snapshots, key=lambda sn: abs(sn["t_step"] - 0.5 * dur))
+    t_mid = float(mid["t_step"])
+    h_mid = min(sr.steps_taken, key=lambda th: abs(th[0] - t_mid))[1]
+    out = []
+    for t0, st, h0 in ((0.0, state, None), (t_mid, mid["state"], h_mid)):
+        m = OracleModel(cfg, g, gas, step, state)     # the step's model is built from its initial state
+        integ = OracleTRBDF2(m, cfg, h_max=cfg.solver.h_max_fraction * dur, h_init=h0)
+        w0 = time.time()
+        res = integ.integrate(m.pack(st), dur, np.array([dur]), stop_fn=lambda t, y: time.time() - w0,
+                              stop_value=0.5 * budget_s, t0=t0)
+        out.append((t0, res.t - t0, time.time() - w0, res.n_steps))
+    return out
 
 
 def balance_errors(sr, species):
@@ -111,34 +151,55 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-sample-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweeps", type=int, default=2, help="line Gauss-Seidel sweeps of the preconditioner")
+    ap.add_argument("--cons-tol", type=float, default=1e-5,
+                    help="solver.conservation_tol (upstream default 1e-3; the metric needs balances <= 1e-6)")
+    ap.add_argument("--rtol", type=float, default=None, help="solver.rtol (default: config)")
+    ap.add_argument("--conserve", choices=("polish", "project"), default="project",
+                    help="stage conservation: upstream polish, or the conservation projection first")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the traffic-counter calibration copy")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (independent cases would shard "
               "one group per GPU with no communication; no multi-GPU run is part of this build)", file=sys.stderr)
         return 2
     from dac_vtsa_amd import device as dv
-    cfg, grid, props, state = setup(args.precision)
+    over = {}
+    if args.cons_tol is not None:
+        over["solver.conservation_tol"] = args.cons_tol
+    if args.rtol is not None:
+        over["solver.rtol"] = args.rtol
+    cfg, grid, props, state = setup(args.precision, over)
     dur = float(cfg.cycle[STEP_INDEX]["duration"])
     print(f"[bench] workload 10x16x50 heating {dur:.0f} s, precision {args.precision}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         t0 = time.time()
-        one_step(cfg, grid, props, state, args.precision)
+        one_step(cfg, grid, props, state, args.precision, args.sweeps, args.conserve)
         print(f"[bench] warmup {i + 1}/{args.warmup}: {time.time() - t0:.2f} s", file=sys.stderr, flush=True)
+    from dac_vtsa_amd.core import residual
+    residual.reset_kernel_profile(8)
     dv.synchronize()
     t0 = time.time()
     results = []
     for i in range(args.steps):
-        _, sr = one_step(cfg, grid, props, state, args.precision)
+        _, sr = one_step(cfg, grid, props, state, args.precision, args.sweeps, args.conserve)
         results.append(sr)
         print(f"[bench] step {i + 1}/{args.steps}: {sr.solver['steps']} accepted steps, "
               f"{sr.solver['rejected']} rejected, {sr.solver.get('gmres_iters', 0)} GMRES iterations",
               file=sys.stderr, flush=True)
     dv.synchronize()
     elapsed = time.time() - t0
+    prof = dict(residual.KERNEL_PROFILE)
+    residual.reset_kernel_profile(0)
     value = args.steps * dur / elapsed
     mb, eb = balance_errors(results[-1], props.species)
-    avg_s, algo_bytes = kernel_roofline(cfg, grid, props, state)
-    achieved = algo_bytes / avg_s / 1e9
+    avg_s = prof["ms"] / 1e3 / max(prof["launches"], 1)
+    bytes_per_launch = prof["bytes"] / max(prof["launches"], 1)
+    achieved = bytes_per_launch / avg_s / 1e9
+    res_s, res_bytes = residual_roofline(cfg, grid, props, state)
     line = {
         "metric": "cycle sim-s/wall-s, 1 GPU: learned+FP32 vs plain FP64 at MB/EB err <=1e-6",
         "value": value, "unit": "sim-s/wall-s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -146,21 +207,32 @@ def main():
         "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 preconditioner)",
         "data": "synthetic seeded mid-cycle state (seed 0)",
         "config": {"workload": "heating step 22400 s, grid 10x16x50 (48800 unknowns), P_vac 1.5e4 Pa",
-                   "preconditioner": "zline", "precision": args.precision, "solver": "sdirk2 Newton-GMRES(30)"},
+                   "preconditioner": f"zline_gs ({args.sweeps} sweep{'s' if args.sweeps > 1 else ''})", "precision": args.precision,
+                   "solver": "sdirk2 Newton-GMRES(30)", "conserve": args.conserve, "rtol": cfg.solver.rtol,
+                   "conservation_tol": cfg.solver.conservation_tol},
         "balance": {"max_mass_rel_err": mb, "max_energy_rel_err": eb},
         "solver": {k: results[-1].solver.get(k) for k in ("steps", "rejected", "newton_fail", "jacobians", "lu",
-                                                          "gmres_iters", "nfev")},
-        "roofline": {"bound": "hbm", "kernel": "residual (k_props + k_flux)", "achieved": achieved,
+                                                          "gmres_iters", "gmres_polish", "balance_unclosed",
+                                                          "cons_projected", "nfev")},
+        "roofline": {"bound": "hbm", "kernel": "k_gs_half (line Gauss-Seidel half-sweep)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_us": avg_s * 1e6, "algo_bytes_per_launch": algo_bytes},
+                     "avg_launch_us": avg_s * 1e6, "algo_bytes_per_launch": bytes_per_launch,
+                     "sampled_launches": prof["launches"]},
+        "residual_pair": {"kernels": "k_props + k_flux", "avg_launch_us": res_s * 1e6,
+                          "algo_bytes_per_launch": res_bytes, "achieved_GBs": res_bytes / res_s / 1e9},
     }
     if not args.no_cpu:
         from threadpoolctl import threadpool_limits
         with threadpool_limits(1):
-            t_sim, wall, nsteps = cpu_baseline(cfg, state, args.cpu_sample_s)
-        line["cpu_baseline"] = {"value": t_sim / wall, "unit": "sim-s/wall-s", "cores": 1, "kind": "port",
-                                "sample": f"first {t_sim:.3g} s of the same heating step ({nsteps} accepted "
-                                          f"steps) in {wall:.1f} s wall on 1 host thread"}
+            wins = cpu_baseline(cfg, state, args.cpu_sample_s, results[-1])
+        (_, s0, w0, n0), (tm, s1, w1, n1) = wins
+        g_wall = gpu_window(cfg, grid, props, state, s0, args.precision, args.sweeps, args.conserve)
+        line["cpu_baseline"] = {"value": s1 / w1, "unit": "sim-s/wall-s", "cores": 1, "kind": "port",
+                                "sample": f"oracle on 1 host thread, two windows of the same heating step: "
+                                          f"from t={tm:.0f} s (GPU snapshot, mid-step) {s1:.3g} s simulated in "
+                                          f"{w1:.1f} s ({n1} steps, this rate is the value); from t=0 {s0:.3g} s "
+                                          f"in {w0:.1f} s ({n0} steps, {s0 / w0:.3g} sim-s/wall-s; the GPU runs "
+                                          f"that first window in {g_wall:.3f} s)"}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -82,6 +82,8 @@ _SIGS = {
     "vtsa_newton_try": [P, D, P, P, D, D, P, P, P, P, P, C.POINTER(D), C.POINTER(D), C.POINTER(D), P],
     "vtsa_cons": [P, P, P, P, P, P],
     "vtsa_inventories": [P, P, P, P],
+    "vtsa_cons_norm2": [P, P, P, P],
+    "vtsa_cons_shift": [P, P, P, P, P, P],
     "vtsa_vec_op": [P, I32, D, D, D, D, P, P, P, P, P, P],
     "vtsa_error_norm": [P, P, P, P, D, C.POINTER(D), P],
     "vtsa_device_count": [C.POINTER(I32)],
@@ -93,6 +95,8 @@ _SIGS = {
     "vtsa_fill": [P, D, C.c_size_t, P],
     "vtsa_sync": [P],
     "vtsa_time_rhs": [P, D, P, P, I32, C.POINTER(D), P],
+    "vtsa_profile": [P, I32],
+    "vtsa_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(D), C.POINTER(D)],
     "vtsa_version": [],
     "vtsa_last_error": [],
 }

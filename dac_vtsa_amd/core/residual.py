@@ -26,6 +26,15 @@ BED = {"zehner_schlunder": 0, "parallel": 1, "constant": 2}
 WALLHTC = {"martin_nilles": 0, "constant": 1}
 VALVE_INDEX = {"V1": 0, "V2": 1, "V3": 2}
 
+# Sampled kernel timing (no upstream counterpart): when KERNEL_PROFILE["every"] > 0, every model
+# times each every-th launch of its dominant kernel (the line Gauss-Seidel half-sweep) with HIP
+# events on the launch stream and adds launches / milliseconds / algorithmic bytes here on close.
+KERNEL_PROFILE = {"every": 0, "launches": 0, "ms": 0.0, "bytes": 0.0}
+
+
+def reset_kernel_profile(every: int):
+    KERNEL_PROFILE.update(every=int(every), launches=0, ms=0.0, bytes=0.0)
+
 
 class Thermo:
     """Conservative energy variable e <-> T (upstream physics/energy.py:31-70), host side."""
@@ -261,12 +270,20 @@ class TVSAModel:
         ptrs = [a.ctypes.data_as(C.c_void_p) for a in arrs]
         _abi.call("vtsa_create", C.byref(self.desc), *ptrs, C.byref(h))
         self._handle = h
+        if KERNEL_PROFILE["every"] > 0:
+            _abi.call("vtsa_profile", h, int(KERNEL_PROFILE["every"]))
         self._y = dv.empty(self.n_core)
         self._f = dv.empty(self.n_core)
         self._acc = dv.empty(self.n_acc)
 
     def close(self):
         if self._handle is not None:
+            if KERNEL_PROFILE["every"] > 0:
+                n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+                _abi.call("vtsa_profile_read", self._handle, C.byref(n), C.byref(ms), C.byref(b))
+                KERNEL_PROFILE["launches"] += n.value
+                KERNEL_PROFILE["ms"] += ms.value
+                KERNEL_PROFILE["bytes"] += b.value
             _abi.lib().vtsa_destroy(self._handle)
             self._handle = None
 

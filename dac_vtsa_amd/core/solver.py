@@ -119,6 +119,16 @@ class DeviceOps:
         _abi.call("vtsa_cons", self.h, dv.ptr(R), dv.ptr(s), wr, ws, self.st())
         return np.array(wr[:]), np.array(ws[:])
 
+    def cons_norm2(self, s):
+        out = (C.c_double * 4)()
+        _abi.call("vtsa_cons_norm2", self.h, dv.ptr(s), out, self.st())
+        return np.array(out[:])
+
+    def cons_shift(self, z, s, lam, out):
+        lam = (C.c_double * 4)(*[float(v) for v in lam])
+        _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, dv.ptr(out), self.st())
+        return out
+
     def error_norm(self, y, z2, y3, rtol):
         en = C.c_double()
         _abi.call("vtsa_error_norm", self.h, dv.ptr(y), dv.ptr(z2), dv.ptr(y3), float(rtol), C.byref(en), self.st())
@@ -148,9 +158,55 @@ class Factorisation:
         self.h = h
 
 
+def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3):
+    """Shift a converged stage value along S^2 W^T so that its residual conserves the four totals.
+
+    No upstream counterpart (``conserve="project"``).  The shift z(lambda) = z - S^2 W^T lambda is
+    the minimum-norm change in the s-scaled variables that moves the conserved totals W z; the
+    four multipliers solve W R(z(lambda)) = 0 by Newton's method on this 4x4 system, its matrix
+    B = d(W R)/d(lambda) taken by finite differences along the four directions (four residual
+    evaluations per iteration; rows with no entries are left out).  The shift is of the size of
+    the Newton tolerance.  Returns (z, f, R) or None if the totals do not close."""
+    d2 = ops.cons_norm2(s)
+    rows = np.nonzero(d2 > 0.0)[0]
+    zc, fc, Rc = z, f, R
+    zt, ft, Rt = dv.empty(ops.N), dv.empty(ops.N), dv.empty(ops.N)
+    for _ in range(iters):
+        wr, ws = ops.cons(Rc, s)
+        if np.all(np.abs(wr) <= cons_tol * ws):
+            return zc, fc, Rc
+        B = np.zeros((4, 4))
+        for l in rows:
+            eps = 1e-2 * ws[l] / d2[l]
+            lam = np.zeros(4)
+            lam[l] = eps
+            ops.cons_shift(zc, s, lam, zt)
+            rn, _ = ops.newton_eval(t, zt, dh, rhs_known, s, ft, Rt)
+            if not _finite(rn):
+                return None
+            wl, _ = ops.cons(Rt, s)
+            B[:, l] = (wl - wr) / eps
+        lam = np.zeros(4)
+        try:
+            lam[rows] = np.linalg.solve(B[np.ix_(rows, rows)], -wr[rows])
+        except np.linalg.LinAlgError:
+            return None
+        zn, fn, Rn = dv.empty(ops.N), dv.empty(ops.N), dv.empty(ops.N)
+        ops.cons_shift(zc, s, lam, zn)
+        rn, _ = ops.newton_eval(t, zn, dh, rhs_known, s, fn, Rn)
+        if not _finite(rn):
+            return None
+        zc, fc, Rc = zn, fn, Rn
+    wr, ws = ops.cons(Rc, s)
+    return (zc, fc, Rc) if np.all(np.abs(wr) <= cons_tol * ws) else None
+
+
 def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_dim=30, eta=1e-2, counter=None,
-                  cons_tol=0.0, stats=None, trace=None, refresh=None):
+                  cons_tol=0.0, stats=None, trace=None, refresh=None, project=False):
     """Inexact Newton-Krylov stage solve on the device (core/solver.py:154-294).
+
+    ``project`` (no upstream counterpart): when the converged value misses the conservation
+    test, try project_conservation before the upstream polish iterations.
 
     z0, rhs_known: device vectors.  Returns (z, ok, iterations, rate, new_lu)."""
     N = ops.N
@@ -218,6 +274,12 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
                 best = (rn, z.clone())
             if cons_ok(ops, cons_tol, R, ops.scale(z, rtol, s)):
                 return z, True, it + 1, rate, new_lu
+            if project and cons_tol > 0.0:
+                pr = project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol)
+                if pr is not None:
+                    if stats is not None:
+                        stats["cons_projected"] = stats.get("cons_projected", 0) + 1
+                    return pr[0], True, it + 1, rate, new_lu
             if polish < 2:
                 polish += 1
                 step_prev = None
@@ -242,13 +304,14 @@ class TRBDF2:
 
     Extra keyword arguments (no upstream counterpart): ``precond`` ("zline_gs" | "zline" |
     "block_jacobi"), ``gs_sweeps`` (symmetric line Gauss-Seidel sweeps of "zline_gs"),
-    ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``.  The
+    ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``, ``conserve``
+    ("polish": the upstream conservation polish; "project": project_conservation first).  The
     preconditioner stands in for the upstream SuperLU factor of the theta-block matrix wherever
     that factor is used: GMRES, the Newton residual norm and the error-estimate filter."""
 
     def __init__(self, model, cfg, h_max: float, h_init: Optional[float] = None, logger=None,
                  precond: str = "zline_gs", precision: str = "fp64", krylov_dim: int = 30,
-                 gs_sweeps: int = 1):
+                 gs_sweeps: int = 2, conserve: str = "polish"):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -277,6 +340,9 @@ class TRBDF2:
         self.precision = PRECISIONS[precision]
         self.krylov_dim = krylov_dim
         self.gs_sweeps = int(gs_sweeps)
+        if conserve not in ("polish", "project"):
+            raise ValueError(f"conserve={conserve!r}")
+        self.conserve = conserve
         self.cap_after_fail = 0.5
         self.cap_relax = 1.15
         self.ops = DeviceOps(model)
@@ -309,7 +375,8 @@ class TRBDF2:
         z, ok, it, rate, new_lu = newton_krylov(self.ops, t, z0, rhs_known, dh, lu, self.rtol, self.max_iter,
                                                 self.kappa, krylov_dim=self.krylov_dim, counter=self._gm,
                                                 cons_tol=self.cons_tol, stats=self.stats,
-                                                refresh=refresh if (self.newton_refresh and not self._fresh) else None)
+                                                refresh=refresh if (self.newton_refresh and not self._fresh) else None,
+                                                project=self.conserve == "project")
         if new_lu is not None:
             self._new_lu = new_lu
         res.n_krylov += 1

@@ -47,6 +47,8 @@ static void free_handle(Handle* h) {
   if (h->gs_order) (void)hipFree(h->gs_order);
   if (h->gsb) (void)hipFree(h->gsb);
   if (h->gsu) (void)hipFree(h->gsu);
+  for (hipEvent_t e : h->prof_pending) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
   delete h;
 }
 
@@ -125,6 +127,17 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   {
     std::vector<int> order;
     gs_colour_lines(Nr, Nth, h.nw > 0 ? Nth : 0, order, h.gs_cstart);
+    const int ncol_ = h.ncol;
+    const int C = (int)h.gs_cstart.size() - 1;
+    h.gs_ncell.assign(C, 0);
+    h.gs_nouter.assign(C, 0);
+    h.gs_nwall.assign(C, 0);
+    for (int c = 0; c < C; ++c)
+      for (int q = h.gs_cstart[c]; q < h.gs_cstart[c + 1]; ++q) {
+        const int L = order[q];
+        if (L >= ncol_) ++h.gs_nwall[c];
+        else { ++h.gs_ncell[c]; if (h.nw > 0 && L / Nth == Nr - 1) ++h.gs_nouter[c]; }
+      }
     rc |= check(hipMalloc((void**)&h.gs_order, sizeof(int) * order.size()), "hipMalloc");
     if (rc == 0)
       rc |= check(hipMemcpy(h.gs_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice), "hipMemcpy");
@@ -299,6 +312,18 @@ int vtsa_cons(vtsa_handle* H, const double* R, const double* s, double* wr, doub
   return rc;
 }
 
+int vtsa_cons_norm2(vtsa_handle* H, const double* s, double* out, vtsa_stream st) {
+  if (!H || !s || !out) { set_error("vtsa_cons_norm2: null argument"); return -1; }
+  return reduce_weighted(H->h, s, nullptr, 0, 2, out, S(st));
+}
+
+int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, double* out,
+                    vtsa_stream st) {
+  if (!H || !z || !s || !lambda || !out) { set_error("vtsa_cons_shift: null argument"); return -1; }
+  cons_shift_impl(H->h, z, s, lambda, out, S(st));
+  return finish(S(st));
+}
+
 int vtsa_inventories(vtsa_handle* H, const double* y, double* out, vtsa_stream st) {
   if (!H || !y || !out) return -1;
   return reduce_weighted(H->h, y, nullptr, 0, 1, out, S(st));
@@ -379,6 +404,26 @@ int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc ? rc : finish(S(st));
+}
+
+int vtsa_profile(vtsa_handle* H, int32_t every) {
+  if (!H || every < 0) { set_error("vtsa_profile: bad argument"); return -1; }
+  Handle& h = H->h;
+  if (prof_resolve(h)) return -1;
+  h.prof_every = every;
+  h.prof_seen = h.prof_launches = 0;
+  h.prof_ms = h.prof_bytes = 0.0;
+  return 0;
+}
+
+int vtsa_profile_read(vtsa_handle* H, int64_t* launches, double* total_ms, double* total_bytes) {
+  if (!H || !launches || !total_ms || !total_bytes) { set_error("vtsa_profile_read: null argument"); return -1; }
+  Handle& h = H->h;
+  if (prof_resolve(h)) return -1;
+  *launches = h.prof_launches;
+  *total_ms = h.prof_ms;
+  *total_bytes = h.prof_bytes;
+  return 0;
 }
 
 }  // extern "C"

@@ -63,6 +63,13 @@ struct Handle {
   int gs_sweeps = 1;
   double* gsb = nullptr;             // [N] scaled right-hand side
   double* gsu = nullptr;             // [N] unscaled iterate
+  std::vector<int> gs_ncell, gs_nouter, gs_nwall;   // per colour: cell lines, outer-ring lines, wall lines
+  // sampled timing of the half-sweep launches: every prof_every-th launch between HIP events
+  int prof_every = 0;
+  long long prof_seen = 0, prof_launches = 0;
+  double prof_ms = 0.0, prof_bytes = 0.0;
+  std::vector<hipEvent_t> prof_pending, prof_pool;  // (start, stop) pairs awaiting read; spare events
+  std::vector<double> prof_pending_bytes;
   // GMRES
   double* Vb = nullptr;       // basis [(MAX_RESTART+1) * N]
   int vb_cols = 0;
@@ -94,6 +101,8 @@ void launch_matvec_residual(Handle& h, double t, const double* z, const double* 
 // linear algebra (vtsa_linalg.hip)
 void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, int sweeps, hipStream_t s);
 void gs_colour_lines(int Nr, int Nth, int nwall_lines, std::vector<int>& order, std::vector<int>& cstart);
+int prof_resolve(Handle& h);
+void cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], double* out, hipStream_t st);
 // out = post^-1 * M^-1 (pre * in * (in_scale ? *in_scale : 1)); optionally stores the scaled
 // input to `store_in`; optionally accumulates sum(out^2) and sets sigma in gs (matvec use).
 void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,

@@ -126,23 +126,35 @@ __global__ void __launch_bounds__(RBLK) k_weighted(Model m, int mode, const doub
           loc[src * 4 + 2] += g2;
           loc[src * 4 + 3] += V * x[5 * n + c];
         }
-      } else {
+      } else if (mode == 1) {
         for (int s = 0; s < 3; ++s) loc[s] += (d.eps_t * a[s * n + c]) * V;
         loc[3] += (d.rho_b * a[3 * n + c]) * V;
         loc[4] += (d.rho_b * a[4 * n + c]) * V;
         loc[5] += a[5 * n + c] * V;
+      } else {
+        // sum over each conserved row of (W_i a_i)^2
+        const double g0 = (d.eps_t * V) * a[c], g1 = (d.eps_t * V) * a[n + c], g2 = (d.eps_t * V) * a[2 * n + c];
+        const double a0 = (d.rho_b * V) * a[3 * n + c], a1 = (d.rho_b * V) * a[4 * n + c];
+        const double e = V * a[5 * n + c];
+        loc[0] += g0 * g0 + a0 * a0;
+        loc[1] += g1 * g1 + a1 * a1;
+        loc[2] += g2 * g2;
+        loc[3] += e * e;
       }
     } else {
       const int w = c - n;
       if (mode == 0) {
         loc[3] += d.C_wall * a[6 * n + w];
         loc[7] += d.C_wall * b[6 * n + w];
-      } else {
+      } else if (mode == 1) {
         loc[6] += d.C_wall * (a[6 * n + w] - d.T_ref);
+      } else {
+        const double e = d.C_wall * a[6 * n + w];
+        loc[3] += e * e;
       }
     }
   }
-  const int nout = mode == 0 ? 8 : 7;
+  const int nout = mode == 0 ? 8 : (mode == 1 ? 7 : 4);
   __shared__ double tot[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {   // constant trip count keeps loc[] in registers
@@ -152,6 +164,29 @@ __global__ void __launch_bounds__(RBLK) k_weighted(Model m, int mode, const doub
     }
   }
   if (last_block_totals(counter, &slast, partial, nout, tot) && threadIdx.x < nout) result[threadIdx.x] = tot[threadIdx.x];
+}
+
+// out = z - S^2 W^T lambda: each entry moved along its conserved row k (gas species, adsorbed
+// species into the same row, energy of bed and wall) by s_i^2 W_ki lambda_k.
+__global__ void k_cons_shift(Model m, const double* z, const double* __restrict__ s, double l0, double l1,
+                             double l2, double l3, double* out) {   // out may alias z (elementwise)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const vtsa_model_desc& d = m.d;
+  const int n = m.n;
+  double w, lam;
+  if (i < 6 * n) {
+    const int v = i / n, c = i - v * n;
+    const double V = m.V[c / (d.Nth * d.Nz)];
+    const int k = v < 3 ? v : (v < 5 ? v - 3 : 3);
+    w = v < 3 ? d.eps_t * V : (v < 5 ? d.rho_b * V : V);
+    lam = k == 0 ? l0 : (k == 1 ? l1 : (k == 2 ? l2 : l3));
+  } else {
+    w = d.C_wall;
+    lam = l3;
+  }
+  const double si = s[i];
+  out[i] = z[i] - (si * si) * w * lam;
 }
 
 // Integrator vector formulas (core/solver.py:419-476), exact upstream operation order.
@@ -483,12 +518,16 @@ int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mo
   (void)len;
   hipLaunchKernelGGL(k_weighted, dim3(rgrid(h.n + h.nw)), dim3(RBLK), 0, st, h.m, mode, a, w, h.red.partial,
                      h.red.counter + 3, h.red.result);
-  const int nout = mode == 0 ? 8 : 7;
+  const int nout = mode == 0 ? 8 : (mode == 1 ? 7 : 4);
   if (check(hipMemcpyAsync(h.host_scalars, h.red.result, sizeof(double) * nout, hipMemcpyDeviceToHost, st), "memcpy"))
     return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   for (int q = 0; q < nout; ++q) out[q] = h.host_scalars[q];
   return 0;
+}
+
+void cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cons_shift, dim3(eblk(h.N)), dim3(256), 0, st, h.m, z, s, lam[0], lam[1], lam[2], lam[3], out);
 }
 
 // Newton building blocks (host drivers; each returns host scalars)

@@ -745,6 +745,51 @@ size_t partial_size(int ncol, int nw) {
   return a > b ? a : b;
 }
 
+// Algorithmic HBM bytes of one half-sweep over colour c, each datum counted once: per cell of a
+// cell line the 180 factor values, b~, s0 and the U written (6 each) and, unless it is the first
+// half-sweep (U = 0), the four off-line 6x6 J blocks and the neighbours' U; per outer-ring cell
+// Jcw (4) and the wall's U; per wall cell Jw (self, z +- 1), b~, s0, U written and, unless first,
+// Jw (theta +- 1), Jwc (6), the theta-neighbours' U (2) and the outer cell's U (6).
+static double gs_bytes(const Handle& h, int c, bool first, size_t fsz) {
+  const double Nz = h.desc.Nz;
+  const double cells = h.gs_ncell[c] * Nz, outer = h.gs_nouter[c] * Nz, walls = h.gs_nwall[c] * Nz;
+  const double per_cell = 180.0 * fsz + 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24));
+  const double per_outer = first ? 0.0 : 8.0 * 5;
+  const double per_wall = 8.0 * 6 + (first ? 0.0 : 8.0 * 16);
+  return cells * per_cell + outer * per_outer + walls * per_wall;
+}
+
+static hipEvent_t prof_event(Handle& h) {
+  if (!h.prof_pool.empty()) {
+    hipEvent_t e = h.prof_pool.back();
+    h.prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  // no system-scope fence: the bracket should cost as little as possible around the kernel
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Read the pending (start, stop) pairs into the totals (waits for them) and recycle the events.
+int prof_resolve(Handle& h) {
+  int rc = 0;
+  for (size_t q = 0; q + 1 < h.prof_pending.size(); q += 2) {
+    hipEvent_t e0 = h.prof_pending[q], e1 = h.prof_pending[q + 1];
+    float ms = 0.f;
+    if (!rc) rc = check(hipEventSynchronize(e1), "hipEventSynchronize");
+    if (!rc) rc = check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    h.prof_ms += ms;
+    h.prof_bytes += h.prof_pending_bytes[q / 2];
+    h.prof_launches += 1;
+    h.prof_pool.push_back(e0);
+    h.prof_pool.push_back(e1);
+  }
+  h.prof_pending.clear();
+  h.prof_pending_bytes.clear();
+  return rc;
+}
+
 template <typename T>
 static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   const int N = h.N, Nz = h.desc.Nz;
@@ -763,8 +808,22 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     g.first = q == 0 ? 1 : 0;
     const int cnt = h.gs_cstart[c + 1] - h.gs_cstart[c];
     if (cnt == 0) continue;
+    const bool sample = h.prof_every > 0 && (h.prof_seen++ % h.prof_every) == 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (sample) {
+      e0 = prof_event(h);
+      e1 = prof_event(h);
+      if (e0 && e1) (void)hipEventRecord(e0, s);
+    }
     hipLaunchKernelGGL(k_gs_half<T>, dim3(cnt), dim3(PW), zline_lds_bytes<T>(Nz), s, h.n, h.desc.Nr, h.desc.Nth, Nz,
                        h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+    if (sample && e0 && e1) {
+      (void)hipEventRecord(e1, s);
+      h.prof_pending.push_back(e0);
+      h.prof_pending.push_back(e1);
+      h.prof_pending_bytes.push_back(gs_bytes(h, c, g.first != 0, sizeof(T)));
+      if (h.prof_pending_bytes.size() >= 512) (void)prof_resolve(h);
+    }
   }
   hipLaunchKernelGGL(k_gs_final, dim3(rgrid(N)), dim3(RBLK), 0, s, N, a, h.gsu, h.red.partial + RMAXB * 8,
                      h.red.counter + 1);

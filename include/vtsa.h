@@ -150,6 +150,13 @@ int vtsa_newton_try(vtsa_handle* h, double t, const double* z, const double* dz,
 /* conservation test |W R| <= tol W s for every conserved quantity (core/solver.py:88-96);
  * wr[4] = W R and ws[4] = W s returned on the host. */
 int vtsa_cons(vtsa_handle* h, const double* R, const double* s, double* wr, double* ws, vtsa_stream st);
+/* Conservation projection (no upstream counterpart; optional in the Newton stage solve):
+ * out[k] = sum over the entries of conserved row k of (W_ki s_i)^2, k = CO2, H2O, N2, energy. */
+int vtsa_cons_norm2(vtsa_handle* h, const double* s, double* out, vtsa_stream st);
+/* out = z - S^2 W^T lambda (lambda: 4 host values), the minimum-norm shift in the s-scaled
+ * variables that changes the conserved totals W z by -W S^2 W^T lambda.  out may alias z. */
+int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, double* out,
+                    vtsa_stream st);
 /* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall
  * (core/residual.py:426-443) */
 int vtsa_inventories(vtsa_handle* h, const double* y, double* out, vtsa_stream st);
@@ -186,6 +193,13 @@ int vtsa_sync(vtsa_stream st);
 /* Elapsed milliseconds of `repeat` back-to-back residual evaluations f = f(t, y) on stream st,
  * measured with HIP events recorded on that stream (kernel timing for the roofline). */
 int vtsa_time_rhs(vtsa_handle* h, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st);
+
+/* Sampled timing of the line Gauss-Seidel half-sweep kernel, the dominant kernel of a step (no
+ * upstream counterpart: instrumentation for the roofline).  Every `every`-th launch is bracketed
+ * by HIP events on its stream (0 disables); the counters are reset. */
+int vtsa_profile(vtsa_handle* h, int32_t every);
+/* Sampled launches so far, their total milliseconds (events) and algorithmic bytes. */
+int vtsa_profile_read(vtsa_handle* h, int64_t* launches, double* total_ms, double* total_bytes);
 
 /* Library version string and last error message (host memory owned by the library). */
 const char* vtsa_version(void);

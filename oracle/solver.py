@@ -299,20 +299,22 @@ class OracleTRBDF2:
         return z, ok, (0.0 if ok else rate)
 
     def integrate(self, y0, t_end, t_out, on_step: Optional[Callable] = None,
-                  stop_fn: Optional[Callable] = None, stop_value=None, stop_direction=1):
+                  stop_fn: Optional[Callable] = None, stop_value=None, stop_direction=1, t0: float = 0.0):
+        """``t0`` (default 0, as upstream) starts the integration inside the step, for a timed
+        window of the CPU baseline; the first step size is then self.h_init."""
         m = self.m
         wall0 = time.time()
-        t = 0.0
+        t = float(t0)
         y = y0.copy()
         acc = np.zeros(m.n_acc)
         f_n, g_n = m.rhs(t, y)
         res = IntegrationResult(y=y, acc=acc, t=t)
-        t_out = np.asarray(sorted(set(np.clip(t_out, 0.0, t_end)) | {t_end}))
+        t_out = np.asarray(sorted(set(np.clip(t_out, t, t_end)) | {t_end}))
         k_out = 0
-        if t_out[0] <= 0.0:
-            res.snapshots.append((0.0, y.copy(), acc.copy()))
+        if t_out[0] <= t:
+            res.snapshots.append((t, y.copy(), acc.copy()))
             k_out = 1
-        h = min(self.h_init, self.h_max, t_end)
+        h = min(self.h_init, self.h_max, t_end - t)
         J = lu = h_lu = None
         jac_age = 0
         fresh = False
