@@ -13,9 +13,11 @@ for the functions that no closed form, invariant or known answer pins on its own
 * ``jac_cfg1.npz``    : the same on the 4x4x20 seeded state;
 * ``integ_cfg1.npz``  : 20 s of the adsorption step from the seeded 4x4x20 state, sdirk2 and
                         trbdf2: accepted (t, h), counters, end state and accumulators;
+* ``integ_cfg1_hybrid.npz`` : 10 s of the same step with solver.newton_mode = hybrid (modified
+                        Newton on the stored factor, Krylov on failure);
 * ``closures.npz``    : isotherm / p_sat / property values over a grid of inputs.
 
-Usage (from the repository root):  python tests/golden/make_golden.py
+Usage (from the repository root):  python -m tests.golden.make_golden
 """
 from __future__ import annotations
 
@@ -170,6 +172,26 @@ def make_integ(path, t_end=20.0):
     np.savez_compressed(path, **out)
 
 
+def make_integ_hybrid(path, t_end=10.0):
+    """10 s of the 4x4x20 adsorption step with solver.newton_mode = hybrid (modified Newton with
+    the factor first, Krylov on failure; core/solver.py:111-151)."""
+    out = {}
+    cfg = load_config(override_dict={"grid.Nr": 4, "grid.Ntheta": 4, "grid.Nz": 20, "solver.newton_mode": "hybrid"})
+    g, st = _seeded(cfg, "small", 0)
+    props = get_backend(cfg)
+    m = TVSAModel(cfg, g, props, cfg.cycle[0], st)
+    y0 = m.pack(st)
+    integ = TRBDF2(m, cfg, h_max=0.02 * 200.0)
+    times = []
+    res = integ.integrate(y0, t_end, np.linspace(0, t_end, 3), on_step=lambda t, y, a: times.append(t))
+    out["y0"], out["y"], out["acc"], out["t_acc"] = y0, res.y, res.acc, np.array(times)
+    out["counters"] = np.array([res.n_steps, res.n_reject, res.n_newton_fail, res.n_krylov,
+                                res.n_jac, res.n_lu, res.nfev])
+    out["modified_fail"] = integ.stats["modified_fail"]
+    print("hybrid steps", res.n_steps, "modified_fail", integ.stats["modified_fail"])
+    np.savez_compressed(path, **out)
+
+
 def make_closures(path):
     cfg = load_config()
     props = get_backend(cfg)
@@ -209,4 +231,5 @@ if __name__ == "__main__":
     g, st = _seeded(cfg, "small", 0)
     make_jac(os.path.join(HERE, "jac_cfg1.npz"), cfg, st, 5.0)
     make_integ(os.path.join(HERE, "integ_cfg1.npz"))
+    make_integ_hybrid(os.path.join(HERE, "integ_cfg1_hybrid.npz"))
     print("done in %.1f s" % (time.time() - t0))

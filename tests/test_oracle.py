@@ -228,6 +228,29 @@ def test_integrator_matches_golden(scheme):
     np.testing.assert_allclose(res.acc, z[f"{scheme}/acc"], rtol=1e-9, atol=1e-12 * np.abs(z[f"{scheme}/acc"]).max())
 
 
+@pytest.mark.slow
+def test_integrator_hybrid_newton_matches_golden():
+    """newton_mode = hybrid (core/solver.py:325-368): modified Newton on the stored factor first,
+    Newton-Krylov when it stalls; 10 s of adsorption from the seeded 4x4x20 state.  The fixture
+    holds >100 modified-Newton failures, so both branches and their hand-over are compared."""
+    from oracle.solver import OracleTRBDF2
+    z = golden("integ_cfg1_hybrid.npz")
+    zs = golden("integ_cfg1.npz")
+    cfg, m = oracle_model({**CFG1, "solver.newton_mode": "hybrid"}, 0, state_from(zs))
+    y0 = m.pack(state_from(zs))
+    np.testing.assert_allclose(y0, z["y0"], rtol=1e-15)
+    times = []
+    integ = OracleTRBDF2(m, cfg, h_max=4.0)
+    res = integ.integrate(y0, 10.0, np.linspace(0, 10.0, 3), on_step=lambda t, y, a: times.append(t))
+    cnt = np.array([res.n_steps, res.n_reject, res.n_newton_fail, res.n_krylov, res.n_jac, res.n_lu, res.nfev])
+    assert np.array_equal(cnt, z["counters"]), (cnt, z["counters"])
+    assert integ.stats["modified_fail"] == int(z["modified_fail"])
+    np.testing.assert_allclose(times, z["t_acc"], rtol=1e-12)
+    sc = m.atol + 1e-4 * np.abs(z["y"])
+    assert np.max(np.abs(res.y - z["y"]) / sc) < 1e-6
+    np.testing.assert_allclose(res.acc, z["acc"], rtol=1e-9, atol=1e-12 * np.abs(z["acc"]).max())
+
+
 def test_integrator_matches_rk4_short_transient():
     """SDIRK2 Newton-Krylov vs classical RK4 on 2 ms of blowdown, 2x1x4 (upstream tests :182-200)."""
     from oracle.runner import initial_state, make
