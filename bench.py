@@ -13,9 +13,12 @@ wall seconds of the timed region.
 Also reported: the per-step mass/energy balance errors; the roofline of the dominant kernel, the
 line Gauss-Seidel half-sweep, from every 8th of its launches in the timed region bracketed by HIP
 events on its stream (algorithmic bytes per launch / average launch time); the residual kernel
-pair timed the same way as a secondary figure; and a CPU baseline: the NumPy/SciPy oracle (same
+pair timed the same way as a secondary figure; a CPU baseline: the NumPy/SciPy oracle (same
 algorithms as upstream, 1 thread) on the same state for a bounded wall-clock sample, with the GPU
-wall time over the same simulated window.
+wall time over the same simulated window; and the learned form (``learned_fp32``): the same step
+with the learned stage guess and FP32 preconditioner factors, the network trained on the device
+from a plain run of another seeded state, its rate against the plain FP64 rate, its balances and
+its end-field differences from the plain FP64 run.
 """
 from __future__ import annotations
 
@@ -140,6 +143,73 @@ def cpu_baseline(cfg, state, budget_s, sr):
     return out
 
 
+def field_errors(st, ref, cfg):
+    """Per field (c_CO2, c_H2O, c_N2, q_CO2, q_H2O, T, Tw) against a reference end state: the max
+    relative difference where |ref| is above both 1e-3 of the field's maximum and atol, and the max difference
+    in integration-tolerance units |a - b| / (atol + rtol |b|) (atol of the configuration)."""
+    s = cfg.solver
+    fields = [("c_CO2", st.c[0], ref.c[0], s.atol_c_co2), ("c_H2O", st.c[1], ref.c[1], s.atol_c_h2o),
+              ("c_N2", st.c[2], ref.c[2], s.atol_c_inert), ("q_CO2", st.q[0], ref.q[0], s.atol_q),
+              ("q_H2O", st.q[1], ref.q[1], s.atol_q), ("T", st.T, ref.T, s.atol_T), ("Tw", st.Tw, ref.Tw, s.atol_T)]
+    rel, tol = {}, {}
+    for name, a, b, atol in fields:
+        a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+        big = np.abs(b) > max(1e-3 * np.abs(b).max(), atol)
+        rel[name] = float(np.max(np.abs(a - b)[big] / np.abs(b)[big])) if big.any() else 0.0
+        tol[name] = float(np.max(np.abs(a - b) / (atol + s.rtol * np.abs(b))))
+    return {"max_rel": rel, "max_tol_units": tol}
+
+
+def learned_leg(cfg, grid, props, state, args, st_plain, plain_value):
+    """Learned stage guess + FP32 preconditioner on the same heating step.  The network is trained
+    on the device from a plain FP64 run of another seeded state (``--train-seed``): that run
+    records every converged stage, 20 epochs of Adam follow, the output is shrunk per variable on
+    held-out stages; then the seed-0 step is timed with the guess and FP32 factors."""
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.learned import StageDataset, StageNet
+    from dac_vtsa_amd.seeds import seeded_state
+    from dac_vtsa_amd.steps import run_step
+    step = cfg.cycle[STEP_INDEX]
+    dur = float(step["duration"])
+    n = grid.ncell
+    base = {"gs_sweeps": args.sweeps, "conserve": args.conserve}
+    ds = StageDataset(n, 700)
+    t0 = time.time()
+    run_step(seeded_state(grid, "midcycle", args.train_seed), cfg, grid, props, step,
+             integrator_kw={**base, "precision": "fp64", "record": ds})
+    dv.synchronize()
+    t_rec = time.time() - t0
+    net = StageNet(n, norm=ds.norm(), seed=0)
+    hist = ds.train(net, epochs=args.epochs, lr=1e-3, lr_final=1e-4)
+    ep, eb, eg = ds.errors(net, hist["val_ids"])
+    _log(f"learned: {ds.count} stages recorded in {t_rec:.2f} s, trained in {hist['wall_s']:.2f} s "
+         f"({hist['steps']} Adam steps), held-out val loss {hist['val'][-1]:.3g}")
+    kw = {**base, "precision": "fp32", "guess": net}
+    for _ in range(args.warmup):
+        run_step(state, cfg, grid, props, step, integrator_kw=kw)
+    dv.synchronize()
+    t0 = time.time()
+    for _ in range(args.steps):
+        st, sr = run_step(state, cfg, grid, props, step, integrator_kw=kw)
+    dv.synchronize()
+    rate = args.steps * dur / (time.time() - t0)
+    mb, eb_ = balance_errors(sr, props.species)
+    _log(f"learned+fp32: {rate:.0f} sim-s/wall-s ({rate / plain_value:.2f}x plain fp64)")
+    return {
+        "value": rate, "unit": "sim-s/wall-s", "vs_plain_fp64": rate / plain_value,
+        "balance": {"max_mass_rel_err": mb, "max_energy_rel_err": eb_},
+        "field_diff_vs_plain_fp64": field_errors(st, st_plain, cfg),
+        "solver": {k: sr.solver.get(k) for k in ("steps", "rejected", "gmres_iters", "guess_used",
+                                                 "guess_kept_predictor")},
+        "training": {"train_seed": args.train_seed, "stages": ds.count, "rows": ds.count * n,
+                     "record_s": t_rec, "train_s": hist["wall_s"], "adam_steps": hist["steps"],
+                     "epochs": args.epochs, "val_loss": hist["val"][-1], "gain": hist["gain"],
+                     "heldout_rms_err_tol_units": {"predictor": ep.mean(0).tolist(),
+                                                   "preconditioned_step": eb.mean(0).tolist(),
+                                                   "learned_guess": eg.mean(0).tolist()}},
+    }
+
+
 def balance_errors(sr, species):
     from dac_vtsa_amd import balances
     mb, eb, _ = balances.check_cycle([sr], species)
@@ -162,6 +232,9 @@ def main():
     ap.add_argument("--conserve", choices=("polish", "project"), default="project",
                     help="stage conservation: upstream polish, or the conservation projection first")
     ap.add_argument("--copy-probe", action="store_true", help="only run the traffic-counter calibration copy")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess + FP32 preconditioner leg")
+    ap.add_argument("--train-seed", type=int, default=1, help="seed of the state whose run trains the network")
+    ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -189,7 +262,7 @@ def main():
     t0 = time.time()
     results = []
     for i in range(args.steps):
-        _, sr = one_step(cfg, grid, props, state, args.precision, args.sweeps, args.conserve)
+        st_plain, sr = one_step(cfg, grid, props, state, args.precision, args.sweeps, args.conserve)
         results.append(sr)
         print(f"[bench] step {i + 1}/{args.steps}: {sr.solver['steps']} accepted steps, "
               f"{sr.solver['rejected']} rejected, {sr.solver.get('gmres_iters', 0)} GMRES iterations",
@@ -226,6 +299,8 @@ def main():
         "residual_pair": {"kernels": "k_props + k_flux", "avg_launch_us": res_s * 1e6,
                           "algo_bytes_per_launch": res_bytes, "achieved_GBs": res_bytes / res_s / 1e9},
     }
+    if not args.no_learned:
+        line["learned_fp32"] = learned_leg(cfg, grid, props, state, args, st_plain, value)
     if not args.no_cpu:
         from threadpoolctl import threadpool_limits
         with threadpool_limits(1):

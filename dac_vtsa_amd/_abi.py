@@ -97,6 +97,17 @@ _SIGS = {
     "vtsa_time_rhs": [P, D, P, P, I32, C.POINTER(D), P],
     "vtsa_profile": [P, I32],
     "vtsa_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(D), C.POINTER(D)],
+    "vtsa_mlp_sizes": [C.POINTER(I32)] * 5,
+    "vtsa_mlp_create": [I32, C.POINTER(P)],
+    "vtsa_mlp_destroy": [P],
+    "vtsa_mlp_set_params": [P, P, P],
+    "vtsa_mlp_get_params": [P, P, P],
+    "vtsa_mlp_set_norm": [P, P, P],
+    "vtsa_mlp_forward": [P, P, I32, P, P],
+    "vtsa_mlp_train_step": [P, P, P, I32, C.c_float, C.POINTER(C.c_float), P, P],
+    "vtsa_mlp_features": [P, P, P, P, P, D, D, P, P],
+    "vtsa_mlp_targets": [P, P, P, P, P, P, P],
+    "vtsa_mlp_apply": [P, P, P, P, P, P, P, P],
     "vtsa_version": [],
     "vtsa_last_error": [],
 }

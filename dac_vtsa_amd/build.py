@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvtsa.so")
-SOURCES = ["vtsa_kernels.hip", "vtsa_linalg.hip", "vtsa_precond.hip", "vtsa_engine.hip"]
+SOURCES = ["vtsa_kernels.hip", "vtsa_linalg.hip", "vtsa_precond.hip", "vtsa_mlp.hip", "vtsa_engine.hip"]
 ARCH = os.environ.get("VTSA_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]

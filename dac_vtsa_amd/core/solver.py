@@ -305,13 +305,16 @@ class TRBDF2:
     Extra keyword arguments (no upstream counterpart): ``precond`` ("zline_gs" | "zline" |
     "block_jacobi"), ``gs_sweeps`` (symmetric line Gauss-Seidel sweeps of "zline_gs"),
     ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``, ``conserve``
-    ("polish": the upstream conservation polish; "project": project_conservation first).  The
+    ("polish": the upstream conservation polish; "project": project_conservation first),
+    ``guess`` (a learned.StageNet: each stage starts from the network's guess when its
+    preconditioned residual is below the predictor's) and ``record`` (a learned.StageDataset
+    that receives the features and converged corrections of every solved stage).  The
     preconditioner stands in for the upstream SuperLU factor of the theta-block matrix wherever
     that factor is used: GMRES, the Newton residual norm and the error-estimate filter."""
 
     def __init__(self, model, cfg, h_max: float, h_init: Optional[float] = None, logger=None,
                  precond: str = "zline_gs", precision: str = "fp64", krylov_dim: int = 30,
-                 gs_sweeps: int = 2, conserve: str = "polish"):
+                 gs_sweeps: int = 2, conserve: str = "polish", guess=None, record=None):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -346,6 +349,16 @@ class TRBDF2:
         self.cap_after_fail = 0.5
         self.cap_relax = 1.15
         self.ops = DeviceOps(model)
+        self.record = record
+        self.guess = None
+        if guess is not None:
+            from ..learned import StageGuess
+            self.guess = StageGuess(guess, model.n_c // model.ng, self.ops.N)
+        if (guess is not None or record is not None) and (model.ng != 3 or model.na != 2):
+            raise ValueError("the learned stage guess needs the three-gas, two-adsorbate state layout")
+        if guess is not None or record is not None:
+            self.stats.update({"guess_used": 0, "guess_kept_predictor": 0})
+            self._s_ref, self._gf, self._gR, self._gP, self._zg = (dv.empty(self.ops.N) for _ in range(5))
         self.last_err = None
         self._new_lu = None
         self._new_J = None
@@ -359,8 +372,37 @@ class TRBDF2:
                                self.gs_sweeps)
         return Factorisation(h)
 
-    def _solve_stage(self, t, z0, rhs_known, dh, lu, res, y_ref):
+    def _start(self, t, z0, rhs_known, dh, y_ref, stage):
+        """Starting value of a stage: the predictor z0, or the learned guess when its
+        preconditioned stage residual is smaller; records the stage's features."""
+        if self.guess is None and self.record is None:
+            return z0
         h = dh / D
+        ops = self.ops
+        s = ops.scale(y_ref, self.rtol, self._s_ref)
+        _, pn0 = ops.newton_eval(t, z0, dh, rhs_known, s, self._gf, self._gR)
+        if not np.isfinite(pn0):
+            return z0
+        ops.prec_apply(self._gR, self._gP)          # M^-1 R(z0): minus the first Newton step
+        self._recording = self.record is not None and self.record.begin(self.m.handle, y_ref, z0, s, self._gP, h,
+                                                                        stage)
+        if self.guess is None:
+            return z0
+        zg = self.guess.predict(self.m.handle, y_ref, z0, s, self._gP, h, stage, self._zg)
+        _, pn1 = ops.newton_eval(t, zg, dh, rhs_known, s, self._gf, self._gR)
+        if np.isfinite(pn1) and pn0 > 0.0:
+            self.stats["guess_log_pn_ratio"] = self.stats.get("guess_log_pn_ratio", 0.0) + float(np.log(pn1 / pn0))
+        if np.isfinite(pn1) and pn1 < pn0:
+            self.stats["guess_used"] += 1
+            return zg
+        self.stats["guess_kept_predictor"] += 1
+        return z0
+
+    def _solve_stage(self, t, z0, rhs_known, dh, lu, res, y_ref, stage=0):
+        h = dh / D
+        z_pred = z0
+        self._recording = False
+        z0 = self._start(t, z0, rhs_known, dh, y_ref, stage)
 
         def refresh(zc):
             fc = dv.empty(self.ops.N)
@@ -379,6 +421,8 @@ class TRBDF2:
                                                 project=self.conserve == "project")
         if new_lu is not None:
             self._new_lu = new_lu
+        if ok and self._recording:
+            self.record.commit(self.m.handle, z, z_pred, self._gP, self._s_ref)
         res.n_krylov += 1
         self.stats["krylov_ok" if ok else "krylov_fail"] += 1
         return z, ok, (0.0 if ok else rate)
@@ -445,7 +489,7 @@ class TRBDF2:
             z_pred = y if not have_slope else ops.vec(1, zp, x=y, y=slope, a=c_mid * h)
             self._new_lu = None
             self._fresh = fresh
-            z1, ok, r1_rate = self._solve_stage(t + c_mid * h, z_pred, r1, dh, lu, res, y)
+            z1, ok, r1_rate = self._solve_stage(t + c_mid * h, z_pred, r1, dh, lu, res, y, 0)
             if self._new_lu is not None:
                 lu, h_lu = self._new_lu, self._new_J[1]
                 jac_age, fresh, self._new_lu = 0, True, None
@@ -456,7 +500,7 @@ class TRBDF2:
                 else:
                     ops.vec(4, rhs2, x=y, y=z1, a=C1, b=C2)
                     ops.vec(2, z2p, x=y, y=z1, a=GAMMA)
-                z2, ok, r2_rate = self._solve_stage(t + h, z2p, rhs2, dh, lu, res, y)
+                z2, ok, r2_rate = self._solve_stage(t + h, z2p, rhs2, dh, lu, res, y, 1)
                 if self._new_lu is not None:
                     lu, h_lu = self._new_lu, self._new_J[1]
                     jac_age, fresh, self._new_lu = 0, True, None

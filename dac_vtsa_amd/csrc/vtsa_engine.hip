@@ -66,10 +66,6 @@ static int grow_basis(Handle& h, int restart) {
 
 using namespace vtsa;
 
-struct vtsa_handle {
-  Handle h;
-};
-
 static inline hipStream_t S(vtsa_stream s) { return (hipStream_t)s; }
 
 static int finish(hipStream_t s) {

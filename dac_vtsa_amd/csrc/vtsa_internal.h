@@ -140,3 +140,8 @@ struct ApplyArgs {
 void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s);
 
 }  // namespace vtsa
+
+// The opaque handle of the C ABI.
+struct vtsa_handle {
+  vtsa::Handle h;
+};

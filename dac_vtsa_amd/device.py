@@ -25,16 +25,21 @@ def _ensure_device():
 
 
 class DeviceArray:
-    """A contiguous FP64 vector in device memory."""
+    """A contiguous vector in device memory (FP64 unless another dtype is given)."""
 
-    __slots__ = ("n", "_p", "__weakref__")
+    __slots__ = ("n", "dtype", "_p", "__weakref__")
 
-    def __init__(self, n: int):
+    def __init__(self, n: int, dtype=np.float64):
         _ensure_device()
         self.n = int(n)
+        self.dtype = np.dtype(dtype)
         p = C.c_void_p()
-        _abi.call("vtsa_device_alloc", C.c_size_t(8 * max(self.n, 1)), C.byref(p))
+        _abi.call("vtsa_device_alloc", C.c_size_t(self.dtype.itemsize * max(self.n, 1)), C.byref(p))
         self._p = p
+
+    @property
+    def nbytes(self):
+        return self.dtype.itemsize * self.n
 
     @property
     def ptr(self):
@@ -53,42 +58,72 @@ class DeviceArray:
             self._p = None
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        a = np.ascontiguousarray(a, dtype=self.dtype).ravel()
         if a.size != self.n:
             raise ValueError(f"size mismatch {a.size} != {self.n}")
-        _abi.call("vtsa_copy_h2d", self._p, a.ctypes.data_as(C.c_void_p), C.c_size_t(8 * self.n), None)
+        _abi.call("vtsa_copy_h2d", self._p, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), None)
         return self
 
     def numpy(self):
-        out = np.empty(self.n)
-        _abi.call("vtsa_copy_d2h", out.ctypes.data_as(C.c_void_p), self._p, C.c_size_t(8 * self.n), None)
+        out = np.empty(self.n, dtype=self.dtype)
+        _abi.call("vtsa_copy_d2h", out.ctypes.data_as(C.c_void_p), self._p, C.c_size_t(self.nbytes), None)
         return out
 
     def copy_from(self, other: "DeviceArray"):
-        if other.n != self.n:
+        if other.n != self.n or other.dtype != self.dtype:
             raise ValueError("size mismatch")
-        _abi.call("vtsa_copy_d2d", self._p, other._p, C.c_size_t(8 * self.n), None)
+        _abi.call("vtsa_copy_d2d", self._p, other._p, C.c_size_t(self.nbytes), None)
         return self
 
     def clone(self) -> "DeviceArray":
-        return DeviceArray(self.n).copy_from(self)
+        return DeviceArray(self.n, self.dtype).copy_from(self)
+
+    def view(self, offset: int, n: int) -> "DeviceView":
+        """A window [offset, offset + n) of this array (shares its memory)."""
+        if offset < 0 or n < 0 or offset + n > self.n:
+            raise ValueError("view out of range")
+        return DeviceView(self, offset, n)
 
     def fill(self, v: float):
+        if self.dtype != np.float64:
+            raise TypeError("fill is FP64 only")
         _abi.call("vtsa_fill", self._p, float(v), C.c_size_t(self.n), None)
         return self
 
 
-def empty(n):
-    return DeviceArray(n)
+class DeviceView:
+    """A window of a DeviceArray; keeps its parent alive."""
+
+    __slots__ = ("base", "n", "dtype", "_p")
+
+    def __init__(self, base: DeviceArray, offset: int, n: int):
+        self.base = base
+        self.n = n
+        self.dtype = base.dtype
+        self._p = C.c_void_p(int(base.ptr.value) + int(offset) * base.dtype.itemsize)
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _abi.call("vtsa_copy_d2h", out.ctypes.data_as(C.c_void_p), self._p,
+                  C.c_size_t(self.n * self.dtype.itemsize), None)
+        return out
+
+
+def empty(n, dtype=np.float64):
+    return DeviceArray(n, dtype)
 
 
 def zeros(n):
     return DeviceArray(n).fill(0.0)
 
 
-def to_dev(a):
-    a = np.asarray(a, dtype=np.float64)
-    return DeviceArray(a.size).upload(a)
+def to_dev(a, dtype=np.float64):
+    a = np.asarray(a, dtype=dtype)
+    return DeviceArray(a.size, dtype).upload(a)
 
 
 def to_host(x):

@@ -201,6 +201,48 @@ int vtsa_profile(vtsa_handle* h, int32_t every);
 /* Sampled launches so far, their total milliseconds (events) and algorithmic bytes. */
 int vtsa_profile_read(vtsa_handle* h, int64_t* launches, double* total_ms, double* total_bytes);
 
+/* ---- Learned stage guess (no upstream counterpart: the upstream stage predictor is the linear
+ * extrapolation of core/solver.py:420,432-435, which Newton-Krylov then corrects).
+ *
+ * A per-cell network, 32 inputs -> 128 -> 128 -> 128 -> 16 outputs (6 used), ReLU, BF16
+ * weights and activations on the matrix cores with FP32 accumulation, FP32 master weights and
+ * Adam moments (beta1 0.9, beta2 0.999, eps 1e-8).  It predicts the converged stage value
+ * relative to the predictor, (z - z_pred) / s with s = atol + rtol |y|, for the six cell
+ * variables; the Newton-Krylov solve still runs to the usual tolerance from the guess.
+ * Parameters (FP32, flat, W[out][in]): W1[128][32] b1[128] W2[128][128] b2[128] W3[128][128]
+ * b3[128] W4[16][128] b4[16].  Normalisation (FP32[96]): input mean[32], input 1/std[32],
+ * target mean[16], target std[16].  Feature rows and targets are device arrays (row-major,
+ * 32 and 8 floats per row); `rows` is at most the capacity given at creation. */
+typedef struct vtsa_mlp vtsa_mlp;
+int vtsa_mlp_sizes(int32_t* n_param, int32_t* n_in, int32_t* n_hidden, int32_t* n_out, int32_t* n_norm);
+int vtsa_mlp_create(int32_t max_rows, vtsa_mlp** out);
+int vtsa_mlp_destroy(vtsa_mlp* m);
+/* Host parameters in; resets the Adam moments and step counter. */
+int vtsa_mlp_set_params(vtsa_mlp* m, const float* params, vtsa_stream st);
+int vtsa_mlp_get_params(vtsa_mlp* m, float* params, vtsa_stream st);
+int vtsa_mlp_set_norm(vtsa_mlp* m, const float* norm, vtsa_stream st);
+/* out[rows][16] = network(feat[rows][32]) (normalised target units). */
+int vtsa_mlp_forward(vtsa_mlp* m, const float* feat, int32_t rows, float* out, vtsa_stream st);
+/* One training step on one minibatch: forward, mean-squared error over the six used outputs
+ * against the normalised targets, backward, and an Adam step of size lr (lr = 0: gradient only).
+ * loss (host, may be NULL): the minibatch loss before the step; grad (host, may be NULL): the
+ * gradient in the parameter layout.  Either output synchronises the stream. */
+int vtsa_mlp_train_step(vtsa_mlp* m, const float* feat, const float* target, int32_t rows, float lr, float* loss,
+                        float* grad, vtsa_stream st);
+/* Feature rows of one stage for the n cells of a handle: the cell's six variables and those of
+ * its z-neighbours at the step start y, (z_pred - y)/s, r/s, log(h) and the stage index; the
+ * integrator passes r = M^-1 R(z_pred), the preconditioned stage residual at the predictor. */
+int vtsa_mlp_features(vtsa_handle* h, const double* y, const double* z_pred, const double* s, const double* r,
+                      double log_h, double stage, float* feat, vtsa_stream st);
+/* Training targets (z_converged - z_base)/s of the six cell variables (8 floats per cell), with
+ * z_base = z_pred - r, or z_base = z_pred when r is NULL. */
+int vtsa_mlp_targets(vtsa_handle* h, const double* z_conv, const double* z_pred, const double* r, const double* s,
+                     float* target, vtsa_stream st);
+/* Stage guess z = z_base + s (out std + mean) on the cell variables (c and q truncated to at least
+ * 0.1 z_pred as core/solver.py:99-108 does); wall entries = z_base. */
+int vtsa_mlp_apply(vtsa_mlp* m, vtsa_handle* h, const float* out, const double* z_pred, const double* r,
+                   const double* s, double* z, vtsa_stream st);
+
 /* Library version string and last error message (host memory owned by the library). */
 const char* vtsa_version(void);
 const char* vtsa_last_error(void);

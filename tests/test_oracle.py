@@ -275,3 +275,126 @@ def test_integrator_matches_rk4_short_transient():
     res = OracleTRBDF2(m, cfg, h_max=t_end, h_init=1e-6).integrate(y0, t_end, np.array([t_end]))
     sc = m.atol + 1e-4 * np.abs(y)
     assert np.sqrt(np.mean(((res.y - y) / sc) ** 2)) < 1.0
+
+
+# ------------------------------------------------------------------ learned stage guess (no upstream counterpart)
+def _mlp_case(rows=37, seed=3):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    p = mlp.init_params(seed)
+    p = p + (0.05 * rng.standard_normal(p.size)).astype(np.float32)     # non-zero biases too
+    feat = rng.standard_normal((rows, mlp.MI)).astype(np.float32) * 3.0 + 1.0
+    norm = np.concatenate([np.full(mlp.MI, 1.0), np.full(mlp.MI, 1.0 / 3.0), rng.standard_normal(mlp.MO),
+                           0.5 + rng.random(mlp.MO)]).astype(np.float32)
+    target = rng.standard_normal((rows, 8)).astype(np.float32)
+    return mlp, p, feat, norm, target
+
+
+def test_mlp_bf16_rounding_matches_torch():
+    import torch
+    from oracle import mlp
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(10000).astype(np.float32) * 10.0 ** rng.integers(-30, 30, 10000),
+                        np.array([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -1.0 - 2.0 ** -8, 0.0, -0.0], np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(mlp.bf16(x).view(np.uint32), ref.view(np.uint32))
+
+
+def test_mlp_forward_matches_torch_linear_layers():
+    import torch
+    import torch.nn.functional as Fn
+    mlp, p, feat, norm, _ = _mlp_case()
+    out, _ = mlp.forward(p, feat, norm, rounding=False)
+    P = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in mlp.unflatten(p.astype(np.float64)).items()}
+    x = torch.from_numpy(((feat - norm[:32]) * norm[32:64]).astype(np.float64))
+    h = x
+    for L in (1, 2, 3):
+        h = torch.relu(Fn.linear(h, P[f"W{L}"], P[f"b{L}"]))
+    ref = Fn.linear(h, P["W4"], P["b4"]).numpy()
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_mlp_backward_matches_finite_differences():
+    """Central differences of the float64 loss in 40 random parameter directions, and per
+    coordinate for a sample of entries of every layer (a transposed or dropped term fails)."""
+    mlp, p, feat, norm, target = _mlp_case(rows=23)
+    p64 = p.astype(np.float64)
+    g = mlp.backward(p64, feat, target, norm, rounding=False)
+
+    # the oracle's forward casts parameters to float32; differentiate a float64 forward instead
+    def Lf(q):
+        out, _ = _fwd64(mlp, q, feat, norm)
+        tn = (target[:, :6] - norm[64:70]) / norm[80:86]
+        return float(np.mean((out[:, :6] - tn) ** 2))
+
+    rng = np.random.default_rng(1)
+    eps = 1e-6
+    for _ in range(40):
+        d = rng.standard_normal(p64.size)
+        fd = (Lf(p64 + eps * d) - Lf(p64 - eps * d)) / (2 * eps)
+        assert abs(fd - g @ d) <= 1e-6 * max(1.0, abs(fd)), (fd, g @ d)
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in mlp.SHAPES])
+    for a, b in zip(offs[:-1], offs[1:]):
+        for q in rng.integers(a, b, 5):
+            e = np.zeros(p64.size)
+            e[q] = 1.0
+            fd = (Lf(p64 + eps * e) - Lf(p64 - eps * e)) / (2 * eps)
+            assert abs(fd - g[q]) <= 1e-6 + 1e-5 * abs(fd), (q, fd, g[q])
+
+
+def _fwd64(mlp, q, feat, norm):
+    """float64 forward without the float32 parameter cast (for finite differences)."""
+    P = mlp.unflatten(q)
+    x = (feat.astype(np.float64) - norm[:32]) * norm[32:64]
+    h = x
+    for L in (1, 2, 3):
+        h = np.maximum(h @ P[f"W{L}"].T + P[f"b{L}"], 0.0)
+    return h @ P["W4"].T + P["b4"], None
+
+
+def test_mlp_adam_matches_torch():
+    import torch
+    from oracle import mlp
+    rng = np.random.default_rng(2)
+    p0 = rng.standard_normal(50).astype(np.float32)
+    grads = [rng.standard_normal(50).astype(np.float32) for _ in range(3)]
+    t = torch.nn.Parameter(torch.from_numpy(p0.copy()))
+    opt = torch.optim.Adam([t], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    p, m, v = p0, np.zeros(50, np.float32), np.zeros(50, np.float32)
+    for k, g in enumerate(grads, 1):
+        t.grad = torch.from_numpy(g.copy())
+        opt.step()
+        p, m, v = mlp.adam(p, g, m, v, k, 1e-2)
+    np.testing.assert_allclose(p, t.detach().numpy(), rtol=2e-6, atol=2e-7)
+
+
+def test_mlp_feature_target_guess_maps():
+    from oracle import mlp
+    rng = np.random.default_rng(4)
+    Nr, Nth, Nz = 2, 3, 5
+    n = Nr * Nth * Nz
+    N = 6 * n + Nth * Nz
+    y = rng.random(N) + 0.5
+    s = 1e-3 + 1e-4 * np.abs(y)
+    zp = y + s * rng.standard_normal(N)
+    zc = zp + s * rng.standard_normal(N)
+    R = rng.standard_normal(N)
+    F = mlp.features(y, zp, s, R, n, Nz, np.log(2.0), 1.0)
+    # z-neighbours: interior cells see k-1 / k+1, column ends see themselves
+    c = 7                                   # k = 2 of column 1
+    assert np.allclose(F[c, 6:12], np.float32(y[:6 * n].reshape(6, n)[:, c - 1]))
+    assert np.allclose(F[c, 12:18], np.float32(y[:6 * n].reshape(6, n)[:, c + 1]))
+    assert np.allclose(F[5, 6:12], F[5, 0:6]) and np.allclose(F[9, 12:18], F[9, 0:6])
+    assert np.all(F[:, 30] == np.float32(np.log(2.0))) and np.all(F[:, 31] == 1.0)
+    # guess of the exact targets with identity normalisation reproduces z_conv on the cells
+    T = mlp.targets(zc, zp, s, n)
+    norm = np.concatenate([np.zeros(32), np.ones(32), np.zeros(16), np.ones(16)]).astype(np.float32)
+    out = np.zeros((n, 16), np.float32)
+    out[:, :8] = T
+    z = mlp.apply_guess(out, norm, zp, s, n)
+    np.testing.assert_allclose(z[:6 * n], zc[:6 * n], rtol=1e-6)
+    assert np.array_equal(z[6 * n:], zp[6 * n:])
+    # a large negative correction is truncated to 0.1 z_pred on c and q, not on e
+    z = mlp.apply_guess(np.full((n, 16), -1e9, np.float32), norm, zp, s, n)
+    assert np.array_equal(z[:5 * n], 0.1 * zp[:5 * n])
+    assert np.all(z[5 * n:6 * n] < zp[5 * n:6 * n] - 1e5)
