@@ -15,7 +15,8 @@ line Gauss-Seidel half-sweep, from every 8th of its launches in the timed region
 events on its stream (algorithmic bytes per launch / average launch time); the residual kernel
 pair timed the same way as a secondary figure; a CPU baseline: the NumPy/SciPy oracle (same
 algorithms as upstream, 1 thread) on the same state for a bounded wall-clock sample, with the GPU
-wall time over the same simulated window; and the learned form (``learned_fp32``): the same step
+wall time over the same simulated window; one full cycle (``cycle``: adsorption, blowdown,
+heating, cooling, 30,860 s) on the same grid; and the learned form (``learned_fp32``): the same step
 with the learned stage guess and FP32 preconditioner factors, the network trained on the device
 from a plain run of another seeded state, its rate against the plain FP64 rate, its balances and
 its end-field differences from the plain FP64 run.
@@ -160,6 +161,28 @@ def field_errors(st, ref, cfg):
     return {"max_rel": rel, "max_tol_units": tol}
 
 
+def cycle_leg(cfg, grid, props, state, args):
+    """One full adsorption -> blowdown -> heating -> cooling cycle (30,860 s) on the same grid from
+    the same seeded state, plain FP64, through the cycle driver (vtsa_cycle.py:82-84 order)."""
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.cycle import run_cycle
+    from dac_vtsa_amd import balances
+    kw = {"integrator_kw": {"precision": "fp64", "gs_sweeps": args.sweeps, "conserve": args.conserve}}
+    dv.synchronize()
+    t0 = time.time()
+    _, steps, _ = run_cycle(state, cfg, grid, props, 1, **kw)
+    dv.synchronize()
+    wall = time.time() - t0
+    sim = sum(s.duration for s in steps)
+    mb, eb, _ = balances.check_cycle(steps, props.species)
+    _log(f"cycle: {sim:.0f} s simulated in {wall:.2f} s")
+    return {"value": sim / wall, "unit": "sim-s/wall-s", "sim_s": sim, "wall_s": wall,
+            "max_mass_rel_err": max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE"),
+            "max_energy_rel_err": max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE"),
+            "steps": {s.name: {"sim_s": s.duration, "wall_s": s.wall_time, "accepted": s.solver["steps"],
+                               "gmres_iters": s.solver.get("gmres_iters")} for s in steps}}
+
+
 def learned_leg(cfg, grid, props, state, args, st_plain, plain_value):
     """Learned stage guess + FP32 preconditioner on the same heating step.  The network is trained
     on the device from a plain FP64 run of another seeded state (``--train-seed``): that run
@@ -233,6 +256,7 @@ def main():
                     help="stage conservation: upstream polish, or the conservation projection first")
     ap.add_argument("--copy-probe", action="store_true", help="only run the traffic-counter calibration copy")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess + FP32 preconditioner leg")
+    ap.add_argument("--no-cycle", action="store_true", help="skip the full-cycle leg")
     ap.add_argument("--train-seed", type=int, default=1, help="seed of the state whose run trains the network")
     ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
@@ -299,6 +323,8 @@ def main():
         "residual_pair": {"kernels": "k_props + k_flux", "avg_launch_us": res_s * 1e6,
                           "algo_bytes_per_launch": res_bytes, "achieved_GBs": res_bytes / res_s / 1e9},
     }
+    if not args.no_cycle:
+        line["cycle"] = cycle_leg(cfg, grid, props, state, args)
     if not args.no_learned:
         line["learned_fp32"] = learned_leg(cfg, grid, props, state, args, st_plain, value)
     if not args.no_cpu:
