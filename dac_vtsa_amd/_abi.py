@@ -94,6 +94,8 @@ _SIGS = {
     "vtsa_copy_d2d": [P, P, C.c_size_t, P],
     "vtsa_fill": [P, D, C.c_size_t, P],
     "vtsa_sync": [P],
+    "vtsa_stream_create": [C.POINTER(P)],
+    "vtsa_stream_destroy": [P],
     "vtsa_time_rhs": [P, D, P, P, I32, C.POINTER(D), P],
     "vtsa_profile": [P, I32],
     "vtsa_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(D), C.POINTER(D)],

@@ -1,0 +1,132 @@
+"""Many independent cases on one GPU (no upstream counterpart in the engine; the reference runs
+sensitivity cases one per process, analysis/sensitivity.py:63-67,121).
+
+Each worker thread owns a HIP stream and runs its share of the cases through the drop-in step
+runners; every device call of a case goes to its worker's stream, so the kernels of different
+cases overlap on the GPU while each case's own sequence stays ordered.  A case's arithmetic does
+not depend on what runs beside it: results are identical to running the cases one after another.
+
+Cases follow the batched-sensitivity workload: grid N_r in 4..12, N_theta in {1, 3..16},
+N_z in 20..100, T_heat ~ U(363, 383) K, P_vacuum ~ U(5e3, 2e4) Pa, feed RH ~ U(0.2, 0.8),
+feed y_CO2 = 400 ppm, drawn per case from one seed.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import device as dv
+
+
+def draw_cases(n_cases: int, seed: int = 0) -> List[Dict]:
+    """Per-case grid and operating parameters (fixed draw order per case)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n_cases):
+        Nr = int(rng.integers(4, 13))
+        Nth = int(rng.choice([1] + list(range(3, 17))))
+        Nz = int(rng.integers(20, 101))
+        cases.append({"id": k, "seed": int(rng.integers(0, 2 ** 31 - 1)),
+                      "overrides": {"grid.Nr": Nr, "grid.Ntheta": Nth, "grid.Nz": Nz,
+                                    "operation.T_heat": float(rng.uniform(363.0, 383.0)),
+                                    "operation.P_vacuum": float(rng.uniform(5e3, 2e4)),
+                                    "feed.RH": float(rng.uniform(0.2, 0.8)), "feed.y_co2_dry": 4e-4}})
+    return cases
+
+
+def run_case(case: Dict, mode: str = "cycle", state_kind: str = "small", base_overrides: Optional[Dict] = None,
+             integrator_kw: Optional[Dict] = None, t_end: Optional[float] = None) -> Dict:
+    """One case on the calling thread's current stream.  mode: "cycle" (one full cycle) or the
+    name of one step of the cycle.  t_end (optional) shortens every step to that many seconds.
+    Returns the case's simulated and wall seconds, balances, KPIs (cycle) and end state."""
+    from .balances import check_cycle, cycle_kpis
+    from .config import load_config
+    from .core.grid import Grid
+    from .cycle import run_cycle
+    from .properties import get_backend
+    from .seeds import seeded_state
+    from .steps import run_step
+    over = {**(base_overrides or {}), **case["overrides"]}
+    cfg = load_config(override_dict=over)
+    if t_end is not None:
+        for st in cfg.cycle:
+            st["duration"] = min(float(st["duration"]), float(t_end))
+    grid = Grid.from_config(cfg)
+    props = get_backend(cfg)
+    qstar = None
+    if state_kind == "small":
+        # the seeded loadings are fractions of the local equilibrium: device isotherms of a
+        # model built on a provisional state of the same grid
+        from .core.residual import TVSAModel
+        probe = TVSAModel(cfg, grid, props, cfg.cycle[0], seeded_state(grid, "midcycle", case["seed"]))
+        qstar = probe.equilibrium
+    state = seeded_state(grid, state_kind, case["seed"], qstar)
+    if qstar is not None:
+        probe.close()
+    kw = {"integrator_kw": dict(integrator_kw or {})}
+    out = {"id": case["id"], "cells": grid.ncell, "grid": [grid.Nr, grid.Nth, grid.Nz], "ok": True}
+    t0 = time.time()
+    try:
+        if mode == "cycle":
+            st, steps, _ = run_cycle(state, cfg, grid, props, 1, **kw)
+        else:
+            step = next(s for s in cfg.cycle if s["name"] == mode)
+            st, sr = run_step(state, cfg, grid, props, step, **kw)
+            steps = [sr]
+        dv.synchronize()
+    except Exception as e:                     # a failed case is reported, the batch goes on
+        out.update(ok=False, error=f"{type(e).__name__}: {e}", wall_s=time.time() - t0, sim_s=0.0)
+        return out
+    out["wall_s"] = time.time() - t0
+    out["sim_s"] = sum(s.duration for s in steps)
+    mb, eb, _ = check_cycle(steps, props.species)
+    out["max_mass_rel_err"] = max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE")
+    out["max_energy_rel_err"] = max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE")
+    if mode == "cycle":
+        k = cycle_kpis(steps, cfg, props.species)
+        out["kpis"] = {q: k.get(q) for q in ("recovery_pct", "purity_dry_pct", "specific_total_kJ_per_kg",
+                                             "productivity_kg_m3_day") if q in k}
+    out["state"] = st
+    return out
+
+
+def run_batch(cases: List[Dict], workers: int = 16, log=None, **kw) -> Dict:
+    """Run the cases on ``workers`` threads, one HIP stream each; larger cases are started first.
+    ``log`` (optional) receives one line per finished case.  Returns {"cases": [per-case dicts in
+    input order], "wall_s", "sim_s", "value" (aggregate simulated seconds per wall second)}."""
+    order = sorted(range(len(cases)), key=lambda i: -np.prod(list(cases[i]["overrides"].get(g, 1) for g in
+                                                                 ("grid.Nr", "grid.Ntheta", "grid.Nz"))))
+    local = threading.local()
+    streams, lock = [], threading.Lock()
+
+    def work(i):
+        s = getattr(local, "stream", None)
+        if s is None:
+            s = local.stream = dv.new_stream()
+            with lock:
+                streams.append(s)
+        with dv.use_stream(s):
+            r = run_case(cases[i], **kw)
+        if log:
+            log(f"case {r['id']} {r['grid']}: {'ok' if r['ok'] else r['error']}, {r['sim_s']:.0f} s simulated "
+                f"in {r['wall_s']:.2f} s")
+        return i, r
+
+    results = [None] * len(cases)
+    dv.synchronize()
+    t0 = time.time()
+    try:
+        with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+            for i, r in ex.map(work, order):
+                results[i] = r
+        wall = time.time() - t0
+    finally:
+        for s in streams:
+            dv.free_stream(s)
+    sim = sum(r["sim_s"] for r in results)
+    return {"cases": results, "wall_s": wall, "sim_s": sim, "value": sim / wall,
+            "failed": [r["id"] for r in results if not r["ok"]]}

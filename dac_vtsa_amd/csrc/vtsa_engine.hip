@@ -153,6 +153,9 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
     rc |= check(hipMemset(h.Jc, 0, sizeof(double) * (size_t)h.n * NSLOT * 36), "hipMemset");
   }
   if (rc == 0) rc = grow_basis(h, 30);
+  // the set-up copies and fills above run on the null stream: finish them before the handle is
+  // used from a stream that does not synchronise with it
+  if (rc == 0) rc |= check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
   if (rc != 0) {
     free_handle(new Handle(h));
     delete H;
@@ -381,6 +384,21 @@ int vtsa_fill(double* dst, double value, size_t count, vtsa_stream st) {
 }
 
 int vtsa_sync(vtsa_stream st) { return check(hipStreamSynchronize(S(st)), "hipStreamSynchronize"); }
+
+int vtsa_stream_create(vtsa_stream* out) {
+  if (!out) { set_error("vtsa_stream_create: null argument"); return -1; }
+  hipStream_t s = nullptr;
+  // default priority; does not synchronise with the null stream
+  if (check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return -1;
+  *out = (vtsa_stream)s;
+  return 0;
+}
+
+int vtsa_stream_destroy(vtsa_stream st) {
+  if (!st) return 0;
+  if (check(hipStreamSynchronize(S(st)), "hipStreamSynchronize")) return -1;
+  return check(hipStreamDestroy(S(st)), "hipStreamDestroy");
+}
 
 int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st) {
   if (!H || !y || !f || !ms || repeat < 1) return -1;

@@ -442,7 +442,8 @@ int vtsa_mlp_create(int32_t max_rows, vtsa_mlp** out) {
           check(hipMemset(m.m2, 0, NPARAM * sizeof(float)), "hipMemset") ||
           check(hipMemset(m.wb, 0, NPARAM * sizeof(__bf16)), "hipMemset") ||
           check(hipMemset(m.wt, 0, NPARAM * sizeof(__bf16)), "hipMemset") ||
-          check(hipMemcpy(m.norm, ones.data(), NNORM * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+          check(hipMemcpy(m.norm, ones.data(), NNORM * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy") ||
+          check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");   // null-stream fills done
   }
   if (bad) {
     free_mlp(m);

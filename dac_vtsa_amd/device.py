@@ -1,17 +1,29 @@
-"""Device vectors owned through the libvtsa C ABI (HBM allocation, copies, default stream).
+"""Device vectors owned through the libvtsa C ABI (HBM allocation, copies, streams).
 
 The package drives the GPU only through libvtsa: it needs no other HIP runtime in the process
 (PyTorch-ROCm ships its own, which must not be mixed with the one libvtsa links).
+
+Streams: every call takes the calling thread's current stream (``stream()``), the null stream
+unless ``use_stream`` set another one, so that independent cases driven from different threads
+run concurrently.  Allocation: freed arrays go to a cache keyed by (stream, bytes) and are
+reused by later allocations on the same stream, which keeps hipMalloc/hipFree (and the device
+synchronisation of hipFree) out of the integrator's loops; reuse on the same stream is ordered
+after every earlier use by the stream itself.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import threading
 
 import numpy as np
 
 from . import _abi
 
 _checked = False
+_tls = threading.local()
+_cache = {}                      # (stream value, nbytes) -> [pointer values]
+_cache_lock = threading.Lock()
 
 
 def _ensure_device():
@@ -24,18 +36,75 @@ def _ensure_device():
         _checked = True
 
 
+def stream():
+    """The calling thread's current stream (None = the null stream)."""
+    return getattr(_tls, "stream", None)
+
+
+@contextlib.contextmanager
+def use_stream(s):
+    """Run the enclosed calls of this thread on stream s."""
+    prev = stream()
+    _tls.stream = s
+    try:
+        yield s
+    finally:
+        _tls.stream = prev
+
+
+def new_stream():
+    _ensure_device()
+    s = C.c_void_p()
+    _abi.call("vtsa_stream_create", C.byref(s))
+    return s
+
+
+def free_stream(s):
+    """Synchronise and destroy a stream; its cached buffers are freed."""
+    if s is None or not s.value:
+        return
+    _abi.call("vtsa_stream_destroy", s)
+    release_cache(s)
+
+
+def release_cache(s=None):
+    """Free the cached buffers of stream s (all streams when s is None)."""
+    key = None if s is None else (s.value if isinstance(s, C.c_void_p) else s)
+    with _cache_lock:
+        keys = [k for k in _cache if s is None or k[0] == key]
+        blocks = [p for k in keys for p in _cache.pop(k)]
+    if blocks:
+        _abi.call("vtsa_sync", None)
+        for p in blocks:
+            _abi.lib().vtsa_device_free(C.c_void_p(p))
+
+
+def _stream_key():
+    s = stream()
+    return s.value if isinstance(s, C.c_void_p) else s
+
+
 class DeviceArray:
     """A contiguous vector in device memory (FP64 unless another dtype is given)."""
 
-    __slots__ = ("n", "dtype", "_p", "__weakref__")
+    __slots__ = ("n", "dtype", "_p", "_key", "__weakref__")
 
     def __init__(self, n: int, dtype=np.float64):
         _ensure_device()
         self.n = int(n)
         self.dtype = np.dtype(dtype)
-        p = C.c_void_p()
-        _abi.call("vtsa_device_alloc", C.c_size_t(self.dtype.itemsize * max(self.n, 1)), C.byref(p))
+        nb = self.dtype.itemsize * max(self.n, 1)
+        key = (_stream_key(), nb)
+        with _cache_lock:
+            lst = _cache.get(key)
+            pv = lst.pop() if lst else None
+        if pv is None:
+            p = C.c_void_p()
+            _abi.call("vtsa_device_alloc", C.c_size_t(nb), C.byref(p))
+        else:
+            p = C.c_void_p(pv)
         self._p = p
+        self._key = key
 
     @property
     def nbytes(self):
@@ -52,7 +121,8 @@ class DeviceArray:
         p = getattr(self, "_p", None)
         if p is not None and p.value:
             try:
-                _abi.lib().vtsa_device_free(p)
+                with _cache_lock:
+                    _cache.setdefault(self._key, []).append(p.value)
             except Exception:
                 pass
             self._p = None
@@ -61,18 +131,18 @@ class DeviceArray:
         a = np.ascontiguousarray(a, dtype=self.dtype).ravel()
         if a.size != self.n:
             raise ValueError(f"size mismatch {a.size} != {self.n}")
-        _abi.call("vtsa_copy_h2d", self._p, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), None)
+        _abi.call("vtsa_copy_h2d", self._p, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), stream())
         return self
 
     def numpy(self):
         out = np.empty(self.n, dtype=self.dtype)
-        _abi.call("vtsa_copy_d2h", out.ctypes.data_as(C.c_void_p), self._p, C.c_size_t(self.nbytes), None)
+        _abi.call("vtsa_copy_d2h", out.ctypes.data_as(C.c_void_p), self._p, C.c_size_t(self.nbytes), stream())
         return out
 
     def copy_from(self, other: "DeviceArray"):
         if other.n != self.n or other.dtype != self.dtype:
             raise ValueError("size mismatch")
-        _abi.call("vtsa_copy_d2d", self._p, other._p, C.c_size_t(self.nbytes), None)
+        _abi.call("vtsa_copy_d2d", self._p, other._p, C.c_size_t(self.nbytes), stream())
         return self
 
     def clone(self) -> "DeviceArray":
@@ -87,7 +157,7 @@ class DeviceArray:
     def fill(self, v: float):
         if self.dtype != np.float64:
             raise TypeError("fill is FP64 only")
-        _abi.call("vtsa_fill", self._p, float(v), C.c_size_t(self.n), None)
+        _abi.call("vtsa_fill", self._p, float(v), C.c_size_t(self.n), stream())
         return self
 
 
@@ -109,7 +179,7 @@ class DeviceView:
     def numpy(self):
         out = np.empty(self.n, dtype=self.dtype)
         _abi.call("vtsa_copy_d2h", out.ctypes.data_as(C.c_void_p), self._p,
-                  C.c_size_t(self.n * self.dtype.itemsize), None)
+                  C.c_size_t(self.n * self.dtype.itemsize), stream())
         return out
 
 
@@ -135,10 +205,6 @@ def ptr(x):
     return None if x is None else x.ptr
 
 
-def stream():
-    """All work runs on the default stream (NULL)."""
-    return None
-
-
 def synchronize():
-    _abi.call("vtsa_sync", None)
+    """Wait for the calling thread's current stream."""
+    _abi.call("vtsa_sync", stream())

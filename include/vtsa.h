@@ -190,6 +190,11 @@ int vtsa_copy_d2h(void* dst, const void* src, size_t bytes, vtsa_stream st);
 int vtsa_copy_d2d(void* dst, const void* src, size_t bytes, vtsa_stream st);
 int vtsa_fill(double* dst, double value, size_t count, vtsa_stream st);
 int vtsa_sync(vtsa_stream st);
+/* A stream of its own for one case (default priority, not synchronised with the null stream),
+ * so that independent cases run concurrently on one GPU.  Handles may be used from any stream;
+ * calls on one handle must stay on one stream. */
+int vtsa_stream_create(vtsa_stream* out);
+int vtsa_stream_destroy(vtsa_stream st);
 /* Elapsed milliseconds of `repeat` back-to-back residual evaluations f = f(t, y) on stream st,
  * measured with HIP events recorded on that stream (kernel timing for the roofline). */
 int vtsa_time_rhs(vtsa_handle* h, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st);
