@@ -95,6 +95,13 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   h.ncol = Nr * Nth;
   const size_t nwall = (size_t)Nth * Nz;
   int rc = 0;
+  {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      h.lds_max = v;
+    (void)hipGetLastError();
+  }
   rc |= upload(&h.V, V, Nr);
   rc |= upload(&h.Az, Az, Nr);
   rc |= upload(&h.Ath, Ath, Nr);
@@ -418,6 +425,14 @@ int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc ? rc : finish(S(st));
+}
+
+int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {
+  if (!H) { set_error("vtsa_set_option: null handle"); return -1; }
+  switch (option) {
+    case VTSA_OPT_GS_WHOLE_LINES: H->h.gs_full = value != 0; return 0;
+    default: set_error("vtsa_set_option: unknown option"); return -2;
+  }
 }
 
 int vtsa_profile(vtsa_handle* H, int32_t every) {

@@ -8,6 +8,8 @@
 //  * wall rows: point Jacobi on the wall diagonal.
 //  * line Gauss-Seidel: the z-line solves above, swept over coloured lines (cell z-lines and wall
 //    lines) so that the r and theta couplings are taken into account as well.
+#include <mutex>
+
 #include "vtsa_internal.h"
 #include "vtsa_reduce.h"
 
@@ -373,6 +375,93 @@ __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, dou
   lds_barrier();
 }
 
+// ------------------------------------------------------------------ whole-line staging
+// When the device allows a block that much LDS, a line Gauss-Seidel half-sweep copies its line's
+// whole factor (180 values per cell, contiguous in HBM) into LDS at kernel start, so the copy
+// runs under the right-hand-side gather and the sweeps never wait for memory.
+template <typename T>
+__host__ __device__ constexpr size_t line_buf_bytes(int Nz) {   // whole 1 KiB copies
+  return ((size_t)Nz * 180 * sizeof(T) + 1023) / 1024 * 1024;
+}
+template <typename T>
+__host__ __device__ constexpr size_t gs_full_lds_bytes(int Nz) {
+  return sizeof(double) * ((size_t)Nz * 6 + 8) + line_buf_bytes<T>(Nz);
+}
+
+// Copy the factor of cells [X0, X0 + Nz) into dst: 16-byte chunks, 64 per instruction, straight
+// into LDS.  Lanes past the end re-read the last chunk into the buffer's padding.
+template <typename T>
+__device__ inline void stage_line(const T* __restrict__ F, int X0, int Nz, T* dst) {
+  const int total = Nz * 180 * (int)sizeof(T) / 16;
+  const char* base = reinterpret_cast<const char*>(F + (size_t)X0 * 180);
+  char* d = reinterpret_cast<char*>(dst);
+  for (int j = 0; j * 64 < total; ++j) {
+    const int c = j * 64 + (int)threadIdx.x;
+    const int cc = c < total ? c : total - 1;
+    __builtin_amdgcn_global_load_lds(base + (size_t)cc * 16, (__attribute__((address_space(3))) void*)(d + j * 1024),
+                                     16, 0, 0);
+  }
+}
+
+// zline_sweeps with the line's factor resident in LDS (Fl[k][180]: L2 | L1 | Dinv | U1 | U2).
+// Two cells per loop pass with alternating register roles, so the recurrence carries no copies;
+// every lane stores its row (lanes past 5 repeat row 0's value), so no exec-mask switching.
+template <typename T>
+__device__ inline void zline_sweeps_full(const T* Fl, int Nz, double* xs) {
+  const int lane = threadIdx.x;
+  const int r = lane < 6 ? lane : 0;
+  // forward: z_k = b_k - L1 z_{k-1} - L2 z_{k-2}; za, zb hold the last two z (roles alternate)
+  double za[6], zb[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) { za[c] = 0.0; zb[c] = 0.0; }
+  auto fwd = [&](int k, const double (&zm1)[6], double (&zm2_out)[6]) {
+    const T* R = Fl + (size_t)k * 180 + r * 6;
+    double s1 = 0.0, s2 = 0.0;   // the factor holds zero blocks where k < 2
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      s1 = fma((double)R[36 + c], zm1[c], s1);
+      s2 = fma((double)R[c], zm2_out[c], s2);
+    }
+    const double zk = (xs[k * 6 + r] - s1) - s2;
+    xs[k * 6 + r] = zk;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) zm2_out[c] = lane_value(zk, c);
+  };
+  int k = 0;
+  for (; k + 1 < Nz; k += 2) {
+    fwd(k, za, zb);        // zb <- z_k       (za = z_{k-1})
+    fwd(k + 1, zb, za);    // za <- z_{k+1}   (zb = z_k)
+  }
+  if (k < Nz) fwd(k, za, zb);
+  // backward: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2})
+  double xa[6], xb[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) { xa[c] = 0.0; xb[c] = 0.0; }
+  auto bwd = [&](int k, const double (&xp1)[6], double (&xp2_out)[6]) {
+    const T* R = Fl + (size_t)k * 180 + 72 + r * 6;
+    double s1 = 0.0, s2 = 0.0;   // zero blocks where k + 2 > Nz - 1
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      s1 = fma((double)R[36 + c], xp1[c], s1);
+      s2 = fma((double)R[72 + c], xp2_out[c], s2);
+    }
+    const double tk = (xs[k * 6 + r] - s1) - s2;
+    double xk = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xk = fma((double)R[c], lane_value(tk, c), xk);
+    xs[k * 6 + r] = xk;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xp2_out[c] = lane_value(xk, c);
+  };
+  k = Nz - 1;
+  for (; k - 1 >= 0; k -= 2) {
+    bwd(k, xa, xb);        // xb <- x_k       (xa = x_{k+1})
+    bwd(k - 1, xb, xa);    // xa <- x_{k-1}   (xb = x_k)
+  }
+  if (k >= 0) bwd(k, xa, xb);
+  lds_barrier();
+}
+
 // ------------------------------------------------------------------ z-line / block-Jacobi apply
 // One z-column per 64-lane block; wall rows (point Jacobi) in blocks of their own.
 template <typename T>
@@ -543,7 +632,8 @@ __global__ void k_gs_gather(int N, ApplyArgs a, const double* __restrict__ s0, d
   U[i] = 0.0;
 }
 
-template <typename T>
+// FULL: the line's whole factor is staged into LDS at kernel start (see stage_line).
+template <typename T, bool FULL>
 __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw,
                                                 const T* __restrict__ F, GsArgs g, int cbeg,
                                                 const GmresScalars* __restrict__ gs, int respect_done) {
@@ -559,13 +649,15 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
     const int X0 = L * Nz;
     T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);
     T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
-    zline_stage_first<T>(F, X0, Nz, fb0);
+    if (FULL) stage_line<T>(F, X0, Nz, fb0);
+    else zline_stage_first<T>(F, X0, Nz, fb0);
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
     const int nb[4] = {i > 0 ? ((i - 1) * Nth + j) * Nz : -1, i + 1 < Nr ? ((i + 1) * Nth + j) * Nz : -1,
                        Nth > 1 ? (i * Nth + jm) * Nz : -1, Nth > 1 ? (i * Nth + jp) * Nz : -1};
     const bool wall = nw > 0 && i == Nr - 1;
-    for (int e = lane; e < 6 * Nz; e += PW) {
+    // right-hand side of element e = (row r, cell k): b~ + (dh / s0) (off-line J) U
+    auto rhs = [&](int e) -> double {
       const int r = e / Nz, k = e - r * Nz;
       const int X = X0 + k;
       double acc = 0.0;
@@ -584,9 +676,24 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
         }
       }
       const int idx = r * n + X;
-      xs[k * 6 + r] = g.bs[idx] + (dh / g.s0[idx]) * acc;
+      return g.bs[idx] + (dh / g.s0[idx]) * acc;
+    };
+    // two elements per lane per pass, so both elements' loads are in flight together
+    for (int e = lane; e < 6 * Nz; e += 2 * PW) {
+      const int e1 = e + PW;
+      const double v0 = rhs(e);
+      if (e1 < 6 * Nz) {
+        const double v1 = rhs(e1);
+        xs[(e1 % Nz) * 6 + e1 / Nz] = v1;
+      }
+      xs[(e % Nz) * 6 + e / Nz] = v0;
     }
-    zline_sweeps<T>(F, X0, Nz, xs, fb0, fb1);
+    if (FULL) {
+      stage_wait();
+      zline_sweeps_full<T>(fb0, Nz, xs);
+    } else {
+      zline_sweeps<T>(F, X0, Nz, xs, fb0, fb1);
+    }
     for (int e = lane; e < 6 * Nz; e += PW) {
       const int r = e / Nz, k = e - r * Nz;
       const int idx = r * n + X0 + k;
@@ -790,9 +897,30 @@ int prof_resolve(Handle& h) {
   return rc;
 }
 
+// Whether the whole-line variant of the half-sweep can run: its LDS fits the device's limit for
+// one block and the kernel accepts that much dynamic LDS (asked for once per element type).
+template <typename T>
+static bool gs_full_ok(const Handle& h) {
+  static std::mutex mu;
+  static int granted = -1;   // -1 not asked, 0 refused, 1 granted (for the largest size asked)
+  static size_t granted_bytes = 0;
+  const size_t need = gs_full_lds_bytes<T>(h.desc.Nz);
+  if (need > (size_t)h.lds_max) return false;
+  std::lock_guard<std::mutex> lock(mu);
+  if (granted == 1 && need <= granted_bytes) return true;
+  if (granted == 0 && need >= granted_bytes) return false;
+  const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gs_half<T, true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) == hipSuccess;
+  (void)hipGetLastError();
+  granted = ok ? 1 : 0;
+  granted_bytes = need;
+  return ok;
+}
+
 template <typename T>
 static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   const int N = h.N, Nz = h.desc.Nz;
+  const bool full = h.gs_full && gs_full_ok<T>(h);
   hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
   GsArgs g{h.Jc, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1};
   const int C = (int)h.gs_cstart.size() - 1;
@@ -815,8 +943,12 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       e1 = prof_event(h);
       if (e0 && e1) (void)hipEventRecord(e0, s);
     }
-    hipLaunchKernelGGL(k_gs_half<T>, dim3(cnt), dim3(PW), zline_lds_bytes<T>(Nz), s, h.n, h.desc.Nr, h.desc.Nth, Nz,
-                       h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+    if (full)
+      hipLaunchKernelGGL((k_gs_half<T, true>), dim3(cnt), dim3(PW), gs_full_lds_bytes<T>(Nz), s, h.n, h.desc.Nr,
+                         h.desc.Nth, Nz, h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+    else
+      hipLaunchKernelGGL((k_gs_half<T, false>), dim3(cnt), dim3(PW), zline_lds_bytes<T>(Nz), s, h.n, h.desc.Nr,
+                         h.desc.Nth, Nz, h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
     if (sample && e0 && e1) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);

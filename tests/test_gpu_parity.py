@@ -270,6 +270,7 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     FP32 factors are checked on the 3x4x6 state, whose scaled z-line blocks have condition
     numbers up to 3e6; on the seeded 4x4x20 state they reach 3e11 at this dh, beyond what FP32
     factors can represent."""
+    from dac_vtsa_amd import _abi
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.core.jacobian import FDJacobian
     from dac_vtsa_amd.core.solver import DeviceOps, PRECONDITIONERS
@@ -283,10 +284,15 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     s0 = ops.scale(dv.to_dev(y), 1e-4)
     ops.precond_setup(dh, s0, PRECONDITIONERS["zline_gs"], prec, sweeps)
     r = np.random.default_rng(0).standard_normal(m.n_core) * np.abs(y)
-    out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
     ref = _line_gs_reference(np.eye(m.n_core) - dh * J, m, r, sweeps)
     sc = dv.to_host(s0)
-    assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
+    outs = []
+    for whole in (1, 0):     # whole-line LDS staging and segmented staging: identical arithmetic
+        _abi.call("vtsa_set_option", m.handle, 1, whole)
+        out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
+        assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
+        outs.append(out)
+    assert np.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("dh,rtol", [(0.05, 1e-2), (0.05, 1e-4), (0.3, 1e-2)])
