@@ -430,7 +430,10 @@ int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t 
 int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {
   if (!H) { set_error("vtsa_set_option: null handle"); return -1; }
   switch (option) {
-    case VTSA_OPT_GS_WHOLE_LINES: H->h.gs_full = value != 0; return 0;
+    case VTSA_OPT_GS_WHOLE_LINES:
+      if (value < 0 || value > 1) { set_error("vtsa_set_option: staging level must be 0 or 1"); return -2; }
+      H->h.gs_full = value;
+      return 0;
     default: set_error("vtsa_set_option: unknown option"); return -2;
   }
 }

@@ -61,7 +61,7 @@ struct Handle {
   int* gs_order = nullptr;           // device [ncol + wall lines]
   std::vector<int> gs_cstart;        // colour c owns gs_order[gs_cstart[c] .. gs_cstart[c + 1])
   int gs_sweeps = 1;
-  int gs_full = 1;                   // stage whole lines into LDS when the device allows it
+  int gs_full = 1;                   // LDS staging of a half-sweep: 0 segments, 1 whole line
   int lds_max = 65536;               // LDS bytes one block may use on this device
   double* gsb = nullptr;             // [N] scaled right-hand side
   double* gsu = nullptr;             // [N] unscaled iterate

@@ -378,34 +378,41 @@ __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, dou
 // ------------------------------------------------------------------ whole-line staging
 // When the device allows a block that much LDS, a line Gauss-Seidel half-sweep copies its line's
 // whole factor (180 values per cell, contiguous in HBM) into LDS at kernel start, so the copy
-// runs under the right-hand-side gather and the sweeps never wait for memory.
+// runs under the right-hand-side gather and the sweeps never wait for memory.  The copy is
+// spread over the block's GSW threads (four wavefronts).
+constexpr int GSW = 256;   // threads of a whole-line half-sweep block
+
 template <typename T>
-__host__ __device__ constexpr size_t line_buf_bytes(int Nz) {   // whole 1 KiB copies
-  return ((size_t)Nz * 180 * sizeof(T) + 1023) / 1024 * 1024;
+__host__ __device__ constexpr size_t line_buf_bytes(int Nz) {   // whole copies of GSW x 16 bytes
+  return ((size_t)Nz * 180 * sizeof(T) + GSW * 16 - 1) / (GSW * 16) * (GSW * 16);
 }
 template <typename T>
 __host__ __device__ constexpr size_t gs_full_lds_bytes(int Nz) {
   return sizeof(double) * ((size_t)Nz * 6 + 8) + line_buf_bytes<T>(Nz);
 }
 
-// Copy the factor of cells [X0, X0 + Nz) into dst: 16-byte chunks, 64 per instruction, straight
-// into LDS.  Lanes past the end re-read the last chunk into the buffer's padding.
+// Copy the factor of cells [X0, X0 + Nz) into dst: 16-byte chunks, GSW per pass (64 per
+// wavefront instruction), straight into LDS.  Lanes past the end re-read the last chunk into the
+// buffer's padding.  Called by all GSW threads of the block.
 template <typename T>
 __device__ inline void stage_line(const T* __restrict__ F, int X0, int Nz, T* dst) {
   const int total = Nz * 180 * (int)sizeof(T) / 16;
+  const int tid = threadIdx.x, w = tid >> 6;
   const char* base = reinterpret_cast<const char*>(F + (size_t)X0 * 180);
   char* d = reinterpret_cast<char*>(dst);
-  for (int j = 0; j * 64 < total; ++j) {
-    const int c = j * 64 + (int)threadIdx.x;
+  for (int j = 0; j * GSW < total; ++j) {
+    const int c = j * GSW + tid;
     const int cc = c < total ? c : total - 1;
-    __builtin_amdgcn_global_load_lds(base + (size_t)cc * 16, (__attribute__((address_space(3))) void*)(d + j * 1024),
+    __builtin_amdgcn_global_load_lds(base + (size_t)cc * 16,
+                                     (__attribute__((address_space(3))) void*)(d + (size_t)j * GSW * 16 + w * 1024),
                                      16, 0, 0);
   }
 }
 
 // zline_sweeps with the line's factor resident in LDS (Fl[k][180]: L2 | L1 | Dinv | U1 | U2).
-// Two cells per loop pass with alternating register roles, so the recurrence carries no copies;
-// every lane stores its row (lanes past 5 repeat row 0's value), so no exec-mask switching.
+// Run by the block's first wavefront only; the caller synchronises afterwards.  Two cells per
+// loop pass with alternating register roles, so the recurrence carries no copies; every lane
+// stores its row (lanes past 5 repeat row 0's value), so no exec-mask switching.
 template <typename T>
 __device__ inline void zline_sweeps_full(const T* Fl, int Nz, double* xs) {
   const int lane = threadIdx.x;
@@ -414,52 +421,73 @@ __device__ inline void zline_sweeps_full(const T* Fl, int Nz, double* xs) {
   double za[6], zb[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) { za[c] = 0.0; zb[c] = 0.0; }
-  auto fwd = [&](int k, const double (&zm1)[6], double (&zm2_out)[6]) {
+  // a cell's rows and right-hand side are read one cell ahead, before the current cell's store
+  // (the compiler cannot move LDS reads across an LDS store it cannot disambiguate)
+  struct FRow { T l1[6], l2[6]; double b; };
+  auto ldf = [&](int k, FRow& w) {
     const T* R = Fl + (size_t)k * 180 + r * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { w.l2[c] = R[c]; w.l1[c] = R[36 + c]; }
+    w.b = xs[k * 6 + r];
+  };
+  auto fwd = [&](int k, const FRow& w, const double (&zm1)[6], double (&zm2_out)[6]) {
     double s1 = 0.0, s2 = 0.0;   // the factor holds zero blocks where k < 2
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      s1 = fma((double)R[36 + c], zm1[c], s1);
-      s2 = fma((double)R[c], zm2_out[c], s2);
+      s1 = fma((double)w.l1[c], zm1[c], s1);
+      s2 = fma((double)w.l2[c], zm2_out[c], s2);
     }
-    const double zk = (xs[k * 6 + r] - s1) - s2;
+    const double zk = (w.b - s1) - s2;
     xs[k * 6 + r] = zk;
 #pragma unroll
     for (int c = 0; c < 6; ++c) zm2_out[c] = lane_value(zk, c);
   };
+  FRow fa, fb;
+  ldf(0, fa);
   int k = 0;
   for (; k + 1 < Nz; k += 2) {
-    fwd(k, za, zb);        // zb <- z_k       (za = z_{k-1})
-    fwd(k + 1, zb, za);    // za <- z_{k+1}   (zb = z_k)
+    ldf(k + 1, fb);
+    fwd(k, fa, za, zb);        // zb <- z_k       (za = z_{k-1})
+    if (k + 2 < Nz) ldf(k + 2, fa);
+    fwd(k + 1, fb, zb, za);    // za <- z_{k+1}   (zb = z_k)
   }
-  if (k < Nz) fwd(k, za, zb);
+  if (k < Nz) fwd(k, fa, za, zb);
   // backward: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2})
   double xa[6], xb[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) { xa[c] = 0.0; xb[c] = 0.0; }
-  auto bwd = [&](int k, const double (&xp1)[6], double (&xp2_out)[6]) {
+  struct BRow { T d[6], u1[6], u2[6]; double z; };
+  auto ldb = [&](int k, BRow& w) {
     const T* R = Fl + (size_t)k * 180 + 72 + r * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { w.d[c] = R[c]; w.u1[c] = R[36 + c]; w.u2[c] = R[72 + c]; }
+    w.z = xs[k * 6 + r];
+  };
+  auto bwd = [&](int k, const BRow& w, const double (&xp1)[6], double (&xp2_out)[6]) {
     double s1 = 0.0, s2 = 0.0;   // zero blocks where k + 2 > Nz - 1
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      s1 = fma((double)R[36 + c], xp1[c], s1);
-      s2 = fma((double)R[72 + c], xp2_out[c], s2);
+      s1 = fma((double)w.u1[c], xp1[c], s1);
+      s2 = fma((double)w.u2[c], xp2_out[c], s2);
     }
-    const double tk = (xs[k * 6 + r] - s1) - s2;
+    const double tk = (w.z - s1) - s2;
     double xk = 0.0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) xk = fma((double)R[c], lane_value(tk, c), xk);
+    for (int c = 0; c < 6; ++c) xk = fma((double)w.d[c], lane_value(tk, c), xk);
     xs[k * 6 + r] = xk;
 #pragma unroll
     for (int c = 0; c < 6; ++c) xp2_out[c] = lane_value(xk, c);
   };
+  BRow ba, bb;
+  ldb(Nz - 1, ba);
   k = Nz - 1;
   for (; k - 1 >= 0; k -= 2) {
-    bwd(k, xa, xb);        // xb <- x_k       (xa = x_{k+1})
-    bwd(k - 1, xb, xa);    // xa <- x_{k-1}   (xb = x_k)
+    ldb(k - 1, bb);
+    bwd(k, ba, xa, xb);        // xb <- x_k       (xa = x_{k+1})
+    if (k - 2 >= 0) ldb(k - 2, ba);
+    bwd(k - 1, bb, xb, xa);    // xa <- x_{k-1}   (xb = x_k)
   }
-  if (k >= 0) bwd(k, xa, xb);
-  lds_barrier();
+  if (k >= 0) bwd(k, ba, xa, xb);
 }
 
 // ------------------------------------------------------------------ z-line / block-Jacobi apply
@@ -632,13 +660,18 @@ __global__ void k_gs_gather(int N, ApplyArgs a, const double* __restrict__ s0, d
   U[i] = 0.0;
 }
 
-// FULL: the line's whole factor is staged into LDS at kernel start (see stage_line).
-template <typename T, bool FULL>
-__global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw,
-                                                const T* __restrict__ F, GsArgs g, int cbeg,
-                                                const GmresScalars* __restrict__ gs, int respect_done) {
+// MODE 0: one wavefront; the factor is staged in segments while the sweeps run (zline_sweeps).
+// MODE 1: GSW threads; the line's whole factor is staged into LDS at kernel start (stage_line),
+//         spread over four wavefronts so that no wavefront stalls on its count of outstanding
+//         copies, the right-hand side is gathered by all of them, and the first wavefront sweeps.
+//         Same arithmetic in both modes.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
+    k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ F, GsArgs g, int cbeg,
+              const GmresScalars* __restrict__ gs, int respect_done) {
   extern __shared__ __align__(16) double lds[];
   if (respect_done && gs->done) return;
+  constexpr int NT = MODE == 0 ? PW : GSW;
   const int lane = threadIdx.x;
   const int L = g.order[cbeg + blockIdx.x];
   double* xs = lds;
@@ -649,7 +682,7 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
     const int X0 = L * Nz;
     T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);
     T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
-    if (FULL) stage_line<T>(F, X0, Nz, fb0);
+    if (MODE >= 1) stage_line<T>(F, X0, Nz, fb0);
     else zline_stage_first<T>(F, X0, Nz, fb0);
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
@@ -679,8 +712,8 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
       return g.bs[idx] + (dh / g.s0[idx]) * acc;
     };
     // two elements per lane per pass, so both elements' loads are in flight together
-    for (int e = lane; e < 6 * Nz; e += 2 * PW) {
-      const int e1 = e + PW;
+    for (int e = lane; e < 6 * Nz; e += 2 * NT) {
+      const int e1 = e + NT;
       const double v0 = rhs(e);
       if (e1 < 6 * Nz) {
         const double v1 = rhs(e1);
@@ -688,13 +721,14 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
       }
       xs[(e % Nz) * 6 + e / Nz] = v0;
     }
-    if (FULL) {
+    if (MODE == 1) {
       stage_wait();
-      zline_sweeps_full<T>(fb0, Nz, xs);
+      if (lane < PW) zline_sweeps_full<T>(fb0, Nz, xs);
+      lds_barrier();
     } else {
       zline_sweeps<T>(F, X0, Nz, xs, fb0, fb1);
     }
-    for (int e = lane; e < 6 * Nz; e += PW) {
+    for (int e = lane; e < 6 * Nz; e += NT) {
       const int r = e / Nz, k = e - r * Nz;
       const int idx = r * n + X0 + k;
       g.U[idx] = g.s0[idx] * xs[k * 6 + r];
@@ -706,7 +740,7 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     double* rhs = xs;          // [Nz]
     double* cp = xs + Nz;      // [Nz] Thomas coefficients
-    for (int k = lane; k < Nz; k += PW) {
+    for (int k = lane; k < Nz; k += NT) {
       const int W = j * Nz + k;
       const int X = ((Nr - 1) * Nth + j) * Nz + k;
       double acc = 0.0;
@@ -739,7 +773,7 @@ __global__ void __launch_bounds__(PW) k_gs_half(int n, int Nr, int Nth, int Nz, 
       }
     }
     lds_barrier();
-    for (int k = lane; k < Nz; k += PW) {
+    for (int k = lane; k < Nz; k += NT) {
       const int idx = 6 * n + j * Nz + k;
       g.U[idx] = g.s0[idx] * rhs[k];
     }
@@ -897,19 +931,24 @@ int prof_resolve(Handle& h) {
   return rc;
 }
 
-// Whether the whole-line variant of the half-sweep can run: its LDS fits the device's limit for
-// one block and the kernel accepts that much dynamic LDS (asked for once per element type).
-template <typename T>
-static bool gs_full_ok(const Handle& h) {
+template <typename T, int MODE>
+static size_t gs_lds_bytes(int Nz) {
+  return MODE == 1 ? gs_full_lds_bytes<T>(Nz) : zline_lds_bytes<T>(Nz);
+}
+
+// Whether half-sweep variant MODE can run: its LDS fits the device's limit for one block and the
+// kernel accepts that much dynamic LDS (asked for once per variant and size).
+template <typename T, int MODE>
+static bool gs_mode_ok(const Handle& h) {
   static std::mutex mu;
   static int granted = -1;   // -1 not asked, 0 refused, 1 granted (for the largest size asked)
   static size_t granted_bytes = 0;
-  const size_t need = gs_full_lds_bytes<T>(h.desc.Nz);
+  const size_t need = gs_lds_bytes<T, MODE>(h.desc.Nz);
   if (need > (size_t)h.lds_max) return false;
   std::lock_guard<std::mutex> lock(mu);
   if (granted == 1 && need <= granted_bytes) return true;
   if (granted == 0 && need >= granted_bytes) return false;
-  const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gs_half<T, true>),
+  const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gs_half<T, MODE>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) == hipSuccess;
   (void)hipGetLastError();
   granted = ok ? 1 : 0;
@@ -920,7 +959,9 @@ static bool gs_full_ok(const Handle& h) {
 template <typename T>
 static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   const int N = h.N, Nz = h.desc.Nz;
-  const bool full = h.gs_full && gs_full_ok<T>(h);
+  // whole-line staging when the option allows it and the device grants the LDS
+  const int mode = (h.gs_full >= 1 && gs_mode_ok<T, 1>(h)) ? 1 : 0;
+  const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds0 = gs_lds_bytes<T, 0>(Nz);
   hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
   GsArgs g{h.Jc, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1};
   const int C = (int)h.gs_cstart.size() - 1;
@@ -943,12 +984,12 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       e1 = prof_event(h);
       if (e0 && e1) (void)hipEventRecord(e0, s);
     }
-    if (full)
-      hipLaunchKernelGGL((k_gs_half<T, true>), dim3(cnt), dim3(PW), gs_full_lds_bytes<T>(Nz), s, h.n, h.desc.Nr,
-                         h.desc.Nth, Nz, h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+    if (mode == 1)
+      hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
+                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
     else
-      hipLaunchKernelGGL((k_gs_half<T, false>), dim3(cnt), dim3(PW), zline_lds_bytes<T>(Nz), s, h.n, h.desc.Nr,
-                         h.desc.Nth, Nz, h.ncol, h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+      hipLaunchKernelGGL((k_gs_half<T, 0>), dim3(cnt), dim3(PW), lds0, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
+                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
     if (sample && e0 && e1) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);

@@ -199,9 +199,9 @@ int vtsa_stream_destroy(vtsa_stream st);
  * measured with HIP events recorded on that stream (kernel timing for the roofline). */
 int vtsa_time_rhs(vtsa_handle* h, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st);
 
-/* Engine options (no upstream counterpart).  VTSA_OPT_GS_WHOLE_LINES (default 1): a line
+/* Engine options (no upstream counterpart).  VTSA_OPT_GS_WHOLE_LINES: 1 (default) a line
  * Gauss-Seidel half-sweep stages its line's whole factor in LDS when the device grants a block
- * that much; 0 selects the segmented staging (same arithmetic, same results). */
+ * that much; 0 stages factor segments during the sweep.  Same arithmetic, bit-identical results. */
 enum { VTSA_OPT_GS_WHOLE_LINES = 1 };
 int vtsa_set_option(vtsa_handle* h, int32_t option, int32_t value);
 

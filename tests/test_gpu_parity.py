@@ -287,8 +287,8 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     ref = _line_gs_reference(np.eye(m.n_core) - dh * J, m, r, sweeps)
     sc = dv.to_host(s0)
     outs = []
-    for whole in (1, 0):     # whole-line LDS staging and segmented staging: identical arithmetic
-        _abi.call("vtsa_set_option", m.handle, 1, whole)
+    for level in (1, 0):     # whole-line and segmented LDS staging: identical arithmetic
+        _abi.call("vtsa_set_option", m.handle, 1, level)
         out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
         assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
         outs.append(out)
