@@ -183,6 +183,23 @@ def cycle_leg(cfg, grid, props, state, args):
                                "gmres_iters": s.solver.get("gmres_iters")} for s in steps}}
 
 
+def batch_leg(args):
+    """Independent sensitivity cases (grid N_r 4-12, N_theta 1-16, N_z 20-100, T_heat, P_vacuum and
+    feed RH drawn per case) run concurrently, one HIP stream per worker thread: one heating step
+    of 22,400 s each from a seeded mid-cycle state.  value = total simulated seconds / wall."""
+    from dac_vtsa_amd.batch import draw_cases, run_batch
+    cases = draw_cases(args.batch, seed=0)
+    r = run_batch(cases, workers=args.workers, log=_log, mode="heating", state_kind="midcycle",
+                  base_overrides={"solver.conservation_tol": args.cons_tol},
+                  integrator_kw={"gs_sweeps": args.sweeps, "conserve": args.conserve})
+    ok = [c for c in r["cases"] if c["ok"]]
+    return {"value": r["value"], "unit": "sim-s/wall-s (all cases)", "cases": len(cases), "workers": args.workers,
+            "wall_s": r["wall_s"], "sim_s": r["sim_s"], "failed": r["failed"],
+            "cells_range": [min(c["cells"] for c in r["cases"]), max(c["cells"] for c in r["cases"])],
+            "max_mass_rel_err": max((c["max_mass_rel_err"] for c in ok), default=None),
+            "max_energy_rel_err": max((c["max_energy_rel_err"] for c in ok), default=None)}
+
+
 def learned_leg(cfg, grid, props, state, args, st_plain, plain_value):
     """Learned stage guess + FP32 preconditioner on the same heating step.  The network is trained
     on the device from a plain FP64 run of another seeded state (``--train-seed``): that run
@@ -257,6 +274,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the traffic-counter calibration copy")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess + FP32 preconditioner leg")
     ap.add_argument("--no-cycle", action="store_true", help="skip the full-cycle leg")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also run this many independent sensitivity cases concurrently (heating step each)")
+    ap.add_argument("--workers", type=int, default=16, help="worker threads (HIP streams) of the batch leg")
     ap.add_argument("--train-seed", type=int, default=1, help="seed of the state whose run trains the network")
     ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
@@ -325,6 +345,8 @@ def main():
     }
     if not args.no_cycle:
         line["cycle"] = cycle_leg(cfg, grid, props, state, args)
+    if args.batch > 0:
+        line["batch"] = batch_leg(args)
     if not args.no_learned:
         line["learned_fp32"] = learned_leg(cfg, grid, props, state, args, st_plain, value)
     if not args.no_cpu:
