@@ -83,7 +83,7 @@ _SIGS = {
     "vtsa_cons": [P, P, P, P, P, P],
     "vtsa_inventories": [P, P, P, P],
     "vtsa_cons_norm2": [P, P, P, P],
-    "vtsa_cons_shift": [P, P, P, P, P, P],
+    "vtsa_cons_shift": [P, P, P, P, I32, P, P],
     "vtsa_vec_op": [P, I32, D, D, D, D, P, P, P, P, P, P],
     "vtsa_error_norm": [P, P, P, P, D, C.POINTER(D), P],
     "vtsa_device_count": [C.POINTER(I32)],

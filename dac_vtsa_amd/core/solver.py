@@ -124,10 +124,13 @@ class DeviceOps:
         _abi.call("vtsa_cons_norm2", self.h, dv.ptr(s), out, self.st())
         return np.array(out[:])
 
-    def cons_shift(self, z, s, lam, out):
+    def cons_shift(self, z, s, lam, out, truncate=False):
+        """out = z - S^2 W^T lam.  Returns True when a concentration or loading would fall more
+        than one tolerance unit below zero (with truncate=True those keep 0.1 of their value)."""
         lam = (C.c_double * 4)(*[float(v) for v in lam])
-        _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, dv.ptr(out), self.st())
-        return out
+        rc = _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, int(bool(truncate)), dv.ptr(out),
+                       self.st())
+        return rc == 1
 
     def error_norm(self, y, z2, y3, rtol):
         en = C.c_double()
@@ -158,15 +161,19 @@ class Factorisation:
         self.h = h
 
 
-def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3):
+def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, max_shift=1.0):
     """Shift a converged stage value along S^2 W^T so that its residual conserves the four totals.
 
     No upstream counterpart (``conserve="project"``).  The shift z(lambda) = z - S^2 W^T lambda is
     the minimum-norm change in the s-scaled variables that moves the conserved totals W z; the
     four multipliers solve W R(z(lambda)) = 0 by Newton's method on this 4x4 system, its matrix
     B = d(W R)/d(lambda) taken by finite differences along the four directions (four residual
-    evaluations per iteration; rows with no entries are left out).  The shift is of the size of
-    the Newton tolerance.  Returns (z, f, R) or None if the totals do not close."""
+    evaluations per iteration; rows with no entries are left out).  When the shift would take a
+    concentration or loading more than one tolerance unit below zero, those keep 0.1 of their
+    value instead (the integrator's truncation rule).  The projection is
+    accepted only when the totals close and the shift stays within max_shift tolerance units
+    (rms of the change over s); otherwise None is returned and the caller falls back to the
+    upstream polish."""
     d2 = ops.cons_norm2(s)
     rows = np.nonzero(d2 > 0.0)[0]
     zc, fc, Rc = z, f, R
@@ -192,13 +199,22 @@ def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3):
         except np.linalg.LinAlgError:
             return None
         zn, fn, Rn = dv.empty(ops.N), dv.empty(ops.N), dv.empty(ops.N)
-        ops.cons_shift(zc, s, lam, zn)
+        if ops.cons_shift(zc, s, lam, zn):        # a concentration or loading would go negative:
+            ops.cons_shift(zc, s, lam, zn, True)  # truncate them as a Newton update does
         rn, _ = ops.newton_eval(t, zn, dh, rhs_known, s, fn, Rn)
         if not _finite(rn):
             return None
         zc, fc, Rc = zn, fn, Rn
+        if _shift(ops, z, zc, s, zt) > max_shift:
+            return None
     wr, ws = ops.cons(Rc, s)
     return (zc, fc, Rc) if np.all(np.abs(wr) <= cons_tol * ws) else None
+
+
+def _shift(ops, z0, z1, s, tmp):
+    """rms((z1 - z0) / s): the size of a change in tolerance units."""
+    ops.vec(7, tmp, x=z0, y=z1)
+    return ops.rms(tmp, s)
 
 
 def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_dim=30, eta=1e-2, counter=None,
@@ -373,14 +389,14 @@ class TRBDF2:
         return Factorisation(h)
 
     def _start(self, t, z0, rhs_known, dh, y_ref, stage):
-        """Starting value of a stage: the predictor z0, or the learned guess when its
-        preconditioned stage residual is smaller; records the stage's features."""
+        """Starting value of a stage: the predictor z0, or the learned guess when both its
+        preconditioned and its plain stage residual are smaller; records the stage's features."""
         if self.guess is None and self.record is None:
             return z0
         h = dh / D
         ops = self.ops
         s = ops.scale(y_ref, self.rtol, self._s_ref)
-        _, pn0 = ops.newton_eval(t, z0, dh, rhs_known, s, self._gf, self._gR)
+        rn0, pn0 = ops.newton_eval(t, z0, dh, rhs_known, s, self._gf, self._gR)
         if not np.isfinite(pn0):
             return z0
         ops.prec_apply(self._gR, self._gP)          # M^-1 R(z0): minus the first Newton step
@@ -389,10 +405,12 @@ class TRBDF2:
         if self.guess is None:
             return z0
         zg = self.guess.predict(self.m.handle, y_ref, z0, s, self._gP, h, stage, self._zg)
-        _, pn1 = ops.newton_eval(t, zg, dh, rhs_known, s, self._gf, self._gR)
+        rn1, pn1 = ops.newton_eval(t, zg, dh, rhs_known, s, self._gf, self._gR)
         if np.isfinite(pn1) and pn0 > 0.0:
             self.stats["guess_log_pn_ratio"] = self.stats.get("guess_log_pn_ratio", 0.0) + float(np.log(pn1 / pn0))
-        if np.isfinite(pn1) and pn1 < pn0:
+        # both the preconditioned and the plain residual must improve: a guess that only the
+        # preconditioned norm favours can leave conservation errors Newton does not remove
+        if np.isfinite(pn1) and np.isfinite(rn1) and pn1 < pn0 and rn1 <= rn0:
             self.stats["guess_used"] += 1
             return zg
         self.stats["guess_kept_predictor"] += 1

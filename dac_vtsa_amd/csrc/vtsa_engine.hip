@@ -323,11 +323,13 @@ int vtsa_cons_norm2(vtsa_handle* H, const double* s, double* out, vtsa_stream st
   return reduce_weighted(H->h, s, nullptr, 0, 2, out, S(st));
 }
 
-int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, double* out,
-                    vtsa_stream st) {
+int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, int32_t truncate,
+                    double* out, vtsa_stream st) {
   if (!H || !z || !s || !lambda || !out) { set_error("vtsa_cons_shift: null argument"); return -1; }
-  cons_shift_impl(H->h, z, s, lambda, out, S(st));
-  return finish(S(st));
+  const int rc = cons_shift_impl(H->h, z, s, lambda, truncate ? 1 : 0, out, S(st));
+  if (rc < 0) return rc;
+  const int e = finish(S(st));
+  return e != 0 ? e : rc;
 }
 
 int vtsa_inventories(vtsa_handle* H, const double* y, double* out, vtsa_stream st) {

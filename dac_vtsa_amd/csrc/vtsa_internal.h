@@ -104,7 +104,10 @@ void launch_matvec_residual(Handle& h, double t, const double* z, const double* 
 void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, int sweeps, hipStream_t s);
 void gs_colour_lines(int Nr, int Nth, int nwall_lines, std::vector<int>& order, std::vector<int>& cstart);
 int prof_resolve(Handle& h);
-void cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], double* out, hipStream_t st);
+// returns 1 when a non-negative c or q entry would fall below -s; truncate = 1: such entries keep
+// 0.1 of their value
+int cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], int truncate, double* out,
+                    hipStream_t st);
 // out = post^-1 * M^-1 (pre * in * (in_scale ? *in_scale : 1)); optionally stores the scaled
 // input to `store_in`; optionally accumulates sum(out^2) and sets sigma in gs (matvec use).
 void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,

@@ -166,10 +166,18 @@ __global__ void __launch_bounds__(RBLK) k_weighted(Model m, int mode, const doub
   if (last_block_totals(counter, &slast, partial, nout, tot) && threadIdx.x < nout) result[threadIdx.x] = tot[threadIdx.x];
 }
 
+__global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // out = z - S^2 W^T lambda: each entry moved along its conserved row k (gas species, adsorbed
-// species into the same row, energy of bed and wall) by s_i^2 W_ki lambda_k.
+// species into the same row, energy of bed and wall) by s_i^2 W_ki lambda_k.  *flag is set
+// when a non-negative c or q entry would fall below -s; with truncate such entries keep 0.1 of
+// their value.
 __global__ void k_cons_shift(Model m, const double* z, const double* __restrict__ s, double l0, double l1,
-                             double l2, double l3, double* out) {   // out may alias z (elementwise)
+                             double l2, double l3, double* out,   // out may alias z (elementwise)
+                             int truncate, unsigned* flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const vtsa_model_desc& d = m.d;
@@ -185,8 +193,16 @@ __global__ void k_cons_shift(Model m, const double* z, const double* __restrict_
     w = d.C_wall;
     lam = l3;
   }
+  const double zo = z[i];
   const double si = s[i];
-  out[i] = z[i] - (si * si) * w * lam;
+  double v = zo - (si * si) * w * lam;
+  // c and q: an excursion below zero by more than the tolerance scale s is flagged; truncated,
+  // such entries keep at least 0.1 of their value, as a Newton update (core/solver.py:99-108)
+  if (i < 5 * n && zo >= 0.0 && v < -si) {
+    if (truncate) v = 0.1 * zo;
+    atomicOr(flag, 1u);
+  }
+  out[i] = v;
 }
 
 // Integrator vector formulas (core/solver.py:419-476), exact upstream operation order.
@@ -526,8 +542,16 @@ int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mo
   return 0;
 }
 
-void cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_cons_shift, dim3(eblk(h.N)), dim3(256), 0, st, h.m, z, s, lam[0], lam[1], lam[2], lam[3], out);
+int cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], int truncate, double* out,
+                    hipStream_t st) {
+  unsigned* flag = h.red.counter + 12;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, flag, 0u, 1);
+  hipLaunchKernelGGL(k_cons_shift, dim3(eblk(h.N)), dim3(256), 0, st, h.m, z, s, lam[0], lam[1], lam[2], lam[3], out,
+                     truncate, flag);
+  unsigned* host = reinterpret_cast<unsigned*>(h.host_scalars + 63);
+  if (check(hipMemcpyAsync(host, flag, sizeof(unsigned), hipMemcpyDeviceToHost, st), "memcpy")) return -1;
+  if (check(hipStreamSynchronize(st), "sync")) return -1;
+  return *host != 0u ? 1 : 0;
 }
 
 // Newton building blocks (host drivers; each returns host scalars)

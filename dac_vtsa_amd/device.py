@@ -23,7 +23,9 @@ from . import _abi
 _checked = False
 _tls = threading.local()
 _cache = {}                      # (stream value, nbytes) -> [pointer values]
-_cache_lock = threading.Lock()
+# Reentrant: DeviceArray.__del__ takes it, and a garbage collection triggered inside a critical
+# section can run another DeviceArray's __del__ on the same thread.
+_cache_lock = threading.RLock()
 
 
 def _ensure_device():
@@ -121,8 +123,9 @@ class DeviceArray:
         p = getattr(self, "_p", None)
         if p is not None and p.value:
             try:
+                pv, fresh = p.value, []          # allocate before taking the lock
                 with _cache_lock:
-                    _cache.setdefault(self._key, []).append(p.value)
+                    _cache.setdefault(self._key, fresh).append(pv)
             except Exception:
                 pass
             self._p = None

@@ -12,7 +12,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from ..core.residual import TVSAModel
-from ..core.solver import TRBDF2
+from ..core.solver import TRBDF2, SolverError
 from ..properties import R_GAS
 from ..valves import humid_air_composition
 from .result import StepResult
@@ -113,9 +113,26 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
     wall0 = time.time()
     if cfg.solver.method != "trbdf2":
         raise RuntimeError(f"solver.method = {cfg.solver.method}: only the implicit trbdf2 path runs on the GPU")
-    integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **(integrator_kw or {}))
-    res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
-                          stop_direction=direction)
+    kw = dict(integrator_kw or {})
+    integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **kw)
+    try:
+        res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
+                              stop_direction=direction)
+    except SolverError as e:
+        if kw.get("conserve") != "project":
+            raise
+        # the conservation projection can leave states (slightly negative concentrations) the
+        # step controller cannot follow; the step is run again with the upstream polish
+        log(f"    {step['name']}: {e}; running the step again with the conservation polish")
+        outlet.clear()
+        counter[0] = 0
+        if record_outlet:
+            record(0.0, y0, np.zeros(model.n_acc))
+        kw["conserve"] = "polish"
+        integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **kw)
+        res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
+                              stop_direction=direction)
+        integ.stats["projection_fallback"] = 1
     solver_stats = {"method": "trbdf2", "steps": res.n_steps, "rejected": res.n_reject,
                     "newton_fail": res.n_newton_fail, "krylov": res.n_krylov, "jacobians": res.n_jac,
                     "lu": res.n_lu, "nfev": res.nfev, "groups": "all-columns", **integ.stats}

@@ -154,9 +154,12 @@ int vtsa_cons(vtsa_handle* h, const double* R, const double* s, double* wr, doub
  * out[k] = sum over the entries of conserved row k of (W_ki s_i)^2, k = CO2, H2O, N2, energy. */
 int vtsa_cons_norm2(vtsa_handle* h, const double* s, double* out, vtsa_stream st);
 /* out = z - S^2 W^T lambda (lambda: 4 host values), the minimum-norm shift in the s-scaled
- * variables that changes the conserved totals W z by -W S^2 W^T lambda.  out may alias z. */
-int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, double* out,
-                    vtsa_stream st);
+ * variables that changes the conserved totals W z by -W S^2 W^T lambda.  out may alias z.
+ * Returns 1 when a non-negative concentration or loading would fall below -s (one tolerance
+ * unit under zero); with truncate = 1 such entries keep 0.1 of their value instead, as a Newton
+ * update does (core/solver.py:99-108).  Synchronises the stream. */
+int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, int32_t truncate,
+                    double* out, vtsa_stream st);
 /* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall
  * (core/residual.py:426-443) */
 int vtsa_inventories(vtsa_handle* h, const double* y, double* out, vtsa_stream st);

@@ -136,3 +136,35 @@ def test_balance_rows_on_synthetic_step():
     assert eb["residual_J"] == 0.0
     sr.inv_end = dict(inv1, tot_CO2=2.5 + 1e-7)
     assert abs(balances.mass_balance_rows(sr, ["CO2"])[0]["relative_error"] - 1e-7 / (2.5 + 1e-7)) < 1e-14
+
+
+# ------------------------------------------------------------------ device buffer cache (host logic)
+def test_buffer_cache_finalises_inside_its_own_critical_section(monkeypatch):
+    """A garbage collection that starts inside the buffer cache's critical section finalises
+    other device arrays on the same thread, whose __del__ takes the same lock: it must be
+    reentrant (a plain lock deadlocks there).  No device call is made: allocation is faked."""
+    import ctypes as C
+    import gc
+    from dac_vtsa_amd import _abi
+    from dac_vtsa_amd import device as dv
+    monkeypatch.setattr(dv, "_ensure_device", lambda: None)
+    nxt = [0x10000]
+
+    def fake_call(name, *args):
+        assert name == "vtsa_device_alloc", name
+        nxt[0] += 0x1000
+        args[1]._obj.value = nxt[0]
+        return 0
+    monkeypatch.setattr(_abi, "call", fake_call)
+    monkeypatch.setattr(dv, "_cache", {})
+    a = dv.DeviceArray(12345)
+    ptr = a.ptr.value
+    cycle = [a]
+    cycle.append(cycle)          # unreachable only through the collector
+    del a, cycle
+    with dv._cache_lock:
+        gc.collect()
+    assert ptr in dv._cache[(None, 8 * 12345)]
+    b = dv.DeviceArray(12345)     # the cached block is reused
+    assert b.ptr.value == ptr
+    b._p = None                   # nothing to return to a cache the test is about to drop
