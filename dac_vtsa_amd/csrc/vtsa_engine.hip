@@ -34,11 +34,12 @@ static int upload(double** dst, const double* src, size_t count) {
 static void free_handle(Handle* h) {
   if (!h) return;
   double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2,
-                    h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jcw, h->Jw, h->Jwc, h->Fw, h->s0,
+                    h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
                     h->Vb, h->red.partial, h->red.result};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (h->F) (void)hipFree(h->F);
+  if (h->P) (void)hipFree(h->P);
   if (h->gs) (void)hipFree(h->gs);
   if (h->red.counter) (void)hipFree(h->red.counter);
   if (h->gs_host) (void)hipHostFree(h->gs_host);
@@ -122,11 +123,14 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   rc |= dalloc(&h.Jcw, nwall * 4);
   rc |= dalloc(&h.Jw, nwall * 5);
   rc |= dalloc(&h.Jwc, nwall * 6);
+  rc |= dalloc(&h.Tw, nwall * 3);
   rc |= check(hipMalloc(&h.F, sizeof(double) * (size_t)h.n * 180), "hipMalloc");
+  rc |= check(hipMalloc(&h.P, sizeof(double) * gs_pairs_count(h.ncol, h.desc.Nz)), "hipMalloc");
   rc |= dalloc(&h.Fw, nwall);
   rc |= dalloc(&h.s0, h.N);
   rc |= dalloc(&h.gsb, h.N);
   rc |= dalloc(&h.gsu, h.N);
+  rc |= dalloc(&h.Jo, (size_t)h.n * 144);
   {
     std::vector<int> order;
     gs_colour_lines(Nr, Nth, h.nw > 0 ? Nth : 0, order, h.gs_cstart);

@@ -51,9 +51,12 @@ struct Handle {
   double* accbuf = nullptr;  // [NACC]
   // Jacobian blocks
   double *Jc = nullptr, *Jcw = nullptr, *Jw = nullptr, *Jwc = nullptr;
+  double* Jo = nullptr;       // [144][n] off-line blocks (r-1, r+1, theta-1, theta+1), cell fastest
   // preconditioner factors (FP64 or FP32), scale at factorisation
   void* F = nullptr;          // [n][5][36] blocks (L2, L1, Dinv, U1, U2) in the chosen precision
+  void* P = nullptr;          // line Gauss-Seidel pair form of F, gs_pairs_count values
   double* Fw = nullptr;       // [nw] wall diagonal of M
+  double* Tw = nullptr;       // [nw][3] wall-line tridiagonal LU: lo, 1 / den, c (line Gauss-Seidel)
   double* s0 = nullptr;       // [N] scale of the factorisation
   int prec_kind = -1, prec_prec = 0, prec_ready = 0;
   double prec_dh = 0.0;
@@ -132,6 +135,7 @@ void vec_op_impl(Handle& h, int op, int npos, double a, double b, double c, doub
                  const double* y, const double* z, const double* w, double* out, hipStream_t st);
 size_t partial_size(int ncol, int nw);
 bool zline_fits(int Nz);
+size_t gs_pairs_count(int ncol, int Nz);   // values of Handle::P
 
 // Preconditioner apply: out = post^-1 * S0 M~^-1 S0^-1 (pre * scale * in); optional side outputs:
 //   store_in  <- scale * in (normalises a GMRES basis vector in place; may alias in)

@@ -200,6 +200,29 @@ __global__ void k_prec_wall(int nw, const double* __restrict__ Jw, double dh, do
   if (w < nw) Fw[w] = 1.0 - dh * Jw[w * 5 + 0];
 }
 
+// Line Gauss-Seidel wall lines: LU (Thomas) of each wall line's tridiagonal block of M~ (axial
+// conduction; entries scaled by s0), one thread per line, sequential in z.  Per cell
+// Tw = (lo_k, 1 / den_k, c_k) with den_k = diag_k - lo_k c_{k-1}, c_k = up_k / den_k.
+__global__ void k_gs_wall_factor(int Nth, int Nz, int n, const double* __restrict__ Jw,
+                                 const double* __restrict__ s0, double dh, double* __restrict__ Tw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= Nth) return;
+  const double* s0w = s0 + 6 * (size_t)n + (size_t)j * Nz;
+  const double* Jwl = Jw + (size_t)j * Nz * 5;
+  double* T3 = Tw + (size_t)j * Nz * 3;
+  double cprev = 0.0;
+  for (int k = 0; k < Nz; ++k) {
+    const double diag = 1.0 - dh * Jwl[k * 5 + 0];
+    const double lo = k > 0 ? (-dh * Jwl[k * 5 + 1]) * (s0w[k - 1] / s0w[k]) : 0.0;
+    const double up = k + 1 < Nz ? (-dh * Jwl[k * 5 + 2]) * (s0w[k + 1] / s0w[k]) : 0.0;
+    const double den = diag - lo * cprev;
+    cprev = up / den;
+    T3[k * 3 + 0] = lo;
+    T3[k * 3 + 1] = 1.0 / den;
+    T3[k * 3 + 2] = cprev;
+  }
+}
+
 // ------------------------------------------------------------------ preconditioner apply
 // out = post^-1 * S0 M~^-1 S0^-1 (pre * scale * in), side outputs as described at ApplyArgs.
 //
@@ -244,12 +267,59 @@ __device__ inline void stage_rows(const T* __restrict__ F, int X0, int kbeg, int
 static_assert(PW == 64, "the apply kernel assumes one-wavefront blocks");
 __device__ inline void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
-// Lane `src`'s value in every lane (v_readlane into scalar registers, no LDS round trip).
-__device__ inline double lane_value(double v, int src) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+// ---- block rows across lanes
+// A cell's 6-vector lives one component per lane: lane l holds component line_row(l), i.e. lanes
+// 0..5 of every 16-lane DPP row hold components 0..5 and lanes 6..15 repeat component 0, so all
+// four rows of a wavefront carry the same values.  A dot product of a block row with a 6-vector
+// then takes component c from lane c of the lane's own row with DPP `row_newbcast:c`, folded into
+// the FP64 FMA (v_fmac_f64_dpp), with no broadcast instructions of its own.  Each block starts
+// with `s_nop 1`: a DPP read of a VGPR needs two wait states after the VALU write of that VGPR.
+__device__ inline int line_row(int lane) { return (lane & 15) < 6 ? (lane & 15) : 0; }
+
+#define VTSA_FMAC_BC(acc, v, l, c) "v_fmac_f64_dpp " acc ", " v ", " l " row_newbcast:" #c " row_mask:0xf bank_mask:0xf\n\t"
+
+// b - L1 v1 - L2 v2 for block rows l1, l2 (this lane's row) and vectors v1, v2 (this lane's
+// component), as ((b - sum_{c<3} l2_c v2_c) + (-sum_{c>=3} l2_c v2_c))
+//   + ((-sum_{c<3} l1_c v1_c) + (-sum_{c>=3} l1_c v1_c)),
+// each partial sum accumulated with c in order.  The four chains are interleaved, so no FMA
+// waits for the one before it.
+__device__ inline double bsub12(double b, double v1, const double (&l1)[6], double v2, const double (&l2)[6]) {
+  double a2 = 0.0, c1 = 0.0, c2 = 0.0;
+  asm("s_nop 1\n\t"
+      VTSA_FMAC_BC("%0", "-%5", "%12", 0) VTSA_FMAC_BC("%1", "-%5", "%15", 3)
+      VTSA_FMAC_BC("%2", "-%4", "%6", 0)  VTSA_FMAC_BC("%3", "-%4", "%9", 3)
+      VTSA_FMAC_BC("%0", "-%5", "%13", 1) VTSA_FMAC_BC("%1", "-%5", "%16", 4)
+      VTSA_FMAC_BC("%2", "-%4", "%7", 1)  VTSA_FMAC_BC("%3", "-%4", "%10", 4)
+      VTSA_FMAC_BC("%0", "-%5", "%14", 2) VTSA_FMAC_BC("%1", "-%5", "%17", 5)
+      VTSA_FMAC_BC("%2", "-%4", "%8", 2)  VTSA_FMAC_BC("%3", "-%4", "%11", 5)
+      : "+v"(b), "+v"(a2), "+v"(c1), "+v"(c2)
+      : "v"(v1), "v"(v2), "v"(l1[0]), "v"(l1[1]), "v"(l1[2]), "v"(l1[3]), "v"(l1[4]), "v"(l1[5]),
+        "v"(l2[0]), "v"(l2[1]), "v"(l2[2]), "v"(l2[3]), "v"(l2[4]), "v"(l2[5]));
+  return (b + a2) + (c1 + c2);
+}
+
+// L v for block row l: ((l_0 v_0 + l_1 v_1) + (l_2 v_2 + l_3 v_3)) + (l_4 v_4 + l_5 v_5)
+__device__ inline double bdot6(double v, const double (&l)[6]) {
+  double p = 0.0, q = 0.0, w = 0.0;
+  asm("s_nop 1\n\t"
+      VTSA_FMAC_BC("%0", "%3", "%4", 0) VTSA_FMAC_BC("%1", "%3", "%6", 2) VTSA_FMAC_BC("%2", "%3", "%8", 4)
+      VTSA_FMAC_BC("%0", "%3", "%5", 1) VTSA_FMAC_BC("%1", "%3", "%7", 3) VTSA_FMAC_BC("%2", "%3", "%9", 5)
+      : "+v"(p), "+v"(q), "+v"(w)
+      : "v"(v), "v"(l[0]), "v"(l[1]), "v"(l[2]), "v"(l[3]), "v"(l[4]), "v"(l[5]));
+  return (p + q) + w;
+}
+#undef VTSA_FMAC_BC
+
+// One cell of the forward block substitution: z_k = b_k - L1 z_{k-1} - L2 z_{k-2} (component
+// line_row(lane) of each vector in this lane).
+__device__ inline double fwd_cell(double b, const double (&l1)[6], const double (&l2)[6], double zm1, double zm2) {
+  return bsub12(b, zm1, l1, zm2, l2);
+}
+
+// One cell of the backward substitution: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2}).
+__device__ inline double bwd_cell(double z, const double (&d)[6], const double (&u1)[6], const double (&u2)[6],
+                                  double xp1, double xp2) {
+  return bdot6(bsub12(z, xp1, u1, xp2, u2), d);
 }
 
 // Wait for this wavefront's staged copies, then barrier (one-wavefront blocks).
@@ -273,12 +343,10 @@ template <typename T>
 __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, double* xs, T* fb0, T* fb1) {
   const int lane = threadIdx.x;
   const bool active = lane < 6;
-  const int r = active ? lane : 0;
+  const int r = line_row(lane);
   const int nseg = (Nz + ZSEG - 1) / ZSEG;
-  // the last two cells' vectors, wave-uniform (every lane holds all six components)
-  double zm1[6], zm2[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) { zm1[c] = 0.0; zm2[c] = 0.0; }
+  // component r of the last two cells' vectors
+  double zm1 = 0.0, zm2 = 0.0;
   // forward: z_k = b_k - L1 z_{k-1} - L2 z_{k-2}
   for (int sg = 0; sg < nseg; ++sg) {
     const int k0 = sg * ZSEG, kc = min(ZSEG, Nz - k0);
@@ -287,32 +355,23 @@ __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, dou
     if (sg + 1 < nseg) stage_rows<T>(F, X0, k0 + ZSEG, min(ZSEG, Nz - k0 - ZSEG), 0, 72, (sg & 1) ? fb0 : fb1);
     // row r of L2 at [0, 36), of L1 at [36, 72) of each cell; the next cell's rows and right-hand
     // side are read while the current cell is computed
-    T l1[6], l2[6];
+    double l1[6], l2[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) { l1[c] = cur[r * 6 + 36 + c]; l2[c] = cur[r * 6 + c]; }
+    for (int c = 0; c < 6; ++c) { l1[c] = (double)cur[r * 6 + 36 + c]; l2[c] = (double)cur[r * 6 + c]; }
     double bk = xs[k0 * 6 + r];
     for (int kk = 0; kk < kc; ++kk) {
       const int k = k0 + kk;
       const int kn = kk + 1 < kc ? kk + 1 : kk;
-      T n1[6], n2[6];
+      double n1[6], n2[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) { n1[c] = cur[kn * 72 + r * 6 + 36 + c]; n2[c] = cur[kn * 72 + r * 6 + c]; }
+      for (int c = 0; c < 6; ++c) { n1[c] = (double)cur[kn * 72 + r * 6 + 36 + c]; n2[c] = (double)cur[kn * 72 + r * 6 + c]; }
       const double bn = xs[(k0 + kn) * 6 + r];
-      double s1 = 0.0, s2 = 0.0;   // the factor holds zero blocks where k < 2
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        s1 = fma((double)l1[c], zm1[c], s1);
-        s2 = fma((double)l2[c], zm2[c], s2);
-      }
-      const double zk = (bk - s1) - s2;
+      const double zk = fwd_cell(bk, l1, l2, zm1, zm2);   // the factor holds zero blocks where k < 2
       if (active) xs[k * 6 + r] = zk;
+      zm2 = zm1;
+      zm1 = zk;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        zm2[c] = zm1[c];
-        zm1[c] = lane_value(zk, c);
-        l1[c] = n1[c];
-        l2[c] = n2[c];
-      }
+      for (int c = 0; c < 6; ++c) { l1[c] = n1[c]; l2[c] = n2[c]; }
       bk = bn;
     }
   }
@@ -322,9 +381,7 @@ __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, dou
     stage_wait();   // the forward sweep is done with both buffers
     stage_rows<T>(F, X0, k0, Nz - k0, 72, 108, fb0);
   }
-  double xp1[6], xp2[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) { xp1[c] = 0.0; xp2[c] = 0.0; }
+  double xp1 = 0.0, xp2 = 0.0;
   for (int sg = nseg - 1, b = 0; sg >= 0; --sg, b ^= 1) {
     const int k0 = sg * ZSEG, kc = min(ZSEG, Nz - k0);
     T* cur = b ? fb1 : fb0;
@@ -332,73 +389,128 @@ __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, dou
     if (sg > 0) stage_rows<T>(F, X0, k0 - ZSEG, ZSEG, 72, 108, b ? fb0 : fb1);
     // row r of Dinv at [0, 36), U1 at [36, 72), U2 at [72, 108) of each cell; the next (lower)
     // cell's rows and z are read while the current cell is computed
-    T dr[6], u1[6], u2[6];
+    double dr[6], u1[6], u2[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const T* R = cur + (kc - 1) * 108 + r * 6;
-      dr[c] = R[c]; u1[c] = R[36 + c]; u2[c] = R[72 + c];
+      dr[c] = (double)R[c]; u1[c] = (double)R[36 + c]; u2[c] = (double)R[72 + c];
     }
     double zk = xs[(k0 + kc - 1) * 6 + r];
     for (int kk = kc - 1; kk >= 0; --kk) {
       const int k = k0 + kk;
       const int kn = kk > 0 ? kk - 1 : kk;
       const T* Rn = cur + kn * 108 + r * 6;
-      T nd[6], n1[6], n2[6];
+      double nd[6], n1[6], n2[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) { nd[c] = Rn[c]; n1[c] = Rn[36 + c]; n2[c] = Rn[72 + c]; }
+      for (int c = 0; c < 6; ++c) { nd[c] = (double)Rn[c]; n1[c] = (double)Rn[36 + c]; n2[c] = (double)Rn[72 + c]; }
       const double zn = xs[(k0 + kn) * 6 + r];
-      double s1 = 0.0, s2 = 0.0;   // zero blocks where k + 2 > Nz - 1
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        s1 = fma((double)u1[c], xp1[c], s1);
-        s2 = fma((double)u2[c], xp2[c], s2);
-      }
-      const double tk = (zk - s1) - s2;
-      double tt[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) tt[c] = lane_value(tk, c);
-      double xk = 0.0;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) xk = fma((double)dr[c], tt[c], xk);
+      const double xk = bwd_cell(zk, dr, u1, u2, xp1, xp2);   // zero blocks where k + 2 > Nz - 1
       if (active) xs[k * 6 + r] = xk;
+      xp2 = xp1;
+      xp1 = xk;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        xp2[c] = xp1[c];
-        xp1[c] = lane_value(xk, c);
-        dr[c] = nd[c];
-        u1[c] = n1[c];
-        u2[c] = n2[c];
-      }
+      for (int c = 0; c < 6; ++c) { dr[c] = nd[c]; u1[c] = n1[c]; u2[c] = n2[c]; }
       zk = zn;
     }
   }
   lds_barrier();
 }
 
-// ------------------------------------------------------------------ whole-line staging
+// ------------------------------------------------------------------ whole-line pair sweeps
 // When the device allows a block that much LDS, a line Gauss-Seidel half-sweep copies its line's
-// whole factor (180 values per cell, contiguous in HBM) into LDS at kernel start, so the copy
-// runs under the right-hand-side gather and the sweeps never wait for memory.  The copy is
-// spread over the block's GSW threads (four wavefronts).
-constexpr int GSW = 256;   // threads of a whole-line half-sweep block
+// factor into LDS at kernel start (the copy runs under the right-hand-side gather, spread over
+// the block's GSW threads) and solves the line two cells at a time.  Over pairs of cells
+// (2K, 2K+1) the block-pentadiagonal L U of a line is block bidiagonal:
+//   forward   z_K = b'_K - G_K (z_{2K-2}, z_{2K-1}),  b'_K = (b_{2K}, b_{2K+1} - L1_{2K+1} b_{2K})
+//   backward  x_K = z'_K - H_K (x_{2K+2}, x_{2K+3}),  z'_K = (h_{2K} - ^U1_{2K} h_{2K+1}, h_{2K+1}),
+//             h_k = Dinv_k z_k,  ^U = Dinv U
+//   G_K = [ L2_{2K}               L1_{2K}                     ]
+//         [ -L1_{2K+1} L2_{2K}    L2_{2K+1} - L1_{2K+1} L1_{2K} ]
+//   H_K = [ ^U2_{2K} - ^U1_{2K} ^U1_{2K+1}    -^U1_{2K} ^U2_{2K+1} ]
+//         [ ^U1_{2K+1}                        ^U2_{2K+1}           ]
+// so each step is one 12-wide dot product per row (rows in lanes 0..11), half as many dependent
+// steps as cell by cell, and b', h and z' are data-parallel passes of the whole block.  A line
+// with odd Nz ends in a pair whose second cell is a zero phantom.  Per pair, PP values:
+// G (12 x 12) | L1_{2K+1} | H (12 x 12) | Dinv_{2K} | Dinv_{2K+1} | ^U1_{2K}, rows of 12 or 6.
+constexpr int GSW = 512;   // threads of a whole-line half-sweep block
+constexpr int PP = 432;
+constexpr int PO_G = 0, PO_C = 144, PO_H = 180, PO_D = 324, PO_U = 396;
 
+__host__ __device__ constexpr int npairs(int Nz) { return (Nz + 1) / 2; }
+// xs and zs, [2 npairs + 1][6] each (the last cell a spare slot)
+__host__ __device__ constexpr size_t pair_vec_doubles(int Nz) { return (size_t)(2 * npairs(Nz) + 1) * 6; }
 template <typename T>
-__host__ __device__ constexpr size_t line_buf_bytes(int Nz) {   // whole copies of GSW x 16 bytes
-  return ((size_t)Nz * 180 * sizeof(T) + GSW * 16 - 1) / (GSW * 16) * (GSW * 16);
+__host__ __device__ constexpr size_t pair_buf_bytes(int Nz) {   // whole copies of GSW x 16 bytes
+  return ((size_t)npairs(Nz) * PP * sizeof(T) + GSW * 16 - 1) / (GSW * 16) * (GSW * 16);
 }
 template <typename T>
 __host__ __device__ constexpr size_t gs_full_lds_bytes(int Nz) {
-  return sizeof(double) * ((size_t)Nz * 6 + 8) + line_buf_bytes<T>(Nz);
+  return sizeof(double) * 2 * pair_vec_doubles(Nz) + pair_buf_bytes<T>(Nz);
 }
 
-// Copy the factor of cells [X0, X0 + Nz) into dst: 16-byte chunks, GSW per pass (64 per
-// wavefront instruction), straight into LDS.  Lanes past the end re-read the last chunk into the
-// buffer's padding.  Called by all GSW threads of the block.
+size_t gs_pairs_count(int ncol, int Nz) { return (size_t)ncol * npairs(Nz) * PP; }
+
+// Pair form of the factors, one thread per (line, pair, row r of 6).  Products accumulate over m
+// in order in FP64 and are stored in T.
 template <typename T>
-__device__ inline void stage_line(const T* __restrict__ F, int X0, int Nz, T* dst) {
-  const int total = Nz * 180 * (int)sizeof(T) / 16;
+__global__ void __launch_bounds__(256) k_gs_pairs(int Nz, int ncol, const T* __restrict__ F, T* __restrict__ P) {
+  const int NP = npairs(Nz);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncol * NP * 6) return;
+  const int r = t % 6, pr = t / 6;
+  const int L = pr / NP, K = pr - L * NP;
+  const int k0 = 2 * K, k1 = 2 * K + 1;
+  // entry (i, c) of block b (0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2) of cell k; zero for the phantom
+  auto fb = [&](int k, int b, int i, int c) -> double {
+    return k < Nz ? (double)F[(size_t)(L * Nz + k) * 180 + b * 36 + i * 6 + c] : 0.0;
+  };
+  double u1[36], u2[36];   // ^U1_{2K+1}, ^U2_{2K+1}
+  for (int i = 0; i < 6; ++i)
+    for (int c = 0; c < 6; ++c) {
+      double a1 = 0.0, a2 = 0.0;
+      for (int m = 0; m < 6; ++m) { a1 += fb(k1, 2, i, m) * fb(k1, 3, m, c); a2 += fb(k1, 2, i, m) * fb(k1, 4, m, c); }
+      u1[i * 6 + c] = a1;
+      u2[i * 6 + c] = a2;
+    }
+  double w1[6], w2[6];   // row r of ^U1_{2K}, ^U2_{2K}
+  for (int c = 0; c < 6; ++c) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int m = 0; m < 6; ++m) { a1 += fb(k0, 2, r, m) * fb(k0, 3, m, c); a2 += fb(k0, 2, r, m) * fb(k0, 4, m, c); }
+    w1[c] = a1;
+    w2[c] = a2;
+  }
+  T* out = P + (size_t)pr * PP;
+  for (int c = 0; c < 6; ++c) {
+    double a = 0.0, b2 = 0.0, p = 0.0, q = 0.0;
+    for (int m = 0; m < 6; ++m) {
+      a += fb(k1, 1, r, m) * fb(k0, 0, m, c);    // (L1_{2K+1} L2_{2K})[r][c]
+      b2 += fb(k1, 1, r, m) * fb(k0, 1, m, c);   // (L1_{2K+1} L1_{2K})[r][c]
+      p += w1[m] * u1[m * 6 + c];                // (^U1_{2K} ^U1_{2K+1})[r][c]
+      q += w1[m] * u2[m * 6 + c];                // (^U1_{2K} ^U2_{2K+1})[r][c]
+    }
+    out[PO_G + r * 12 + c] = (T)fb(k0, 0, r, c);
+    out[PO_G + r * 12 + 6 + c] = (T)fb(k0, 1, r, c);
+    out[PO_G + (6 + r) * 12 + c] = (T)(-a);
+    out[PO_G + (6 + r) * 12 + 6 + c] = (T)(fb(k1, 0, r, c) - b2);
+    out[PO_C + r * 6 + c] = (T)fb(k1, 1, r, c);
+    out[PO_H + r * 12 + c] = (T)(w2[c] - p);
+    out[PO_H + r * 12 + 6 + c] = (T)(-q);
+    out[PO_H + (6 + r) * 12 + c] = (T)u1[r * 6 + c];
+    out[PO_H + (6 + r) * 12 + 6 + c] = (T)u2[r * 6 + c];
+    out[PO_D + r * 6 + c] = (T)fb(k0, 2, r, c);
+    out[PO_D + 36 + r * 6 + c] = (T)fb(k1, 2, r, c);
+    out[PO_U + r * 6 + c] = (T)w1[c];
+  }
+}
+
+// Copy the NP pairs of line L into dst: 16-byte chunks, GSW per pass (64 per wavefront
+// instruction), straight into LDS.  Lanes past the end re-read the last chunk into the buffer's
+// padding.  Called by all GSW threads of the block.
+template <typename T>
+__device__ inline void stage_pairs(const T* __restrict__ P, int L, int NP, T* dst) {
+  const int total = NP * PP * (int)sizeof(T) / 16;
   const int tid = threadIdx.x, w = tid >> 6;
-  const char* base = reinterpret_cast<const char*>(F + (size_t)X0 * 180);
+  const char* base = reinterpret_cast<const char*>(P + (size_t)L * NP * PP);
   char* d = reinterpret_cast<char*>(dst);
   for (int j = 0; j * GSW < total; ++j) {
     const int c = j * GSW + tid;
@@ -409,85 +521,62 @@ __device__ inline void stage_line(const T* __restrict__ F, int X0, int Nz, T* ds
   }
 }
 
-// zline_sweeps with the line's factor resident in LDS (Fl[k][180]: L2 | L1 | Dinv | U1 | U2).
-// Run by the block's first wavefront only; the caller synchronises afterwards.  Two cells per
-// loop pass with alternating register roles, so the recurrence carries no copies; every lane
-// stores its row (lanes past 5 repeat row 0's value), so no exec-mask switching.
-template <typename T>
-__device__ inline void zline_sweeps_full(const T* Fl, int Nz, double* xs) {
-  const int lane = threadIdx.x;
-  const int r = lane < 6 ? lane : 0;
-  // forward: z_k = b_k - L1 z_{k-1} - L2 z_{k-2}; za, zb hold the last two z (roles alternate)
-  double za[6], zb[6];
+// b - sum_{c<12} g_c v_c, v_c taken from lane c of the lane's 16-lane row (DPP row_newbcast), as
+// ((b - g0 v0 - g1 v1 - g2 v2) + (-g3 v3 - g4 v4 - g5 v5)) + ((-g6 v6 - ..) + (-g9 v9 - ..)),
+// four interleaved chains.
+#define VTSA_FMAC_BC(acc, v, l, c) "v_fmac_f64_dpp " acc ", " v ", " l " row_newbcast:" #c " row_mask:0xf bank_mask:0xf\n\t"
+__device__ inline double bsub_pair(double b, double v, const double (&g)[12]) {
+  double a2 = 0.0, c1 = 0.0, c2 = 0.0;
+  asm("s_nop 1\n\t"
+      VTSA_FMAC_BC("%0", "-%4", "%5", 0) VTSA_FMAC_BC("%1", "-%4", "%8", 3)
+      VTSA_FMAC_BC("%2", "-%4", "%11", 6) VTSA_FMAC_BC("%3", "-%4", "%14", 9)
+      VTSA_FMAC_BC("%0", "-%4", "%6", 1) VTSA_FMAC_BC("%1", "-%4", "%9", 4)
+      VTSA_FMAC_BC("%2", "-%4", "%12", 7) VTSA_FMAC_BC("%3", "-%4", "%15", 10)
+      VTSA_FMAC_BC("%0", "-%4", "%7", 2) VTSA_FMAC_BC("%1", "-%4", "%10", 5)
+      VTSA_FMAC_BC("%2", "-%4", "%13", 8) VTSA_FMAC_BC("%3", "-%4", "%16", 11)
+      : "+v"(b), "+v"(a2), "+v"(c1), "+v"(c2)
+      : "v"(v), "v"(g[0]), "v"(g[1]), "v"(g[2]), "v"(g[3]), "v"(g[4]), "v"(g[5]), "v"(g[6]), "v"(g[7]),
+        "v"(g[8]), "v"(g[9]), "v"(g[10]), "v"(g[11]));
+  return (b + a2) + (c1 + c2);
+}
+#undef VTSA_FMAC_BC
+
+// One direction of the pair recurrence, s_K = rhs[K] - M_K s_{K -+ 1}, over the NP pairs of a line
+// resident in LDS (M at offset `off` of each pair: G forward, H backward).  Run by lanes 0..11 of
+// the block's first wavefront, lane rho holding row rho of a pair; the caller synchronises.  A
+// pair's rows and right-hand side are read one pair ahead.  LDS order per pair: the next pair's
+// rows, then this pair's store; the look-ahead index is clamped rather than tested, the
+// scheduler may not move the reads (sched_barrier), and every path into a pair has one store
+// between its rows and the next pair's, so a pair waits for its own rows only.
+template <typename T, bool FORWARD>
+__device__ inline void pair_recurrence(const T* Pl, int NP, int off, const double* rhs, double* out) {
+  const int rho = threadIdx.x;
+  struct Row { double g[12]; double b; };
+  auto ld = [&](int K, Row& w) {
+    const T* R = Pl + (size_t)K * PP + off + rho * 12;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) { za[c] = 0.0; zb[c] = 0.0; }
-  // a cell's rows and right-hand side are read one cell ahead, before the current cell's store
-  // (the compiler cannot move LDS reads across an LDS store it cannot disambiguate)
-  struct FRow { T l1[6], l2[6]; double b; };
-  auto ldf = [&](int k, FRow& w) {
-    const T* R = Fl + (size_t)k * 180 + r * 6;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) { w.l2[c] = R[c]; w.l1[c] = R[36 + c]; }
-    w.b = xs[k * 6 + r];
+    for (int c = 0; c < 12; ++c) w.g[c] = (double)R[c];
+    w.b = rhs[K * 12 + rho];
   };
-  auto fwd = [&](int k, const FRow& w, const double (&zm1)[6], double (&zm2_out)[6]) {
-    double s1 = 0.0, s2 = 0.0;   // the factor holds zero blocks where k < 2
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      s1 = fma((double)w.l1[c], zm1[c], s1);
-      s2 = fma((double)w.l2[c], zm2_out[c], s2);
-    }
-    const double zk = (w.b - s1) - s2;
-    xs[k * 6 + r] = zk;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) zm2_out[c] = lane_value(zk, c);
-  };
-  FRow fa, fb;
-  ldf(0, fa);
-  int k = 0;
-  for (; k + 1 < Nz; k += 2) {
-    ldf(k + 1, fb);
-    fwd(k, fa, za, zb);        // zb <- z_k       (za = z_{k-1})
-    if (k + 2 < Nz) ldf(k + 2, fa);
-    fwd(k + 1, fb, zb, za);    // za <- z_{k+1}   (zb = z_k)
+  const int first = FORWARD ? 0 : NP - 1, step = FORWARD ? 1 : -1;
+  double s = 0.0;   // lane rho: component rho of the previous pair's solution
+  Row wa, wb;
+  ld(first, wa);
+  out[NP * 12 + (rho < 6 ? rho : rho - 6)] = 0.0;   // spare slot
+  int K = first, left = NP;
+  for (; left >= 2; K += 2 * step, left -= 2) {
+    ld(K + step, wb);
+    __builtin_amdgcn_sched_barrier(0);
+    s = bsub_pair(wa.b, s, wa.g);
+    out[K * 12 + rho] = s;
+    __builtin_amdgcn_sched_barrier(0);
+    ld(left > 2 ? K + 2 * step : K + step, wa);
+    __builtin_amdgcn_sched_barrier(0);
+    s = bsub_pair(wb.b, s, wb.g);
+    out[(K + step) * 12 + rho] = s;
+    __builtin_amdgcn_sched_barrier(0);
   }
-  if (k < Nz) fwd(k, fa, za, zb);
-  // backward: x_k = Dinv (z_k - U1 x_{k+1} - U2 x_{k+2})
-  double xa[6], xb[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) { xa[c] = 0.0; xb[c] = 0.0; }
-  struct BRow { T d[6], u1[6], u2[6]; double z; };
-  auto ldb = [&](int k, BRow& w) {
-    const T* R = Fl + (size_t)k * 180 + 72 + r * 6;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) { w.d[c] = R[c]; w.u1[c] = R[36 + c]; w.u2[c] = R[72 + c]; }
-    w.z = xs[k * 6 + r];
-  };
-  auto bwd = [&](int k, const BRow& w, const double (&xp1)[6], double (&xp2_out)[6]) {
-    double s1 = 0.0, s2 = 0.0;   // zero blocks where k + 2 > Nz - 1
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      s1 = fma((double)w.u1[c], xp1[c], s1);
-      s2 = fma((double)w.u2[c], xp2_out[c], s2);
-    }
-    const double tk = (w.z - s1) - s2;
-    double xk = 0.0;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) xk = fma((double)w.d[c], lane_value(tk, c), xk);
-    xs[k * 6 + r] = xk;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) xp2_out[c] = lane_value(xk, c);
-  };
-  BRow ba, bb;
-  ldb(Nz - 1, ba);
-  k = Nz - 1;
-  for (; k - 1 >= 0; k -= 2) {
-    ldb(k - 1, bb);
-    bwd(k, ba, xa, xb);        // xb <- x_k       (xa = x_{k+1})
-    if (k - 2 >= 0) ldb(k - 2, ba);
-    bwd(k - 1, bb, xb, xa);    // xa <- x_{k-1}   (xb = x_k)
-  }
-  if (k >= 0) bwd(k, ba, xa, xb);
+  if (left == 1) out[K * 12 + rho] = bsub_pair(wa.b, s, wa.g);
 }
 
 // ------------------------------------------------------------------ z-line / block-Jacobi apply
@@ -642,9 +731,23 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
 // unscaled, U = S0 x~, so the couplings to other lines need only J and U:
 //     (M~ x~)_R = x~_R - (dh / s0_R) sum_C J_RC U_C      (R, C in different lines)
 struct GsArgs {
-  const double* Jc; const double* Jcw; const double* Jw; const double* Jwc; const double* s0;
-  double* bs; double* U; const int* order; double dh; int first;
+  const double* Jo; const double* Jcw; const double* Jw; const double* Jwc; const double* s0;
+  double* bs; double* U; const int* order; double dh; int first; const double* Tw;
 };
+
+// Inclusive scan of the first-order linear recurrence y_l = a_l + b_l y_{l-1} over the 64 lanes of
+// a wavefront (Hillis-Steele: log2(64) steps of (a, b) <- (a + b a', b b') with the pair `off`
+// lanes below).  On return a holds y_l for y_{-1} = 0.  Called by all 64 lanes.
+__device__ inline void scan_linear(double& a, double& b, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double ap = __shfl_up(a, off, 64), bp = __shfl_up(b, off, 64);
+    if (lane >= off) {
+      a = fma(b, ap, a);
+      b = b * bp;
+    }
+  }
+}
 
 // b~ = (pre * scale * in) / s0 and U = 0 (store_in as in the z-line apply; may alias in).
 __global__ void k_gs_gather(int N, ApplyArgs a, const double* __restrict__ s0, double* __restrict__ bs,
@@ -660,14 +763,36 @@ __global__ void k_gs_gather(int N, ApplyArgs a, const double* __restrict__ s0, d
   U[i] = 0.0;
 }
 
-// MODE 0: one wavefront; the factor is staged in segments while the sweeps run (zline_sweeps).
-// MODE 1: GSW threads; the line's whole factor is staged into LDS at kernel start (stage_line),
-//         spread over four wavefronts so that no wavefront stalls on its count of outstanding
-//         copies, the right-hand side is gathered by all of them, and the first wavefront sweeps.
-//         Same arithmetic in both modes.
+// The off-line blocks of every cell (slots r-1, r+1, theta-1, theta+1: 144 values, contiguous in
+// Jc) transposed to Jo[(q * 36 + r * 6 + c) * n + X], so that a half-sweep's right-hand side reads
+// them with consecutive cells in consecutive lanes.  OT cells per block through LDS; once per
+// preconditioner set-up.
+constexpr int OT = 32;
+__global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restrict__ Jc, double* __restrict__ Jo) {
+  __shared__ double t[144][OT + 1];
+  const int X0 = blockIdx.x * OT;
+  const int nc = min(OT, n - X0);
+  for (int e = threadIdx.x; e < nc * 144; e += blockDim.x) {
+    const int x = e / 144, v = e - x * 144;
+    t[v][x] = Jc[((size_t)(X0 + x) * NSLOT + S_RM1) * 36 + v];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 144 * OT; e += blockDim.x) {
+    const int v = e / OT, x = e - v * OT;
+    if (x < nc) Jo[(size_t)v * n + X0 + x] = t[v][x];
+  }
+}
+
+// MODE 0: one wavefront, Fac the cell factors (F); they are staged in segments while the sweeps
+//         run cell by cell (zline_sweeps).
+// MODE 1: GSW threads, Fac the pair factors (P); the line's pairs are staged into LDS at kernel
+//         start (stage_pairs), spread over eight wavefronts so that no wavefront stalls on its
+//         count of outstanding copies, the right-hand side is gathered by all of them, and the
+//         first wavefront runs the pair recurrences between data-parallel passes.
+// Both solve the same line systems; the roundings differ.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
-    k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ F, GsArgs g, int cbeg,
+    k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ Fac, GsArgs g, int cbeg,
               const GmresScalars* __restrict__ gs, int respect_done) {
   extern __shared__ __align__(16) double lds[];
   if (respect_done && gs->done) return;
@@ -680,53 +805,91 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     // cell z-line (i, j)
     const int i = L / Nth, j = L - (L / Nth) * Nth;
     const int X0 = L * Nz;
-    T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);
+    const int NP = npairs(Nz);
+    double* zs = lds + pair_vec_doubles(Nz);                                  // MODE 1
+    T* pb = reinterpret_cast<T*>(lds + 2 * pair_vec_doubles(Nz));             // MODE 1
+    T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);                  // MODE 0
     T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
-    if (MODE >= 1) stage_line<T>(F, X0, Nz, fb0);
-    else zline_stage_first<T>(F, X0, Nz, fb0);
+    if (MODE >= 1) stage_pairs<T>(Fac, L, NP, pb);
+    else zline_stage_first<T>(Fac, X0, Nz, fb0);
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
     const int nb[4] = {i > 0 ? ((i - 1) * Nth + j) * Nz : -1, i + 1 < Nr ? ((i + 1) * Nth + j) * Nz : -1,
                        Nth > 1 ? (i * Nth + jm) * Nz : -1, Nth > 1 ? (i * Nth + jp) * Nz : -1};
     const bool wall = nw > 0 && i == Nr - 1;
-    // right-hand side of element e = (row r, cell k): b~ + (dh / s0) (off-line J) U
+    // right-hand side of element e = (row r, cell k): b~ + (dh / s0) (off-line J) U.  Branch-free,
+    // so that all of an element's loads are in flight together: a missing neighbour reads its own
+    // line's cell and contributes an exact zero.  Accumulation order q = 0..3, c = 0..5, wall.
     auto rhs = [&](int e) -> double {
       const int r = e / Nz, k = e - r * Nz;
       const int X = X0 + k;
       double acc = 0.0;
       if (!g.first) {
+        double jv[4][6], uv[4][6];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (nb[q] < 0) continue;
-          const double* Jr = g.Jc + ((size_t)X * NSLOT + S_RM1 + q) * 36 + r * 6;
-          const int Y = nb[q] + k;
+          const double* Jr = g.Jo + (size_t)(q * 36 + r * 6) * n + X;   // coalesced over k
+          const int Y = (nb[q] >= 0 ? nb[q] : X0) + k;
 #pragma unroll
-          for (int c = 0; c < 6; ++c) acc += Jr[c] * g.U[c * n + Y];
+          for (int c = 0; c < 6; ++c) { jv[q][c] = Jr[(size_t)c * n]; uv[q][c] = g.U[c * n + Y]; }
         }
-        if (wall && (r <= 2 || r == 5)) {
+        double jw = 0.0, uw = 0.0;
+        if (nw > 0) {
           const int W = j * Nz + k;
-          acc += g.Jcw[W * 4 + (r <= 2 ? r : 3)] * g.U[6 * n + W];
+          jw = g.Jcw[W * 4 + (r <= 2 ? r : 3)];
+          uw = g.U[6 * n + W];
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const bool on = nb[q] >= 0;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc += (on ? jv[q][c] : 0.0) * uv[q][c];
+        }
+        acc += (wall && (r <= 2 || r == 5) ? jw : 0.0) * uw;
       }
       const int idx = r * n + X;
       return g.bs[idx] + (dh / g.s0[idx]) * acc;
     };
-    // two elements per lane per pass, so both elements' loads are in flight together
-    for (int e = lane; e < 6 * Nz; e += 2 * NT) {
-      const int e1 = e + NT;
-      const double v0 = rhs(e);
-      if (e1 < 6 * Nz) {
-        const double v1 = rhs(e1);
-        xs[(e1 % Nz) * 6 + e1 / Nz] = v1;
-      }
-      xs[(e % Nz) * 6 + e / Nz] = v0;
-    }
+    for (int e = lane; e < 6 * Nz; e += NT) xs[(e % Nz) * 6 + e / Nz] = rhs(e);
     if (MODE == 1) {
+      if (lane < 6 * (2 * NP - Nz)) xs[Nz * 6 + lane] = 0.0;   // phantom cell of an odd line
       stage_wait();
-      if (lane < PW) zline_sweeps_full<T>(fb0, Nz, xs);
+      // b'_{2K+1} = b_{2K+1} - L1_{2K+1} b_{2K}
+      for (int e = lane; e < NP * 6; e += NT) {
+        const int K = e / 6, r = e - K * 6;
+        const T* C = pb + (size_t)K * PP + PO_C + r * 6;
+        double v = xs[K * 12 + 6 + r];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v = fma(-(double)C[c], xs[K * 12 + c], v);
+        xs[K * 12 + 6 + r] = v;
+      }
+      lds_barrier();
+      if (lane < 12) pair_recurrence<T, true>(pb, NP, PO_G, xs, xs);
+      lds_barrier();
+      // h_k = Dinv_k z_k
+      for (int e = lane; e < NP * 12; e += NT) {
+        const int K = e / 12, rr = e - K * 12, hh = rr / 6;
+        const T* D = pb + (size_t)K * PP + PO_D + rr * 6;
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v = fma((double)D[c], xs[K * 12 + hh * 6 + c], v);
+        zs[e] = v;
+      }
+      lds_barrier();
+      // z'_{2K} = h_{2K} - ^U1_{2K} h_{2K+1}
+      for (int e = lane; e < NP * 6; e += NT) {
+        const int K = e / 6, r = e - K * 6;
+        const T* U = pb + (size_t)K * PP + PO_U + r * 6;
+        double v = zs[K * 12 + r];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v = fma(-(double)U[c], zs[K * 12 + 6 + c], v);
+        zs[K * 12 + r] = v;
+      }
+      lds_barrier();
+      if (lane < 12) pair_recurrence<T, false>(pb, NP, PO_H, zs, xs);
       lds_barrier();
     } else {
-      zline_sweeps<T>(F, X0, Nz, xs, fb0, fb1);
+      zline_sweeps<T>(Fac, X0, Nz, xs, fb0, fb1);
     }
     for (int e = lane; e < 6 * Nz; e += NT) {
       const int r = e / Nz, k = e - r * Nz;
@@ -735,11 +898,12 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     }
   } else {
     // wall line j: tridiagonal along z (axial conduction), couplings to the outer cells and to
-    // the neighbouring wall lines moved to the right-hand side
+    // the neighbouring wall lines moved to the right-hand side.  Factored at set-up
+    // (k_gs_wall_factor); the forward and backward substitutions are first-order linear
+    // recurrences, solved 64 cells at a time by a scan over the first wavefront.
     const int j = L - ncol;
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     double* rhs = xs;          // [Nz]
-    double* cp = xs + Nz;      // [Nz] Thomas coefficients
     for (int k = lane; k < Nz; k += NT) {
       const int W = j * Nz + k;
       const int X = ((Nr - 1) * Nth + j) * Nz + k;
@@ -752,24 +916,42 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       rhs[k] = g.bs[6 * n + W] + (dh / g.s0[6 * n + W]) * acc;
     }
     lds_barrier();
-    if (lane == 0) {
-      const double* s0w = g.s0 + 6 * n + j * Nz;
-      const double* Jwl = g.Jw + (size_t)j * Nz * 5;
-      double cprev = 0.0, dprev = 0.0;
-      for (int k = 0; k < Nz; ++k) {
-        const double diag = 1.0 - dh * Jwl[k * 5 + 0];
-        const double lo = k > 0 ? (-dh * Jwl[k * 5 + 1]) * (s0w[k - 1] / s0w[k]) : 0.0;
-        const double up = k + 1 < Nz ? (-dh * Jwl[k * 5 + 2]) * (s0w[k + 1] / s0w[k]) : 0.0;
-        const double den = diag - lo * cprev;
-        cprev = up / den;
-        dprev = (rhs[k] - lo * dprev) / den;
-        cp[k] = cprev;
-        rhs[k] = dprev;
+    if (lane < 64) {
+      const double* T3 = g.Tw + (size_t)j * Nz * 3;
+      // forward: d_k = (rhs_k - lo_k d_{k-1}) / den_k
+      double carry = 0.0;
+      for (int k0 = 0; k0 < Nz; k0 += 64) {
+        const int k = k0 + lane;
+        double a = 0.0, b = 0.0;
+        if (k < Nz) {
+          const double rd = T3[k * 3 + 1];
+          a = rhs[k] * rd;
+          b = -T3[k * 3 + 0] * rd;
+        }
+        if (lane == 0) {
+          a = fma(b, carry, a);
+          b = 0.0;
+        }
+        scan_linear(a, b, lane);
+        if (k < Nz) rhs[k] = a;
+        carry = __shfl(a, min(63, Nz - 1 - k0), 64);
       }
-      double x = 0.0;
-      for (int k = Nz - 1; k >= 0; --k) {
-        x = rhs[k] - cp[k] * x;
-        rhs[k] = x;
+      // backward: x_k = d_k - c_k x_{k+1}, lanes running down from k1
+      carry = 0.0;
+      for (int k1 = Nz - 1; k1 >= 0; k1 -= 64) {
+        const int k = k1 - lane;
+        double a = 0.0, b = 0.0;
+        if (k >= 0) {
+          a = rhs[k];
+          b = -T3[k * 3 + 2];
+        }
+        if (lane == 0) {
+          a = fma(b, carry, a);
+          b = 0.0;
+        }
+        scan_linear(a, b, lane);
+        if (k >= 0) rhs[k] = a;
+        carry = __shfl(a, min(63, k1), 64);
       }
     }
     lds_barrier();
@@ -868,6 +1050,19 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
     hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, fkind, h.Jc, h.s0,
                        dh, (double*)h.F);
   if (h.nw > 0) hipLaunchKernelGGL(k_prec_wall, dim3((h.nw + 255) / 256), dim3(256), 0, s, h.nw, h.Jw, dh, h.Fw);
+  if (kind == VTSA_PREC_ZLINE_GS) {
+    hipLaunchKernelGGL(k_gs_offline, dim3((h.n + OT - 1) / OT), dim3(256), 0, s, h.n, h.Jc, h.Jo);
+    if (h.nw > 0)
+      hipLaunchKernelGGL(k_gs_wall_factor, dim3((h.desc.Nth + 63) / 64), dim3(64), 0, s, h.desc.Nth, h.desc.Nz, h.n,
+                         h.Jw, h.s0, dh, h.Tw);
+    const int nt = h.ncol * npairs(h.desc.Nz) * 6;
+    if (prec == VTSA_FP32)
+      hipLaunchKernelGGL(k_gs_pairs<float>, dim3((nt + 255) / 256), dim3(256), 0, s, h.desc.Nz, h.ncol,
+                         (const float*)h.F, (float*)h.P);
+    else
+      hipLaunchKernelGGL(k_gs_pairs<double>, dim3((nt + 255) / 256), dim3(256), 0, s, h.desc.Nz, h.ncol,
+                         (const double*)h.F, (double*)h.P);
+  }
   h.prec_kind = kind;
   h.prec_prec = prec;
   h.prec_dh = dh;
@@ -886,18 +1081,20 @@ size_t partial_size(int ncol, int nw) {
   return a > b ? a : b;
 }
 
-// Algorithmic HBM bytes of one half-sweep over colour c, each datum counted once: per cell of a
-// cell line the 180 factor values, b~, s0 and the U written (6 each) and, unless it is the first
+// Algorithmic HBM bytes of one half-sweep over colour c, each datum counted once: per cell line
+// its factor (Nz x 180 values cell by cell, npairs x PP in pair form), per cell b~, s0 and the U
+// written (6 each) and, unless it is the first
 // half-sweep (U = 0), the four off-line 6x6 J blocks and the neighbours' U; per outer-ring cell
 // Jcw (4) and the wall's U; per wall cell Jw (self, z +- 1), b~, s0, U written and, unless first,
 // Jw (theta +- 1), Jwc (6), the theta-neighbours' U (2) and the outer cell's U (6).
-static double gs_bytes(const Handle& h, int c, bool first, size_t fsz) {
+static double gs_bytes(const Handle& h, int c, bool first, size_t fsz, int mode) {
   const double Nz = h.desc.Nz;
   const double cells = h.gs_ncell[c] * Nz, outer = h.gs_nouter[c] * Nz, walls = h.gs_nwall[c] * Nz;
-  const double per_cell = 180.0 * fsz + 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24));
+  const double per_line = (mode == 1 ? (double)npairs(h.desc.Nz) * PP : Nz * 180.0) * fsz;
+  const double per_cell = 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24));
   const double per_outer = first ? 0.0 : 8.0 * 5;
   const double per_wall = 8.0 * 6 + (first ? 0.0 : 8.0 * 16);
-  return cells * per_cell + outer * per_outer + walls * per_wall;
+  return h.gs_ncell[c] * per_line + cells * per_cell + outer * per_outer + walls * per_wall;
 }
 
 static hipEvent_t prof_event(Handle& h) {
@@ -963,7 +1160,7 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   const int mode = (h.gs_full >= 1 && gs_mode_ok<T, 1>(h)) ? 1 : 0;
   const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds0 = gs_lds_bytes<T, 0>(Nz);
   hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
-  GsArgs g{h.Jc, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1};
+  GsArgs g{h.Jo, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1, h.Tw};
   const int C = (int)h.gs_cstart.size() - 1;
   std::vector<int> seq;
   for (int c = 0; c < C; ++c) seq.push_back(c);
@@ -986,7 +1183,7 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     }
     if (mode == 1)
       hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
-                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done);
     else
       hipLaunchKernelGGL((k_gs_half<T, 0>), dim3(cnt), dim3(PW), lds0, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
@@ -994,7 +1191,7 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);
       h.prof_pending.push_back(e1);
-      h.prof_pending_bytes.push_back(gs_bytes(h, c, g.first != 0, sizeof(T)));
+      h.prof_pending_bytes.push_back(gs_bytes(h, c, g.first != 0, sizeof(T), mode));
       if (h.prof_pending_bytes.size() >= 512) (void)prof_resolve(h);
     }
   }

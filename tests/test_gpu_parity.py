@@ -261,23 +261,37 @@ def _line_gs_reference(M, m, r, sweeps):
     return x
 
 
-@pytest.mark.parametrize("fixture,over,sweeps,prec,tol", [("jac_small.npz", "small", 1, 0, 1e-9),
-                                                           ("jac_small.npz", "small", 2, 0, 1e-9),
-                                                           ("jac_cfg1.npz", "cfg1", 1, 0, 1e-9),
-                                                           ("jac_small.npz", "small", 1, 1, 2e-3)])
-def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, tol):
+@pytest.mark.parametrize("fixture,over,sweeps,prec,tol,tol_pair", [
+    ("jac_small.npz", "small", 1, 0, 1e-9, 1e-9),
+    ("jac_small.npz", "small", 2, 0, 1e-9, 1e-9),
+    ("jac_cfg1.npz", "cfg1", 1, 0, 1e-9, 1e-6),
+    ("jac_small.npz", "small", 1, 1, 2e-3, 2e-3),
+    ("jac_small.npz", "odd", 2, 0, 1e-9, 1e-9),
+    ("jac_small.npz", "odd", 1, 1, 2e-3, 2e-3)])
+def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, tol, tol_pair):
     """The line Gauss-Seidel preconditioner against the same algorithm on the dense matrix.
     FP32 factors are checked on the 3x4x6 state, whose scaled z-line blocks have condition
     numbers up to 3e6; on the seeded 4x4x20 state they reach 3e11 at this dh, beyond what FP32
-    factors can represent."""
+    factors can represent.  "odd": the 3x4x6 state with its last z-slice repeated (Nz = 7), so
+    the whole-line solve ends in an unpaired cell.
+
+    tol is the cell-by-cell solve's (segmented staging); tol_pair the whole-line solve's, which
+    advances two cells per step with products of neighbouring cells' factors (L1 L1, L1 L2, ^U ^U):
+    on the 4x4x20 lines (condition numbers to 3e11, so cond x eps ~ 3e-5) their rounding shows at
+    1.7e-7 of the solution, where the cell-by-cell order stays below 1e-9."""
     from dac_vtsa_amd import _abi
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.core.jacobian import FDJacobian
     from dac_vtsa_amd.core.solver import DeviceOps, PRECONDITIONERS
     z = golden(fixture)
-    cfg, m = gpu_model(SMALL if over == "small" else CFG1, 0, state_from(z))
+    if over == "odd":
+        ext = {k: np.concatenate([z[k], z[k][..., -1:]], axis=-1) for k in ("c", "q", "T", "Tw")}
+        cfg, m = gpu_model({**SMALL, "grid.Nz": 7}, 0, state_from(ext))
+        y = m.pack(state_from(ext))
+    else:
+        cfg, m = gpu_model(SMALL if over == "small" else CFG1, 0, state_from(z))
+        y = z["y"]
     jac = FDJacobian(m)
-    y = z["y"]
     J = jac(5.0, y).toarray()
     ops = DeviceOps(m)
     dh = 0.3
@@ -287,12 +301,12 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     ref = _line_gs_reference(np.eye(m.n_core) - dh * J, m, r, sweeps)
     sc = dv.to_host(s0)
     outs = []
-    for level in (1, 0):     # whole-line and segmented LDS staging: identical arithmetic
+    for level, lt in ((1, tol_pair), (0, tol)):   # whole line (pair recurrences), segmented (cell by cell)
         _abi.call("vtsa_set_option", m.handle, 1, level)
         out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
-        assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
+        assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < lt, level
         outs.append(out)
-    assert np.array_equal(outs[0], outs[1])
+    assert np.max(np.abs(outs[0] - outs[1]) / sc) / np.max(np.abs(ref) / sc) < tol_pair
 
 
 @pytest.mark.parametrize("dh,rtol", [(0.05, 1e-2), (0.05, 1e-4), (0.3, 1e-2)])
@@ -432,11 +446,11 @@ def test_conservation_projection_closes_the_totals():
 
 
 def test_projection_run_completes_small_heating_step():
-    """Regression: the 4x4x20 heating step from the seeded mid-cycle state (seed 1) with
-    conserve="project" runs into step-size underflow at t = 938 s (slightly negative
-    concentrations left by projections); run_step then runs the step again with the upstream
-    polish, which completes it, as the reference algorithm does.  The polish leaves stages
-    unclosed here (the reference leaves 100 of 260), so the balance gate is loose."""
+    """The 4x4x20 heating step from the seeded mid-cycle state (seed 1) with conserve="project"
+    completes: either the projection closes its stages, or (as in some roundings of the line
+    solves, where projections leave slightly negative concentrations and the step size
+    underflows near t = 938 s) run_step runs the step again with the upstream polish.  The polish
+    leaves stages unclosed here (the reference leaves 100 of 260), so its balance gate is loose."""
     from dac_vtsa_amd.core.grid import Grid
     from dac_vtsa_amd.properties import get_backend
     from dac_vtsa_amd.seeds import seeded_state
@@ -448,12 +462,42 @@ def test_projection_run_completes_small_heating_step():
     st, sr = run_step(seeded_state(g, "midcycle", 1), cfg, g, props, cfg.cycle[2],
                       integrator_kw={"conserve": "project"})
     assert sr.duration == 22400.0
-    assert sr.solver.get("projection_fallback") == 1
+    fell_back = sr.solver.get("projection_fallback") == 1
     mb, eb, _ = balances.check_cycle([sr], props.species)
-    print("balance after the polish fallback:", max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE"),
-          max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE"))
-    assert max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE") <= 1e-3
-    assert max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE") <= 1e-3
+    m_err = max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE")
+    e_err = max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE")
+    print("fallback:", fell_back, "balances:", m_err, e_err)
+    gate = 1e-3 if fell_back else 1e-6
+    assert m_err <= gate and e_err <= gate
+
+
+def test_projection_failure_reruns_the_step_with_the_polish(monkeypatch):
+    """run_step's recovery path: a SolverError from the run with the conservation projection is
+    answered by running the step again with the upstream polish (flag projection_fallback).  The
+    error is raised from the host on the first integrate call; 200 s of the 4x4x10 heating step."""
+    from dac_vtsa_amd.core import solver as S
+    from dac_vtsa_amd.core.grid import Grid
+    from dac_vtsa_amd.properties import get_backend
+    from dac_vtsa_amd.seeds import seeded_state
+    from dac_vtsa_amd.steps import run_step
+    seen = []
+    orig = S.TRBDF2.integrate
+
+    def integrate(self, *a, **k):
+        seen.append(self.conserve)
+        if len(seen) == 1:
+            raise S.SolverError("step size underflow (forced)")
+        return orig(self, *a, **k)
+
+    monkeypatch.setattr(S.TRBDF2, "integrate", integrate)
+    cfg = load_config(override_dict={**MID, "solver.conservation_tol": 1e-5})
+    g, props = Grid.from_config(cfg), get_backend(cfg)
+    step = {**cfg.cycle[2], "duration": 200.0}
+    st, sr = run_step(seeded_state(g, "midcycle", 0), cfg, g, props, step, integrator_kw={"conserve": "project"})
+    assert seen == ["project", "polish"]
+    assert sr.solver.get("projection_fallback") == 1
+    assert sr.duration == 200.0
+    assert np.all(np.isfinite(st.c)) and np.all(np.isfinite(st.T))
 
 
 # ------------------------------------------------------------------ integration
