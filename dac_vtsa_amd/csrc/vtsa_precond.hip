@@ -795,10 +795,10 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ Fac, GsArgs g, int cbeg,
               const GmresScalars* __restrict__ gs, int respect_done) {
   extern __shared__ __align__(16) double lds[];
+  const int L = g.order[cbeg + blockIdx.x];   // loaded together with the flag below
   if (respect_done && gs->done) return;
   constexpr int NT = MODE == 0 ? PW : GSW;
   const int lane = threadIdx.x;
-  const int L = g.order[cbeg + blockIdx.x];
   double* xs = lds;
   const double dh = g.dh;
   if (L < ncol) {
