@@ -33,7 +33,7 @@ static int upload(double** dst, const double* src, size_t count) {
 
 static void free_handle(Handle* h) {
   if (!h) return;
-  double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2,
+  double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2, h->FF,
                     h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
                     h->Vb, h->red.partial, h->red.result};
   for (double* b : bufs)
@@ -114,6 +114,7 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   rc |= upload(&h.cov, cov, nwall);
   rc |= dalloc(&h.pb, (size_t)NPROP * h.n);
   rc |= dalloc(&h.pb2, (size_t)NPROP * h.n);
+  rc |= dalloc(&h.FF, (size_t)4 * FaceTable::of(h.desc).count());
   rc |= dalloc(&h.ytmp, h.N);
   rc |= dalloc(&h.ftmp, h.N);
   rc |= dalloc(&h.vtmp, h.N);

@@ -44,6 +44,7 @@ struct Handle {
   // work buffers
   double* pb = nullptr;      // properties at the last residual point [12n]
   double* pb2 = nullptr;     // properties of a second point (matvec) [12n]
+  double* FF = nullptr;      // face fluxes of the last residual pass [4][FaceTable::count]
   double* ytmp = nullptr;    // perturbed state [N]
   double* ftmp = nullptr;    // residual at perturbed state [N]
   double* vtmp = nullptr;    // scratch [N]

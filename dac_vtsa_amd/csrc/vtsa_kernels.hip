@@ -4,9 +4,11 @@
 // a wavefront reads 64 consecutive cells of a z column with unit stride):
 //   k_props : thermodynamic state, gas properties, equilibrium and LDF rates of each cell,
 //             cached as 12 SoA property arrays (96 B/cell) and the loading rows of f;
-//   k_flux  : the 6 face fluxes of each cell from its stencil's cached properties, the
-//             telescoping divergence, sinks and wall/exchanger sources (species + energy rows),
-//             plus the wall rows (threads n .. n+nw-1).
+//   k_faces : every face flux once (one thread per face) from the cached properties of the
+//             face's stencil, into a face table;
+//   k_flux  : per cell the telescoping divergence of its 6 faces from the table, sinks and
+//             wall/exchanger sources (species + energy rows), plus the wall rows (threads
+//             n .. n+nw-1).
 // The finite-difference Jacobian perturbs one column per thread and recomputes only the
 // affected row cell (k_fd_cells / k_fd_wall): the values equal the upstream coloured forward
 // differences (core/jacobian.py:71-86) because a row reads only the columns of its stencil.
@@ -53,8 +55,25 @@ __device__ inline double wall_row(const Model& m, const double* __restrict__ Tw_
   return (((wf.Q_jac + wf.Q_amb) - wf.Q_wb) + cond) / d.C_wall;
 }
 
+// Face fluxes (4 rows of nf: F_0..F_2, H).  respect_done: skip when the GMRES cycle has ended.
+__global__ void __launch_bounds__(BLK) k_faces(Model m, double t, const double* __restrict__ y,
+                                               const double* __restrict__ pb, double* __restrict__ FF,
+                                               const GmresScalars* __restrict__ gs, int respect_done) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (respect_done && gs->done) return;
+  const FaceTable ft = FaceTable::of(m.d);
+  const int nf = ft.count();
+  if (f >= nf) return;
+  BaseAccess acc{y, pb, m.n};
+  double F[NG], H;
+  face_by_index(m, acc, ft, f, t, F, H);
+  for (int s = 0; s < NG; ++s) FF[(size_t)s * nf + f] = F[s];
+  FF[(size_t)NG * nf + f] = H;
+}
+
 __global__ void __launch_bounds__(BLK) k_flux(Model m, double t, const double* __restrict__ y,
-                                              const double* __restrict__ pb, double* __restrict__ f) {
+                                              const double* __restrict__ pb, const double* __restrict__ FF,
+                                              double* __restrict__ f) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = m.n;
   const vtsa_model_desc& d = m.d;
@@ -65,7 +84,7 @@ __global__ void __launch_bounds__(BLK) k_flux(Model m, double t, const double* _
     double rate[NA] = {f[3 * n + tid], f[4 * n + tid]};
     double Tw = (d.wall_on && i == d.Nr - 1) ? y[6 * n + j * d.Nz + k] : 0.0;
     double out[4];
-    cell_flux(m, acc, Tw, i, j, k, t, rate, out);
+    cell_flux_table(m, acc, FF, FaceTable::of(d), Tw, i, j, k, t, rate, out);
     f[tid] = out[0]; f[n + tid] = out[1]; f[2 * n + tid] = out[2]; f[5 * n + tid] = out[3];
   } else if (tid < n + m.nw) {
     const int w = tid - n;
@@ -114,7 +133,8 @@ __global__ void __launch_bounds__(BLK) k_mv_props(Model m, const double* __restr
 }
 
 __global__ void __launch_bounds__(BLK) k_mv_flux(Model m, double t, const double* __restrict__ yp,
-                                                 const double* __restrict__ pb, const double* __restrict__ fp,
+                                                 const double* __restrict__ pb, const double* __restrict__ FF,
+                                                 const double* __restrict__ fp,
                                                  const double* __restrict__ u, const double* __restrict__ s,
                                                  const double* __restrict__ f, const GmresScalars* __restrict__ gs,
                                                  int respect_done, double dh, double* __restrict__ w) {
@@ -131,7 +151,7 @@ __global__ void __launch_bounds__(BLK) k_mv_flux(Model m, double t, const double
     double rate[NA] = {fp[3 * n + tid], fp[4 * n + tid]};
     double Tw = (d.wall_on && i == d.Nr - 1) ? yp[6 * n + j * d.Nz + k] : 0.0;
     double out[4];
-    cell_flux(m, acc, Tw, i, j, k, t, rate, out);
+    cell_flux_table(m, acc, FF, FaceTable::of(d), Tw, i, j, k, t, rate, out);
     const int rows[4] = {0, 1, 2, 5};
     for (int r = 0; r < 4; ++r) {
       int idx = rows[r] * n + tid;
@@ -464,7 +484,9 @@ void launch_props(Handle& h, const double* y, double* pb, double* f, double* qst
 }
 
 void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s) {
-  hipLaunchKernelGGL(k_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, s, h.m, t, y, pb, f);
+  const int nf = FaceTable::of(h.desc).count();
+  hipLaunchKernelGGL(k_faces, dim3(nblk(nf)), dim3(BLK), 0, s, h.m, t, y, pb, h.FF, h.gs, 0);
+  hipLaunchKernelGGL(k_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, s, h.m, t, y, pb, h.FF, f);
 }
 
 void launch_acc(Handle& h, double t, const double* y, const double* pb, double* acc, hipStream_t s) {
@@ -500,8 +522,10 @@ void launch_matvec_residual(Handle& h, double t, const double* z, const double* 
   const int rd = respect_done ? 1 : 0;
   hipLaunchKernelGGL(k_mv_props, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, z, u, s, f, h.gs, rd, dh, h.ytmp,
                      h.pb2, h.ftmp, w);
-  hipLaunchKernelGGL(k_mv_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, t, h.ytmp, h.pb2, h.ftmp, u, s, f,
-                     h.gs, rd, dh, w);
+  const int nf = FaceTable::of(h.desc).count();
+  hipLaunchKernelGGL(k_faces, dim3(nblk(nf)), dim3(BLK), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.gs, rd);
+  hipLaunchKernelGGL(k_mv_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.ftmp, u,
+                     s, f, h.gs, rd, dh, w);
 }
 
 }  // namespace vtsa

@@ -608,62 +608,15 @@ __device__ inline double wall_conduction(const vtsa_model_desc& d, const double*
 }
 
 // ---------------------------------------------------------------- one cell's flux rows
-// d(c_s)/dt (s = 0..2) and de/dt of cell (i, j, k): telescoping divergence of the face fluxes in
-// the upstream order (physics/mass.py:61-74), sink, wall and exchanger sources
-// (core/residual.py:294-298,326,352-359).  rate[] are the cell's LDF rates; Tw_self is the wall
-// temperature next to the cell (used for the outer ring when the wall is resolved).
+// d(c_s)/dt (s = 0..2) and de/dt of cell (i, j, k) from the divergence of its face fluxes:
+// sink, wall and exchanger sources (core/residual.py:294-298,326,352-359).  rate[] are the cell's
+// LDF rates; Tw_self is the wall temperature next to the cell (used for the outer ring when the
+// wall is resolved).
 template <class Acc>
-__device__ inline void cell_flux(const Model& m, const Acc& acc, double Tw_self, int i, int j, int k, double t,
-                                 const double rate[NA], double out[4]) {
+__device__ inline void cell_rows(const Model& m, const Acc& acc, const double div[NG + 1], double Tw_self, int i,
+                                 int j, int k, double t, const double rate[NA], double out[4]) {
   const vtsa_model_desc& d = m.d;
-  const int Nr = d.Nr, Nth = d.Nth, Nz = d.Nz;
-  const int cell = (i * Nth + j) * Nz + k;
-  double Fl[NG], Fu[NG], Hl, Hu;
-  if (k == 0) {
-    Props p;
-    acc.props(cell, p);
-    double c[NG] = {acc.c(0, cell), acc.c(1, cell), acc.c(2, cell)};
-    EndFlux b;
-    end_flux(d, 0, t, m.Az[i], m.feed_flux[i * Nth + j], c, p, b);
-    for (int s = 0; s < NG; ++s) Fl[s] = b.F_in[s];
-    Hl = b.H_in;
-  } else {
-    z_face(m, acc, i, j, k - 1, Fl, Hl);
-  }
-  if (k == Nz - 1) {
-    Props p;
-    acc.props(cell, p);
-    double c[NG] = {acc.c(0, cell), acc.c(1, cell), acc.c(2, cell)};
-    EndFlux b;
-    end_flux(d, 1, t, m.Az[i], m.feed_flux[i * Nth + j], c, p, b);
-    for (int s = 0; s < NG; ++s) Fu[s] = -b.F_in[s];
-    Hu = -b.H_in;
-  } else {
-    z_face(m, acc, i, j, k, Fu, Hu);
-  }
-  double div[NG + 1];
-  for (int s = 0; s < NG; ++s) div[s] = Fu[s] - Fl[s];
-  div[NG] = Hu - Hl;
-  if (Nr > 1) {
-    double F[NG], H;
-    if (i < Nr - 1) {
-      r_face(m, acc, i, j, k, F, H);
-      for (int s = 0; s < NG; ++s) div[s] = div[s] + F[s];
-      div[NG] = div[NG] + H;
-    }
-    if (i > 0) {
-      r_face(m, acc, i - 1, j, k, F, H);
-      for (int s = 0; s < NG; ++s) div[s] = div[s] - F[s];
-      div[NG] = div[NG] - H;
-    }
-  }
-  if (Nth > 1) {
-    double Fp[NG], Hp, Fm[NG], Hm;
-    t_face(m, acc, i, j, k, Fp, Hp);
-    t_face(m, acc, i, (j == 0 ? Nth - 1 : j - 1), k, Fm, Hm);
-    for (int s = 0; s < NG; ++s) div[s] = div[s] + (Fp[s] - Fm[s]);
-    div[NG] = div[NG] + (Hp - Hm);
-  }
+  const int cell = (i * d.Nth + j) * d.Nz + k;
   const double V = m.V[i];
   const double eV = d.eps_t * V;
   const double rfac = d.rho_b / d.eps_t;
@@ -671,10 +624,10 @@ __device__ inline void cell_flux(const Model& m, const Acc& acc, double Tw_self,
   out[1] = (-div[1]) / eV - rfac * rate[1];
   out[2] = (-div[2]) / eV - rfac * 0.0;
   double src = 0.0;
-  if (i == Nr - 1) {
+  if (i == d.Nr - 1) {
     Props o;
     acc.props(cell, o);
-    WallFlows wf = wall_flows(m, o, Tw_self, m.cov[j * Nz + k], t);
+    WallFlows wf = wall_flows(m, o, Tw_self, m.cov[j * d.Nz + k], t);
     src = src + wf.Q_wb;
   }
   if (d.hx_on) {
@@ -682,6 +635,149 @@ __device__ inline void cell_flux(const Model& m, const Acc& acc, double Tw_self,
     src = src + (d.hA_v * V) * (Tj - acc.T(cell));
   }
   out[3] = ((-div[NG]) + src) / V;
+}
+
+// The boundary flux of an end face of column (i, j) with the sign the telescoping divergence
+// uses: inflow at the inlet end (k = 0, face below the cell), minus the inflow at the outlet end
+// (face above cell Nz - 1).
+template <class Acc>
+__device__ inline void end_face(const Model& m, const Acc& acc, int end, int i, int j, double t, double F[NG],
+                                double& H) {
+  const vtsa_model_desc& d = m.d;
+  const int cell = (i * d.Nth + j) * d.Nz + (end == 0 ? 0 : d.Nz - 1);
+  Props p;
+  acc.props(cell, p);
+  double c[NG] = {acc.c(0, cell), acc.c(1, cell), acc.c(2, cell)};
+  EndFlux b;
+  end_flux(d, end, t, m.Az[i], m.feed_flux[i * d.Nth + j], c, p, b);
+  for (int s = 0; s < NG; ++s) F[s] = end == 0 ? b.F_in[s] : -b.F_in[s];
+  H = end == 0 ? b.H_in : -b.H_in;
+}
+
+// Telescoping divergence of the face fluxes in the upstream order (physics/mass.py:61-74), the
+// faces given by face(kind, i, j, k, F, H): kind 0 the z face below cell k, 3 the z face above
+// cell k, 1 the radial face between rings i and i+1, 2 the theta face between j and j+1.
+template <class FaceFn>
+__device__ inline void cell_divergence(const vtsa_model_desc& d, int i, int j, int k, FaceFn face,
+                                       double div[NG + 1]) {
+  double Fl[NG], Fu[NG], Hl, Hu;
+  face(0, i, j, k, Fl, Hl);
+  face(3, i, j, k, Fu, Hu);
+  for (int s = 0; s < NG; ++s) div[s] = Fu[s] - Fl[s];
+  div[NG] = Hu - Hl;
+  if (d.Nr > 1) {
+    double F[NG], H;
+    if (i < d.Nr - 1) {
+      face(1, i, j, k, F, H);
+      for (int s = 0; s < NG; ++s) div[s] = div[s] + F[s];
+      div[NG] = div[NG] + H;
+    }
+    if (i > 0) {
+      face(1, i - 1, j, k, F, H);
+      for (int s = 0; s < NG; ++s) div[s] = div[s] - F[s];
+      div[NG] = div[NG] - H;
+    }
+  }
+  if (d.Nth > 1) {
+    double Fp[NG], Hp, Fm[NG], Hm;
+    face(2, i, j, k, Fp, Hp);
+    face(2, i, (j == 0 ? d.Nth - 1 : j - 1), k, Fm, Hm);
+    for (int s = 0; s < NG; ++s) div[s] = div[s] + (Fp[s] - Fm[s]);
+    div[NG] = div[NG] + (Hp - Hm);
+  }
+}
+
+// One face of the divergence computed in place from the stencil (acc).
+template <class Acc>
+__device__ inline void any_face(const Model& m, const Acc& acc, int kind, int i, int j, int k, double t,
+                                double F[NG], double& H) {
+  const int Nz = m.d.Nz;
+  if (kind == 0) {
+    if (k == 0) end_face(m, acc, 0, i, j, t, F, H);
+    else z_face(m, acc, i, j, k - 1, F, H);
+  } else if (kind == 3) {
+    if (k == Nz - 1) end_face(m, acc, 1, i, j, t, F, H);
+    else z_face(m, acc, i, j, k, F, H);
+  } else if (kind == 1) {
+    r_face(m, acc, i, j, k, F, H);
+  } else {
+    t_face(m, acc, i, j, k, F, H);
+  }
+}
+
+// The flux rows of one cell, every face computed by this thread (the finite-difference
+// Jacobian's perturbed evaluations).
+template <class Acc>
+__device__ inline void cell_flux(const Model& m, const Acc& acc, double Tw_self, int i, int j, int k, double t,
+                                 const double rate[NA], double out[4]) {
+  double div[NG + 1];
+  cell_divergence(m.d, i, j, k,
+                  [&](int kind, int fi, int fj, int fk, double* F, double& H) {
+                    any_face(m, acc, kind, fi, fj, fk, t, F, H);
+                  },
+                  div);
+  cell_rows(m, acc, div, Tw_self, i, j, k, t, rate, out);
+}
+
+// ---------------------------------------------------------------- face-parallel flux
+// The residual's faces, each computed once by one thread into a face table (SoA, 4 rows of nf:
+// F_0, F_1, F_2, H), then summed per cell.  Face index space:
+//   [0, ncol (Nz + 1))                z faces: column (i, j), slot k = the face below cell k
+//                                      (slot Nz: the face above the last cell)
+//   [.., + (Nr - 1) Nth Nz)           radial faces (i < Nr - 1, j, k)
+//   [.., + Nth > 1 ? n : 0)           theta faces (i, j, k)
+struct FaceTable {
+  int nzf, nrf, ntf;
+  __device__ __host__ inline int count() const { return nzf + nrf + ntf; }
+  __device__ __host__ static FaceTable of(const vtsa_model_desc& d) {
+    FaceTable f;
+    f.nzf = d.Nr * d.Nth * (d.Nz + 1);
+    f.nrf = (d.Nr - 1) * d.Nth * d.Nz;
+    f.ntf = d.Nth > 1 ? d.Nr * d.Nth * d.Nz : 0;
+    return f;
+  }
+  __device__ inline int index(const vtsa_model_desc& d, int kind, int i, int j, int k) const {
+    if (kind == 0 || kind == 3) return (i * d.Nth + j) * (d.Nz + 1) + k + (kind == 3 ? 1 : 0);
+    if (kind == 1) return nzf + (i * d.Nth + j) * d.Nz + k;
+    return nzf + nrf + (i * d.Nth + j) * d.Nz + k;
+  }
+};
+
+template <class Acc>
+__device__ inline void face_by_index(const Model& m, const Acc& acc, const FaceTable& ft, int f, double t,
+                                     double F[NG], double& H) {
+  const vtsa_model_desc& d = m.d;
+  int kind, i, j, k;
+  if (f < ft.nzf) {
+    const int col = f / (d.Nz + 1);
+    k = f - col * (d.Nz + 1); i = col / d.Nth; j = col - i * d.Nth;
+    kind = 0;
+    if (k == d.Nz) { kind = 3; k = d.Nz - 1; }   // the face above the last cell
+  } else {
+    const int g = f < ft.nzf + ft.nrf ? f - ft.nzf : f - ft.nzf - ft.nrf;
+    kind = f < ft.nzf + ft.nrf ? 1 : 2;
+    k = g % d.Nz;
+    const int col = g / d.Nz;
+    j = col % d.Nth; i = col / d.Nth;
+  }
+  any_face(m, acc, kind, i, j, k, t, F, H);
+}
+
+// The flux rows of one cell from the face table.
+template <class Acc>
+__device__ inline void cell_flux_table(const Model& m, const Acc& acc, const double* __restrict__ FF,
+                                       const FaceTable& ft, double Tw_self, int i, int j, int k, double t,
+                                       const double rate[NA], double out[4]) {
+  const int nf = ft.count();
+  double div[NG + 1];
+  cell_divergence(m.d, i, j, k,
+                  [&](int kind, int fi, int fj, int fk, double* F, double& H) {
+                    const int x = ft.index(m.d, kind, fi, fj, fk);
+                    for (int s = 0; s < NG; ++s) F[s] = FF[(size_t)s * nf + x];
+                    H = FF[(size_t)NG * nf + x];
+                  },
+                  div);
+  cell_rows(m, acc, div, Tw_self, i, j, k, t, rate, out);
 }
 
 }  // namespace vtsa
