@@ -42,10 +42,59 @@ __device__ inline double load_partial(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sum of partial[0 .. cnt) by one wavefront (lane-strided, then butterfly): deterministic.
+// Sums of partial rows q = q0, q0 + qs, .. (R of them, those below nval; rows of stride RMAXB)
+// by one wavefront: per row, lane l adds partials l, l + 64, .. in order, then a butterfly;
+// deterministic.  The loads of RMAXB partials of every row are issued before their adds.  Lane 0
+// stores row q's sum to out[q].
+template <int R>
+__device__ inline void wave_sum_rows(const double* __restrict__ partial, int cnt, int q0, int qs, int nval,
+                                     double* out) {
+  constexpr int CM = RMAXB / 64;
+  const int lane = threadIdx.x & 63;
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int base = 0; base < cnt; base += 64 * CM) {
+    double v[R][CM];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        const int q = q0 + r * qs, b = base + lane + 64 * c;
+        v[r][c] = (q < nval && b < cnt) ? load_partial(partial + (size_t)q * RMAXB + b) : 0.0;
+      }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (base + lane + 64 * c < cnt) acc[r] += v[r][c];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] += __shfl_xor(acc[r], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (q0 + r * qs < nval) out[q0 + r * qs] = acc[r];
+}
+
+// Sum of partial[0 .. cnt) by one wavefront, in the order of wave_sum_rows; every lane returns it.
 __device__ inline double wave_sum_strided(const double* __restrict__ partial, int cnt) {
+  constexpr int CM = RMAXB / 64;
+  const int lane = threadIdx.x & 63;
   double acc = 0.0;
-  for (int b = threadIdx.x & 63; b < cnt; b += 64) acc += load_partial(partial + b);
+  for (int base = 0; base < cnt; base += 64 * CM) {
+    double v[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      const int b = base + lane + 64 * c;
+      v[c] = b < cnt ? load_partial(partial + b) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < CM; ++c)
+      if (base + lane + 64 * c < cnt) acc += v[c];
+  }
   return wave_sum(acc);
 }
 
@@ -83,10 +132,7 @@ __device__ inline bool last_block_totals(unsigned* counter, bool* sflag, const d
                                          double* out) {
   if (!ticket_last(counter, sflag)) return false;
   const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  for (int q = wv; q < nval; q += nwv) {
-    const double t = wave_sum_strided(partial + (size_t)q * RMAXB, gridDim.x);
-    if ((threadIdx.x & 63) == 0) out[q] = t;
-  }
+  for (int q0 = wv; q0 < nval; q0 += 4 * nwv) wave_sum_rows<4>(partial, gridDim.x, q0, nwv, nval, out);
   __syncthreads();
   if (threadIdx.x == 0) ticket_reset(counter);
   return true;
