@@ -197,7 +197,9 @@ def batch_leg(args):
             "wall_s": r["wall_s"], "sim_s": r["sim_s"], "failed": r["failed"],
             "cells_range": [min(c["cells"] for c in r["cases"]), max(c["cells"] for c in r["cases"])],
             "max_mass_rel_err": max((c["max_mass_rel_err"] for c in ok), default=None),
-            "max_energy_rel_err": max((c["max_energy_rel_err"] for c in ok), default=None)}
+            "max_energy_rel_err": max((c["max_energy_rel_err"] for c in ok), default=None),
+            "projection_fallback_cases": sum(1 for c in ok if c.get("projection_fallback")),
+            "unclosed_stages": sum(c.get("balance_unclosed", 0) for c in ok)}
 
 
 def learned_leg(cfg, grid, props, state, args, st_plain, plain_value):

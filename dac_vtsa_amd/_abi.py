@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libvtsa.so")
 D = C.c_double
 I32 = C.c_int32
 P = C.c_void_p
+SHIFT_NO_REPORT = 2   # vtsa_cons_shift: VTSA_SHIFT_NO_REPORT (include/vtsa.h)
 
 
 class ModelDesc(C.Structure):
@@ -79,6 +80,7 @@ _SIGS = {
     "vtsa_scale": [P, P, D, P, P],
     "vtsa_rms": [P, P, P, I32, C.POINTER(D), P],
     "vtsa_newton_eval": [P, D, P, D, P, P, P, P, C.POINTER(D), C.POINTER(D), P],
+    "vtsa_newton_residual": [P, D, P, D, P, P, P, P, P, P],
     "vtsa_newton_try": [P, D, P, P, D, D, P, P, P, P, P, C.POINTER(D), C.POINTER(D), C.POINTER(D), P],
     "vtsa_cons": [P, P, P, P, P, P],
     "vtsa_inventories": [P, P, P, P],

@@ -96,6 +96,15 @@ class DeviceOps:
         self.m.nfev += 1
         return rn.value, pn.value
 
+    def newton_residual(self, t, z, dh, rk, s, f, R, norm=True):
+        """f and R at z as newton_eval, without the preconditioned norm; rn (waited for) only when
+        norm is true, else None and nothing is waited for."""
+        rn = C.c_double()
+        _abi.call("vtsa_newton_residual", self.h, float(t), dv.ptr(z), float(dh), dv.ptr(rk), dv.ptr(s),
+                  dv.ptr(f), dv.ptr(R), C.byref(rn) if norm else None, self.st())
+        self.m.nfev += 1
+        return rn.value if norm else None
+
     def newton_try(self, t, z, dz, lam, dh, rk, s, zt, ft, Rt):
         rn, pn, step = C.c_double(), C.c_double(), C.c_double()
         _abi.call("vtsa_newton_try", self.h, float(t), dv.ptr(z), dv.ptr(dz), float(lam), float(dh), dv.ptr(rk),
@@ -124,12 +133,13 @@ class DeviceOps:
         _abi.call("vtsa_cons_norm2", self.h, dv.ptr(s), out, self.st())
         return np.array(out[:])
 
-    def cons_shift(self, z, s, lam, out, truncate=False):
+    def cons_shift(self, z, s, lam, out, truncate=False, report=True):
         """out = z - S^2 W^T lam.  Returns True when a concentration or loading would fall more
-        than one tolerance unit below zero (with truncate=True those keep 0.1 of their value)."""
+        than one tolerance unit below zero (with truncate=True those keep 0.1 of their value).
+        report=False: the plain shift, not waited for; returns False."""
         lam = (C.c_double * 4)(*[float(v) for v in lam])
-        rc = _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, int(bool(truncate)), dv.ptr(out),
-                       self.st())
+        mode = int(bool(truncate)) if report else _abi.SHIFT_NO_REPORT
+        rc = _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, mode, dv.ptr(out), self.st())
         return rc == 1
 
     def error_norm(self, y, z2, y3, rtol):
@@ -187,11 +197,11 @@ def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, m
             eps = 1e-2 * ws[l] / d2[l]
             lam = np.zeros(4)
             lam[l] = eps
-            ops.cons_shift(zc, s, lam, zt)
-            rn, _ = ops.newton_eval(t, zt, dh, rhs_known, s, ft, Rt)
-            if not _finite(rn):
-                return None
+            ops.cons_shift(zc, s, lam, zt, report=False)
+            ops.newton_residual(t, zt, dh, rhs_known, s, ft, Rt, norm=False)
             wl, _ = ops.cons(Rt, s)
+            if not np.all(np.isfinite(wl)):
+                return None
             B[:, l] = (wl - wr) / eps
         lam = np.zeros(4)
         try:
@@ -201,7 +211,7 @@ def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, m
         zn, fn, Rn = dv.empty(ops.N), dv.empty(ops.N), dv.empty(ops.N)
         if ops.cons_shift(zc, s, lam, zn):        # a concentration or loading would go negative:
             ops.cons_shift(zc, s, lam, zn, True)  # truncate them as a Newton update does
-        rn, _ = ops.newton_eval(t, zn, dh, rhs_known, s, fn, Rn)
+        rn = ops.newton_residual(t, zn, dh, rhs_known, s, fn, Rn)
         if not _finite(rn):
             return None
         zc, fc, Rc = zn, fn, Rn

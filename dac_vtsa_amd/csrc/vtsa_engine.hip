@@ -307,6 +307,14 @@ int vtsa_newton_eval(vtsa_handle* H, double t, const double* z, double dh, const
   return newton_eval_impl(H->h, t, z, dh, rk, s, f, R, rn, pn, S(st));
 }
 
+int vtsa_newton_residual(vtsa_handle* H, double t, const double* z, double dh, const double* rk, const double* s,
+                         double* f, double* R, double* rn, vtsa_stream st) {
+  if (!H || !z || !rk || !s || !f || !R) { set_error("vtsa_newton_residual: null argument"); return -1; }
+  const int rc = newton_residual_impl(H->h, t, z, dh, rk, s, f, R, rn, S(st));
+  if (rc) return rc;
+  return finish(S(st));
+}
+
 int vtsa_newton_try(vtsa_handle* H, double t, const double* z, const double* dz, double lam, double dh,
                     const double* rk, const double* s, double* zt, double* ft, double* Rt, double* rn, double* pn,
                     double* step, vtsa_stream st) {
@@ -331,7 +339,8 @@ int vtsa_cons_norm2(vtsa_handle* H, const double* s, double* out, vtsa_stream st
 int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, int32_t truncate,
                     double* out, vtsa_stream st) {
   if (!H || !z || !s || !lambda || !out) { set_error("vtsa_cons_shift: null argument"); return -1; }
-  const int rc = cons_shift_impl(H->h, z, s, lambda, truncate ? 1 : 0, out, S(st));
+  const int mode = truncate == VTSA_SHIFT_NO_REPORT ? VTSA_SHIFT_NO_REPORT : (truncate ? 1 : 0);
+  const int rc = cons_shift_impl(H->h, z, s, lambda, mode, out, S(st));
   if (rc < 0) return rc;
   const int e = finish(S(st));
   return e != 0 ? e : rc;

@@ -124,6 +124,8 @@ int reduce_rms(Handle& h, const double* a, const double* s, int len, double* out
 int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mode, double* out, hipStream_t st);
 void launch_residual_combine(Handle& h, const double* z, const double* f, double dh, const double* rhs_known,
                              double* R, hipStream_t st);
+int newton_residual_impl(Handle& h, double t, const double* z, double dh, const double* rk, const double* s,
+                         double* f, double* R, double* rn, hipStream_t st);
 int newton_eval_impl(Handle& h, double t, const double* z, double dh, const double* rk, const double* s, double* f,
                      double* R, double* rn, double* pn, hipStream_t st);
 int newton_try_impl(Handle& h, double t, const double* z, const double* dz, double lam, double dh, const double* rk,

@@ -544,6 +544,11 @@ int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mo
 
 int cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], int truncate, double* out,
                     hipStream_t st) {
+  if (truncate == VTSA_SHIFT_NO_REPORT) {   // plain shift; the flag is written but not read
+    hipLaunchKernelGGL(k_cons_shift, dim3(eblk(h.N)), dim3(256), 0, st, h.m, z, s, lam[0], lam[1], lam[2], lam[3],
+                       out, 0, h.red.counter + 12);
+    return 0;
+  }
   unsigned* flag = h.red.counter + 12;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, flag, 0u, 1);
   hipLaunchKernelGGL(k_cons_shift, dim3(eblk(h.N)), dim3(256), 0, st, h.m, z, s, lam[0], lam[1], lam[2], lam[3], out,
@@ -555,6 +560,20 @@ int cons_shift_impl(Handle& h, const double* z, const double* s, const double la
 }
 
 // Newton building blocks (host drivers; each returns host scalars)
+int newton_residual_impl(Handle& h, double t, const double* z, double dh, const double* rk, const double* s,
+                         double* f, double* R, double* rn, hipStream_t st) {
+  launch_props(h, z, h.pb, f, nullptr, st);
+  launch_flux(h, t, z, h.pb, f, st);
+  hipLaunchKernelGGL(k_residual_combine, dim3(rgrid(h.N)), dim3(RBLK), 0, st, h.N, z, f, dh, rk, s, R,
+                     h.red.partial, h.red.counter + 2, h.red.result + 0);
+  if (!rn) return 0;
+  if (check(hipMemcpyAsync(h.host_scalars, h.red.result, sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
+    return -1;
+  if (check(hipStreamSynchronize(st), "sync")) return -1;
+  *rn = h.host_scalars[0];
+  return 0;
+}
+
 int newton_eval_impl(Handle& h, double t, const double* z, double dh, const double* rk, const double* s, double* f,
                      double* R, double* rn, double* pn, hipStream_t st) {
   launch_props(h, z, h.pb, f, nullptr, st);

@@ -143,6 +143,11 @@ int vtsa_rms(vtsa_handle* h, const double* a, const double* s, int32_t len, doub
 /* f = f(t,z); R = z - dh f - rhs_known; rn = rms(R/s); pn = rms(M^-1 R / s) */
 int vtsa_newton_eval(vtsa_handle* h, double t, const double* z, double dh, const double* rhs_known,
                      const double* s, double* f, double* R, double* rn, double* pn, vtsa_stream st);
+/* The residual half of vtsa_newton_eval: f = f(t,z); R = z - dh f - rhs_known; with rn non-NULL
+ * rn = rms(R/s) on the host (synchronises the stream), with rn NULL nothing is waited for.  No
+ * preconditioner apply.  Used where pn is not needed (the conservation projection's residuals). */
+int vtsa_newton_residual(vtsa_handle* h, double t, const double* z, double dh, const double* rhs_known,
+                         const double* s, double* f, double* R, double* rn, vtsa_stream st);
 /* z_try = truncate(z + lam dz, z); f_try, R_try at z_try; rn, pn as above; step = rms((z_try-z)/s) */
 int vtsa_newton_try(vtsa_handle* h, double t, const double* z, const double* dz, double lam, double dh,
                     const double* rhs_known, const double* s, double* z_try, double* f_try, double* R_try,
@@ -157,7 +162,9 @@ int vtsa_cons_norm2(vtsa_handle* h, const double* s, double* out, vtsa_stream st
  * variables that changes the conserved totals W z by -W S^2 W^T lambda.  out may alias z.
  * Returns 1 when a non-negative concentration or loading would fall below -s (one tolerance
  * unit under zero); with truncate = 1 such entries keep 0.1 of their value instead, as a Newton
- * update does (core/solver.py:99-108).  Synchronises the stream. */
+ * update does (core/solver.py:99-108).  Synchronises the stream, except with truncate =
+ * VTSA_SHIFT_NO_REPORT: plain shift, returns 0 without waiting. */
+enum { VTSA_SHIFT_NO_REPORT = 2 };
 int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, int32_t truncate,
                     double* out, vtsa_stream st);
 /* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall
