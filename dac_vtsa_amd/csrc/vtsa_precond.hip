@@ -790,11 +790,49 @@ __global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restr
 //         count of outstanding copies, the right-hand side is gathered by all of them, and the
 //         first wavefront runs the pair recurrences between data-parallel passes.
 // Both solve the same line systems; the roundings differ.
+// The apply's outputs, on a line's last visit: out = U / post for the line's unknowns, and the
+// rms of out / red_s over all unknowns (the sum of squares per line into partial[L]; the last
+// line to finish sums the lines' partials in line order).  Returns this thread's sum of squares.
+__device__ inline double gs_out(const ApplyArgs& a, int idx, double u) {
+  const double o = a.post ? u / a.post[idx] : u;
+  a.out[idx] = o;
+  const double q = a.red_s ? o / a.red_s[idx] : o;
+  return q * q;
+}
+
+__device__ inline void gs_out_norm(const ApplyArgs& a, double local, int L, int nlines, int N,
+                                   double* __restrict__ partial, unsigned* counter, double* sh, bool* sflag) {
+  const double bs = wave_block_sum(local, sh);
+  if (threadIdx.x == 0) store_partial(partial + L, bs);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned tk = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *sflag = (tk == (unsigned)nlines - 1);
+  }
+  __syncthreads();
+  if (!*sflag || threadIdx.x >= 64) return;
+  const double tot = wave_sum_strided(partial, nlines);
+  if (threadIdx.x != 0) return;
+  ticket_reset(counter);
+  const double rms = sqrt(tot / (double)N);
+  if (a.red_out) *a.red_out = rms;
+  if (a.gs) {
+    a.gs->unorm = rms;
+    a.gs->zero_u = (rms == 0.0) ? 1 : 0;
+    a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * a.zs_norm) / rms;
+  }
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ Fac, GsArgs g, int cbeg,
-              const GmresScalars* __restrict__ gs, int respect_done) {
+              const GmresScalars* __restrict__ gs, int respect_done, ApplyArgs a, int final_visit, int nlines,
+              int N, double* __restrict__ partial, unsigned* counter) {
   extern __shared__ __align__(16) double lds[];
+  __shared__ double sh_norm[(MODE == 0 ? PW : GSW) / 64];
+  __shared__ bool s_last;
+  double local = 0.0;   // sum of squares of this thread's outputs (last visit only)
   const int L = g.order[cbeg + blockIdx.x];   // loaded together with the flag below
   if (respect_done && gs->done) return;
   constexpr int NT = MODE == 0 ? PW : GSW;
@@ -894,7 +932,9 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     for (int e = lane; e < 6 * Nz; e += NT) {
       const int r = e / Nz, k = e - r * Nz;
       const int idx = r * n + X0 + k;
-      g.U[idx] = g.s0[idx] * xs[k * 6 + r];
+      const double u = g.s0[idx] * xs[k * 6 + r];
+      g.U[idx] = u;
+      if (final_visit) local += gs_out(a, idx, u);
     }
   } else {
     // wall line j: tridiagonal along z (axial conduction), couplings to the outer cells and to
@@ -957,37 +997,12 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     lds_barrier();
     for (int k = lane; k < Nz; k += NT) {
       const int idx = 6 * n + j * Nz + k;
-      g.U[idx] = g.s0[idx] * rhs[k];
+      const double u = g.s0[idx] * rhs[k];
+      g.U[idx] = u;
+      if (final_visit) local += gs_out(a, idx, u);
     }
   }
-}
-
-// out = U / post and the norm of out / red_s (the apply's side outputs).
-__global__ void __launch_bounds__(RBLK) k_gs_final(int N, ApplyArgs a, const double* __restrict__ U,
-                                                   double* __restrict__ partial, unsigned* counter) {
-  __shared__ double sh[RBLK / 64];
-  __shared__ bool slast;
-  if (a.respect_done && a.gs->done) return;
-  double local = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    double o = U[i];
-    if (a.post) o = o / a.post[i];
-    a.out[i] = o;
-    double q = a.red_s ? o / a.red_s[i] : o;
-    local += q * q;
-  }
-  double bs = wave_block_sum(local, sh);
-  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
-  double tot;
-  if (last_block_total(counter, &slast, partial, &tot)) {
-    double rms = sqrt(tot / (double)N);
-    if (a.red_out) *a.red_out = rms;
-    if (a.gs) {
-      a.gs->unorm = rms;
-      a.gs->zero_u = (rms == 0.0) ? 1 : 0;
-      a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * a.zs_norm) / rms;
-    }
-  }
+  if (final_visit) gs_out_norm(a, local, L, nlines, N, partial, counter, sh_norm, &s_last);
 }
 
 // ------------------------------------------------------------------ host side (preconditioner)
@@ -1075,7 +1090,7 @@ bool zline_fits(int Nz) { return zline_lds_bytes<double>(Nz) <= 65536; }
 
 size_t partial_size(int ncol, int nw) {
   size_t a = (size_t)RMAXB * (MAX_RESTART + 2);
-  size_t pg = (size_t)ncol + (nw + PW * 4 - 1) / (PW * 4) + 1;   // apply grid: one block per column
+  size_t pg = (size_t)ncol + nw + 1;   // apply grid: a block per column; line Gauss-Seidel: a partial per line
   if (pg < (size_t)RMAXB) pg = RMAXB;                            // line Gauss-Seidel final reduction
   size_t b = (size_t)RMAXB * 8 + pg;
   return a > b ? a : b;
@@ -1169,8 +1184,15 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     for (int c = 1; c < C; ++c) seq.push_back(c);
     for (int c = C - 2; c >= 0; --c) seq.push_back(c);
   }
+  // a colour's last visit writes the apply's outputs for its lines
+  std::vector<int> last(C, -1);
+  for (size_t q = 0; q < seq.size(); ++q) last[seq[q]] = (int)q;
+  const int nlines = h.gs_cstart[C];
+  double* partial = h.red.partial + RMAXB * 8;
+  unsigned* counter = h.red.counter + 1;
   for (size_t q = 0; q < seq.size(); ++q) {
     const int c = seq[q];
+    const int fin = last[c] == (int)q ? 1 : 0;
     g.first = q == 0 ? 1 : 0;
     const int cnt = h.gs_cstart[c + 1] - h.gs_cstart[c];
     if (cnt == 0) continue;
@@ -1183,10 +1205,12 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     }
     if (mode == 1)
       hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
-                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done);
+                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fin, nlines, N, partial,
+                         counter);
     else
       hipLaunchKernelGGL((k_gs_half<T, 0>), dim3(cnt), dim3(PW), lds0, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
-                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done);
+                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done, a, fin, nlines, N, partial,
+                         counter);
     if (sample && e0 && e1) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);
@@ -1195,8 +1219,6 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       if (h.prof_pending_bytes.size() >= 512) (void)prof_resolve(h);
     }
   }
-  hipLaunchKernelGGL(k_gs_final, dim3(rgrid(N)), dim3(RBLK), 0, s, N, a, h.gsu, h.red.partial + RMAXB * 8,
-                     h.red.counter + 1);
 }
 
 void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
