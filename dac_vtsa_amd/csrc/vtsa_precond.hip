@@ -793,6 +793,21 @@ __global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restr
 // The apply's outputs, on a line's last visit: out = U / post for the line's unknowns, and the
 // rms of out / red_s over all unknowns (the sum of squares per line into partial[L]; the last
 // line to finish sums the lines' partials in line order).  Returns this thread's sum of squares.
+// The scaled right-hand side b~ = (pre * scale * in) / s0 of unknown idx: computed on the line's
+// first visit (which also stores in * scale to store_in, and keeps b~ for the later visits), read
+// back afterwards.  store_in may alias in: each unknown is read and written by one thread, once.
+__device__ inline double gs_b(const ApplyArgs& a, const double* __restrict__ s0, double* __restrict__ bs, int idx,
+                              int first_visit) {
+  if (!first_visit) return bs[idx];
+  const double sc = a.in_scale ? *a.in_scale : 1.0;
+  double x = a.in[idx] * sc;
+  if (a.store_in) a.store_in[idx] = x;
+  if (a.pre) x = x * a.pre[idx];
+  const double b = x / s0[idx];
+  bs[idx] = b;
+  return b;
+}
+
 __device__ inline double gs_out(const ApplyArgs& a, int idx, double u) {
   const double o = a.post ? u / a.post[idx] : u;
   a.out[idx] = o;
@@ -827,8 +842,8 @@ __device__ inline void gs_out_norm(const ApplyArgs& a, double local, int L, int 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ Fac, GsArgs g, int cbeg,
-              const GmresScalars* __restrict__ gs, int respect_done, ApplyArgs a, int final_visit, int nlines,
-              int N, double* __restrict__ partial, unsigned* counter) {
+              const GmresScalars* __restrict__ gs, int respect_done, ApplyArgs a, int first_visit, int final_visit,
+              int nlines, int N, double* __restrict__ partial, unsigned* counter) {
   extern __shared__ __align__(16) double lds[];
   __shared__ double sh_norm[(MODE == 0 ? PW : GSW) / 64];
   __shared__ bool s_last;
@@ -886,7 +901,7 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
         acc += (wall && (r <= 2 || r == 5) ? jw : 0.0) * uw;
       }
       const int idx = r * n + X;
-      return g.bs[idx] + (dh / g.s0[idx]) * acc;
+      return gs_b(a, g.s0, g.bs, idx, first_visit) + (dh / g.s0[idx]) * acc;
     };
     for (int e = lane; e < 6 * Nz; e += NT) xs[(e % Nz) * 6 + e / Nz] = rhs(e);
     if (MODE == 1) {
@@ -953,7 +968,7 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
 #pragma unroll
         for (int v = 0; v < 6; ++v) acc += g.Jwc[W * 6 + v] * g.U[v * n + X];
       }
-      rhs[k] = g.bs[6 * n + W] + (dh / g.s0[6 * n + W]) * acc;
+      rhs[k] = gs_b(a, g.s0, g.bs, 6 * n + W, first_visit) + (dh / g.s0[6 * n + W]) * acc;
     }
     lds_barrier();
     if (lane < 64) {
@@ -1174,9 +1189,12 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   // whole-line staging when the option allows it and the device grants the LDS
   const int mode = (h.gs_full >= 1 && gs_mode_ok<T, 1>(h)) ? 1 : 0;
   const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds0 = gs_lds_bytes<T, 0>(Nz);
-  hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
   GsArgs g{h.Jo, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1, h.Tw};
   const int C = (int)h.gs_cstart.size() - 1;
+  // With two colours no line's first visit reads a line not yet visited, so the right-hand side
+  // is formed on each line's first visit; with more, a separate pass forms it and zeroes U first.
+  const bool fold = C <= 2;
+  if (!fold) hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
   std::vector<int> seq;
   for (int c = 0; c < C; ++c) seq.push_back(c);
   for (int c = C - 2; c >= 0; --c) seq.push_back(c);
@@ -1185,14 +1203,18 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     for (int c = C - 2; c >= 0; --c) seq.push_back(c);
   }
   // a colour's last visit writes the apply's outputs for its lines
-  std::vector<int> last(C, -1);
-  for (size_t q = 0; q < seq.size(); ++q) last[seq[q]] = (int)q;
+  std::vector<int> last(C, -1), first(C, -1);
+  for (size_t q = 0; q < seq.size(); ++q) {
+    last[seq[q]] = (int)q;
+    if (first[seq[q]] < 0) first[seq[q]] = (int)q;
+  }
   const int nlines = h.gs_cstart[C];
   double* partial = h.red.partial + RMAXB * 8;
   unsigned* counter = h.red.counter + 1;
   for (size_t q = 0; q < seq.size(); ++q) {
     const int c = seq[q];
     const int fin = last[c] == (int)q ? 1 : 0;
+    const int fv = fold && first[c] == (int)q ? 1 : 0;
     g.first = q == 0 ? 1 : 0;
     const int cnt = h.gs_cstart[c + 1] - h.gs_cstart[c];
     if (cnt == 0) continue;
@@ -1205,12 +1227,12 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     }
     if (mode == 1)
       hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
-                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fin, nlines, N, partial,
-                         counter);
+                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
+                         partial, counter);
     else
       hipLaunchKernelGGL((k_gs_half<T, 0>), dim3(cnt), dim3(PW), lds0, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
-                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done, a, fin, nlines, N, partial,
-                         counter);
+                         h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
+                         partial, counter);
     if (sample && e0 && e1) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);
