@@ -37,8 +37,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_gs_half launch on this workload, from two rocprofv3 counter passes over one bench
 # step (FETCH_SIZE x 1.9993 + WRITE_SIZE x 1.0000 KiB, the factors from a 64 MiB device copy):
-# 5616.9 x 1.9993 + 178.3 KiB.  Re-measure when the kernel changes.
-TRAFFIC_BYTES_PER_LAUNCH = (5616.9 * 1.9993 + 178.3) * 1024
+# 5643.4 x 1.9993 + 373.1 KiB.  Re-measure when the kernel changes.
+TRAFFIC_BYTES_PER_LAUNCH = (5643.4 * 1.9993 + 373.1) * 1024
 FP64_PEAK_TFLOPS = 78.6
 WORKLOAD = {"grid.Nr": 10, "grid.Ntheta": 16, "grid.Nz": 50, "operation.P_vacuum": 1.5e4}
 STEP_INDEX = 2          # heating in the default cycle

@@ -1116,14 +1116,19 @@ size_t partial_size(int ncol, int nw) {
 // written (6 each) and, unless it is the first
 // half-sweep (U = 0), the four off-line 6x6 J blocks and the neighbours' U; per outer-ring cell
 // Jcw (4) and the wall's U; per wall cell Jw (self, z +- 1), b~, s0, U written and, unless first,
-// Jw (theta +- 1), Jwc (6), the theta-neighbours' U (2) and the outer cell's U (6).
-static double gs_bytes(const Handle& h, int c, bool first, size_t fsz, int mode) {
+// Jw (theta +- 1), Jwc (6), the theta-neighbours' U (2) and the outer cell's U (6).  On a line's
+// first visit b~ is formed instead of read (in, pre, and store_in and b~ written); on its last
+// visit out (and post, red_s) per unknown as well.
+static double gs_bytes(const Handle& h, int c, bool first, size_t fsz, int mode, bool fv, bool fin,
+                       const ApplyArgs& a) {
   const double Nz = h.desc.Nz;
   const double cells = h.gs_ncell[c] * Nz, outer = h.gs_nouter[c] * Nz, walls = h.gs_nwall[c] * Nz;
   const double per_line = (mode == 1 ? (double)npairs(h.desc.Nz) * PP : Nz * 180.0) * fsz;
-  const double per_cell = 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24));
+  const double per_unknown = (fv ? 8.0 * (1 + (a.pre ? 1 : 0) + (a.store_in ? 1 : 0)) : 0.0) +
+                             (fin ? 8.0 * (1 + (a.post ? 1 : 0) + (a.red_s ? 1 : 0)) : 0.0);
+  const double per_cell = 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24)) + 6.0 * per_unknown;
   const double per_outer = first ? 0.0 : 8.0 * 5;
-  const double per_wall = 8.0 * 6 + (first ? 0.0 : 8.0 * 16);
+  const double per_wall = 8.0 * 6 + (first ? 0.0 : 8.0 * 16) + per_unknown;
   return h.gs_ncell[c] * per_line + cells * per_cell + outer * per_outer + walls * per_wall;
 }
 
@@ -1237,7 +1242,7 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);
       h.prof_pending.push_back(e1);
-      h.prof_pending_bytes.push_back(gs_bytes(h, c, g.first != 0, sizeof(T), mode));
+      h.prof_pending_bytes.push_back(gs_bytes(h, c, g.first != 0, sizeof(T), mode, fv != 0, fin != 0, a));
       if (h.prof_pending_bytes.size() >= 512) (void)prof_resolve(h);
     }
   }
