@@ -522,22 +522,22 @@ __device__ inline void stage_pairs(const T* __restrict__ P, int L, int NP, T* ds
 }
 
 // b - sum_{c<12} g_c v_c, v_c taken from lane c of the lane's 16-lane row (DPP row_newbcast), as
-// ((b - g0 v0 - g1 v1 - g2 v2) + (-g3 v3 - g4 v4 - g5 v5)) + ((-g6 v6 - ..) + (-g9 v9 - ..)),
-// four interleaved chains.
+// (b - g0 v0 - g1 v1 - .. - g5 v5) + (-g6 v6 - .. - g11 v11): two interleaved chains, so no FMA
+// waits for the one before it and a single add joins them.
 #define VTSA_FMAC_BC(acc, v, l, c) "v_fmac_f64_dpp " acc ", " v ", " l " row_newbcast:" #c " row_mask:0xf bank_mask:0xf\n\t"
 __device__ inline double bsub_pair(double b, double v, const double (&g)[12]) {
-  double a2 = 0.0, c1 = 0.0, c2 = 0.0;
+  double a2 = 0.0;
   asm("s_nop 1\n\t"
-      VTSA_FMAC_BC("%0", "-%4", "%5", 0) VTSA_FMAC_BC("%1", "-%4", "%8", 3)
-      VTSA_FMAC_BC("%2", "-%4", "%11", 6) VTSA_FMAC_BC("%3", "-%4", "%14", 9)
-      VTSA_FMAC_BC("%0", "-%4", "%6", 1) VTSA_FMAC_BC("%1", "-%4", "%9", 4)
-      VTSA_FMAC_BC("%2", "-%4", "%12", 7) VTSA_FMAC_BC("%3", "-%4", "%15", 10)
-      VTSA_FMAC_BC("%0", "-%4", "%7", 2) VTSA_FMAC_BC("%1", "-%4", "%10", 5)
-      VTSA_FMAC_BC("%2", "-%4", "%13", 8) VTSA_FMAC_BC("%3", "-%4", "%16", 11)
-      : "+v"(b), "+v"(a2), "+v"(c1), "+v"(c2)
+      VTSA_FMAC_BC("%0", "-%2", "%3", 0) VTSA_FMAC_BC("%1", "-%2", "%9", 6)
+      VTSA_FMAC_BC("%0", "-%2", "%4", 1) VTSA_FMAC_BC("%1", "-%2", "%10", 7)
+      VTSA_FMAC_BC("%0", "-%2", "%5", 2) VTSA_FMAC_BC("%1", "-%2", "%11", 8)
+      VTSA_FMAC_BC("%0", "-%2", "%6", 3) VTSA_FMAC_BC("%1", "-%2", "%12", 9)
+      VTSA_FMAC_BC("%0", "-%2", "%7", 4) VTSA_FMAC_BC("%1", "-%2", "%13", 10)
+      VTSA_FMAC_BC("%0", "-%2", "%8", 5) VTSA_FMAC_BC("%1", "-%2", "%14", 11)
+      : "+v"(b), "+v"(a2)
       : "v"(v), "v"(g[0]), "v"(g[1]), "v"(g[2]), "v"(g[3]), "v"(g[4]), "v"(g[5]), "v"(g[6]), "v"(g[7]),
         "v"(g[8]), "v"(g[9]), "v"(g[10]), "v"(g[11]));
-  return (b + a2) + (c1 + c2);
+  return b + a2;
 }
 #undef VTSA_FMAC_BC
 
