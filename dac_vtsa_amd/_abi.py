@@ -15,6 +15,7 @@ D = C.c_double
 I32 = C.c_int32
 P = C.c_void_p
 SHIFT_NO_REPORT = 2   # vtsa_cons_shift: VTSA_SHIFT_NO_REPORT (include/vtsa.h)
+SHIFT_TRUNCATE_AT_ZERO = 3   # vtsa_cons_shift: VTSA_SHIFT_TRUNCATE_AT_ZERO
 
 
 class ModelDesc(C.Structure):

@@ -86,9 +86,9 @@ def run_case(case: Dict, mode: str = "cycle", state_kind: str = "small", base_ov
     mb, eb, _ = check_cycle(steps, props.species)
     out["max_mass_rel_err"] = max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE")
     out["max_energy_rel_err"] = max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE")
-    # steps run again with the upstream polish after a failed projection run, and stages the
-    # polish left unclosed (both 0 when every stage closed by projection)
-    out["projection_fallback"] = sum(int(s.solver.get("projection_fallback", 0) or 0) for s in steps)
+    # how far a step fell back after a failed projection run (0 none, 1 the projection kept at or
+    # above zero, 2 the upstream polish), and stages the polish left unclosed
+    out["projection_fallback"] = max(int(s.solver.get("projection_fallback", 0) or 0) for s in steps)
     out["balance_unclosed"] = sum(int(s.solver.get("balance_unclosed", 0) or 0) for s in steps)
     if mode == "cycle":
         k = cycle_kpis(steps, cfg, props.species)

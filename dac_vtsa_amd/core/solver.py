@@ -135,10 +135,16 @@ class DeviceOps:
 
     def cons_shift(self, z, s, lam, out, truncate=False, report=True):
         """out = z - S^2 W^T lam.  Returns True when a concentration or loading would fall more
-        than one tolerance unit below zero (with truncate=True those keep 0.1 of their value).
-        report=False: the plain shift, not waited for; returns False."""
+        than one tolerance unit below zero (with truncate=True those keep 0.1 of their value;
+        truncate="zero": every one that would fall below zero does).  report=False: the plain
+        shift, not waited for; returns False."""
         lam = (C.c_double * 4)(*[float(v) for v in lam])
-        mode = int(bool(truncate)) if report else _abi.SHIFT_NO_REPORT
+        if not report:
+            mode = _abi.SHIFT_NO_REPORT
+        elif truncate == "zero":
+            mode = _abi.SHIFT_TRUNCATE_AT_ZERO
+        else:
+            mode = int(bool(truncate))
         rc = _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, mode, dv.ptr(out), self.st())
         return rc == 1
 
@@ -171,7 +177,7 @@ class Factorisation:
         self.h = h
 
 
-def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, max_shift=1.0):
+def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, max_shift=1.0, strict=False):
     """Shift a converged stage value along S^2 W^T so that its residual conserves the four totals.
 
     No upstream counterpart (``conserve="project"``).  The shift z(lambda) = z - S^2 W^T lambda is
@@ -183,7 +189,8 @@ def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, m
     value instead (the integrator's truncation rule).  The projection is
     accepted only when the totals close and the shift stays within max_shift tolerance units
     (rms of the change over s); otherwise None is returned and the caller falls back to the
-    upstream polish."""
+    upstream polish.  strict: no shifted concentration or loading may go below zero (those keep 0.1
+    of their value), at some cost in closure."""
     d2 = ops.cons_norm2(s)
     rows = np.nonzero(d2 > 0.0)[0]
     zc, fc, Rc = z, f, R
@@ -209,8 +216,10 @@ def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, m
         except np.linalg.LinAlgError:
             return None
         zn, fn, Rn = dv.empty(ops.N), dv.empty(ops.N), dv.empty(ops.N)
-        if ops.cons_shift(zc, s, lam, zn):        # a concentration or loading would go negative:
-            ops.cons_shift(zc, s, lam, zn, True)  # truncate them as a Newton update does
+        if strict:
+            ops.cons_shift(zc, s, lam, zn, "zero")
+        elif ops.cons_shift(zc, s, lam, zn):        # a concentration or loading would go negative:
+            ops.cons_shift(zc, s, lam, zn, True)    # truncate them as a Newton update does
         rn = ops.newton_residual(t, zn, dh, rhs_known, s, fn, Rn)
         if not _finite(rn):
             return None
@@ -227,8 +236,19 @@ def _shift(ops, z0, z1, s, tmp):
     return ops.rms(tmp, s)
 
 
+def _unclosed(stats, z0, best, it, rate, new_lu, reject):
+    """A converged stage whose balance stayed open: accepted as upstream (balance_unclosed), or,
+    with reject, reported as a failed solve so that the step is retried with a smaller h
+    (unclosed_rejected; no upstream counterpart)."""
+    key = "unclosed_rejected" if reject else "balance_unclosed"
+    if stats is not None:
+        stats[key] = stats.get(key, 0) + 1
+    return (z0, False, it + 1, rate, new_lu) if reject else (best[1], True, it + 1, rate, new_lu)
+
+
 def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_dim=30, eta=1e-2, counter=None,
-                  cons_tol=0.0, stats=None, trace=None, refresh=None, project=False):
+                  cons_tol=0.0, stats=None, trace=None, refresh=None, project=False, strict=False,
+                  reject_unclosed=False):
     """Inexact Newton-Krylov stage solve on the device (core/solver.py:154-294).
 
     ``project`` (no upstream counterpart): when the converged value misses the conservation
@@ -301,7 +321,7 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
             if cons_ok(ops, cons_tol, R, ops.scale(z, rtol, s)):
                 return z, True, it + 1, rate, new_lu
             if project and cons_tol > 0.0:
-                pr = project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol)
+                pr = project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, strict=strict)
                 if pr is not None:
                     if stats is not None:
                         stats["cons_projected"] = stats.get("cons_projected", 0) + 1
@@ -310,13 +330,9 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
                 polish += 1
                 step_prev = None
                 continue
-            if stats is not None:
-                stats["balance_unclosed"] = stats.get("balance_unclosed", 0) + 1
-            return best[1], True, it + 1, rate, new_lu
+            return _unclosed(stats, z0, best, it, rate, new_lu, reject_unclosed)
         if best is not None:
-            if stats is not None:
-                stats["balance_unclosed"] = stats.get("balance_unclosed", 0) + 1
-            return best[1], True, it + 1, rate, new_lu
+            return _unclosed(stats, z0, best, it, rate, new_lu, reject_unclosed)
         if step_prev is not None and lam == 1.0 and info.info == 0:
             left = max_iter - it - 1
             if it >= 2 and (rate >= 1.0 or rate ** left / (1.0 - rate) * step > kappa * 1e3):
@@ -331,7 +347,9 @@ class TRBDF2:
     Extra keyword arguments (no upstream counterpart): ``precond`` ("zline_gs" | "zline" |
     "block_jacobi"), ``gs_sweeps`` (symmetric line Gauss-Seidel sweeps of "zline_gs"),
     ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``, ``conserve``
-    ("polish": the upstream conservation polish; "project": project_conservation first),
+    ("polish": the upstream conservation polish; "project": project_conservation first;
+    "project_strict": the same with no shifted entry below zero; with either projection a stage
+    whose balance stays open fails its step, which is retried with a smaller h),
     ``guess`` (a learned.StageNet: each stage starts from the network's guess when its
     preconditioned residual is below the predictor's) and ``record`` (a learned.StageDataset
     that receives the features and converged corrections of every solved stage).  The
@@ -369,7 +387,7 @@ class TRBDF2:
         self.precision = PRECISIONS[precision]
         self.krylov_dim = krylov_dim
         self.gs_sweeps = int(gs_sweeps)
-        if conserve not in ("polish", "project"):
+        if conserve not in ("polish", "project", "project_strict"):
             raise ValueError(f"conserve={conserve!r}")
         self.conserve = conserve
         self.cap_after_fail = 0.5
@@ -446,7 +464,9 @@ class TRBDF2:
                                                 self.kappa, krylov_dim=self.krylov_dim, counter=self._gm,
                                                 cons_tol=self.cons_tol, stats=self.stats,
                                                 refresh=refresh if (self.newton_refresh and not self._fresh) else None,
-                                                project=self.conserve == "project")
+                                                project=self.conserve.startswith("project"),
+                                                strict=self.conserve == "project_strict",
+                                                reject_unclosed=self.conserve.startswith("project"))
         if new_lu is not None:
             self._new_lu = new_lu
         if ok and self._recording:

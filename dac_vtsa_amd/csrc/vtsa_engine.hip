@@ -339,7 +339,8 @@ int vtsa_cons_norm2(vtsa_handle* H, const double* s, double* out, vtsa_stream st
 int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, int32_t truncate,
                     double* out, vtsa_stream st) {
   if (!H || !z || !s || !lambda || !out) { set_error("vtsa_cons_shift: null argument"); return -1; }
-  const int mode = truncate == VTSA_SHIFT_NO_REPORT ? VTSA_SHIFT_NO_REPORT : (truncate ? 1 : 0);
+  const int mode = (truncate == VTSA_SHIFT_NO_REPORT || truncate == VTSA_SHIFT_TRUNCATE_AT_ZERO) ? truncate
+                                                                                               : (truncate ? 1 : 0);
   const int rc = cons_shift_impl(H->h, z, s, lambda, mode, out, S(st));
   if (rc < 0) return rc;
   const int e = finish(S(st));

@@ -173,8 +173,8 @@ __global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
 
 // out = z - S^2 W^T lambda: each entry moved along its conserved row k (gas species, adsorbed
 // species into the same row, energy of bed and wall) by s_i^2 W_ki lambda_k.  *flag is set
-// when a non-negative c or q entry would fall below -s; with truncate such entries keep 0.1 of
-// their value.
+// when a non-negative c or q entry would fall below -s (below 0 with truncate = 3); with truncate
+// (1 or 3) such entries keep 0.1 of their value.
 __global__ void k_cons_shift(Model m, const double* z, const double* __restrict__ s, double l0, double l1,
                              double l2, double l3, double* out,   // out may alias z (elementwise)
                              int truncate, unsigned* flag) {
@@ -196,9 +196,10 @@ __global__ void k_cons_shift(Model m, const double* z, const double* __restrict_
   const double zo = z[i];
   const double si = s[i];
   double v = zo - (si * si) * w * lam;
-  // c and q: an excursion below zero by more than the tolerance scale s is flagged; truncated,
-  // such entries keep at least 0.1 of their value, as a Newton update (core/solver.py:99-108)
-  if (i < 5 * n && zo >= 0.0 && v < -si) {
+  // c and q: an excursion below zero by more than the tolerance scale s (any, with truncate = 3)
+  // is flagged; truncated, such entries keep at least 0.1 of their value, as a Newton update
+  // (core/solver.py:99-108)
+  if (i < 5 * n && zo >= 0.0 && v < (truncate == 3 ? 0.0 : -si)) {
     if (truncate) v = 0.1 * zo;
     atomicOr(flag, 1u);
   }

@@ -122,17 +122,28 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
         if kw.get("conserve") != "project":
             raise
         # the conservation projection can leave states (slightly negative concentrations) the
-        # step controller cannot follow; the step is run again with the upstream polish
-        log(f"    {step['name']}: {e}; running the step again with the conservation polish")
-        outlet.clear()
-        counter[0] = 0
-        if record_outlet:
-            record(0.0, y0, np.zeros(model.n_acc))
-        kw["conserve"] = "polish"
-        integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **kw)
-        res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
-                              stop_direction=direction)
-        integ.stats["projection_fallback"] = 1
+        # step controller cannot follow; the step is run again with the projection kept at or
+        # above zero, and if that fails too, with the upstream polish
+        fallback = 0
+        for level, mode in ((1, "project_strict"), (2, "polish")):
+            log(f"    {step['name']}: {e}; running the step again with conserve={mode}")
+            outlet.clear()
+            counter[0] = 0
+            if record_outlet:
+                record(0.0, y0, np.zeros(model.n_acc))
+            kw["conserve"] = mode
+            integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **kw)
+            try:
+                res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
+                                      stop_direction=direction)
+            except SolverError as e2:
+                if level == 2:
+                    raise
+                e = e2
+                continue
+            fallback = level
+            break
+        integ.stats["projection_fallback"] = fallback
     solver_stats = {"method": "trbdf2", "steps": res.n_steps, "rejected": res.n_reject,
                     "newton_fail": res.n_newton_fail, "krylov": res.n_krylov, "jacobians": res.n_jac,
                     "lu": res.n_lu, "nfev": res.nfev, "groups": "all-columns", **integ.stats}

@@ -162,9 +162,10 @@ int vtsa_cons_norm2(vtsa_handle* h, const double* s, double* out, vtsa_stream st
  * variables that changes the conserved totals W z by -W S^2 W^T lambda.  out may alias z.
  * Returns 1 when a non-negative concentration or loading would fall below -s (one tolerance
  * unit under zero); with truncate = 1 such entries keep 0.1 of their value instead, as a Newton
- * update does (core/solver.py:99-108).  Synchronises the stream, except with truncate =
- * VTSA_SHIFT_NO_REPORT: plain shift, returns 0 without waiting. */
-enum { VTSA_SHIFT_NO_REPORT = 2 };
+ * update does (core/solver.py:99-108); with truncate = VTSA_SHIFT_TRUNCATE_AT_ZERO every entry
+ * that would fall below zero does (and is reported).  Synchronises the stream, except with
+ * truncate = VTSA_SHIFT_NO_REPORT: plain shift, returns 0 without waiting. */
+enum { VTSA_SHIFT_NO_REPORT = 2, VTSA_SHIFT_TRUNCATE_AT_ZERO = 3 };
 int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, int32_t truncate,
                     double* out, vtsa_stream st);
 /* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall

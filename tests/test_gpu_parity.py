@@ -471,10 +471,13 @@ def test_projection_run_completes_small_heating_step():
     assert m_err <= gate and e_err <= gate
 
 
-def test_projection_failure_reruns_the_step_with_the_polish(monkeypatch):
+@pytest.mark.parametrize("failures,modes,level", [(1, ["project", "project_strict"], 1),
+                                                  (2, ["project", "project_strict", "polish"], 2)])
+def test_projection_failure_reruns_the_step(monkeypatch, failures, modes, level):
     """run_step's recovery path: a SolverError from the run with the conservation projection is
-    answered by running the step again with the upstream polish (flag projection_fallback).  The
-    error is raised from the host on the first integrate call; 200 s of the 4x4x10 heating step."""
+    answered by running the step again with the projection kept at or above zero, and a second
+    one with the upstream polish (flag projection_fallback = 1 or 2).  The errors are raised from
+    the host on the first integrate calls; 200 s of the 4x4x10 heating step."""
     from dac_vtsa_amd.core import solver as S
     from dac_vtsa_amd.core.grid import Grid
     from dac_vtsa_amd.properties import get_backend
@@ -485,7 +488,7 @@ def test_projection_failure_reruns_the_step_with_the_polish(monkeypatch):
 
     def integrate(self, *a, **k):
         seen.append(self.conserve)
-        if len(seen) == 1:
+        if len(seen) <= failures:
             raise S.SolverError("step size underflow (forced)")
         return orig(self, *a, **k)
 
@@ -494,8 +497,8 @@ def test_projection_failure_reruns_the_step_with_the_polish(monkeypatch):
     g, props = Grid.from_config(cfg), get_backend(cfg)
     step = {**cfg.cycle[2], "duration": 200.0}
     st, sr = run_step(seeded_state(g, "midcycle", 0), cfg, g, props, step, integrator_kw={"conserve": "project"})
-    assert seen == ["project", "polish"]
-    assert sr.solver.get("projection_fallback") == 1
+    assert seen == modes
+    assert sr.solver.get("projection_fallback") == level
     assert sr.duration == 200.0
     assert np.all(np.isfinite(st.c)) and np.all(np.isfinite(st.T))
 
