@@ -478,15 +478,18 @@ __global__ void __launch_bounds__(BLK) k_fd_wall(Model m, double t, const double
 
 // ------------------------------------------------------------------ launchers
 static inline int nblk(long long work, int b = BLK) { return (int)((work + b - 1) / b); }
+// The residual passes have one thread per cell or face and too few cells to fill the device
+// (8,000 at 10x16x50): one-wavefront blocks spread them over four times as many CUs.
+constexpr int RB = 64;
 
 void launch_props(Handle& h, const double* y, double* pb, double* f, double* qstar, hipStream_t s) {
-  hipLaunchKernelGGL(k_props, dim3(nblk(h.n)), dim3(BLK), 0, s, h.m, y, pb, f, qstar);
+  hipLaunchKernelGGL(k_props, dim3(nblk(h.n, RB)), dim3(RB), 0, s, h.m, y, pb, f, qstar);
 }
 
 void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s) {
   const int nf = FaceTable::of(h.desc).count();
-  hipLaunchKernelGGL(k_faces, dim3(nblk(nf)), dim3(BLK), 0, s, h.m, t, y, pb, h.FF, h.gs, 0);
-  hipLaunchKernelGGL(k_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, s, h.m, t, y, pb, h.FF, f);
+  hipLaunchKernelGGL(k_faces, dim3(nblk(nf, RB)), dim3(RB), 0, s, h.m, t, y, pb, h.FF, h.gs, 0);
+  hipLaunchKernelGGL(k_flux, dim3(nblk(h.n + h.nw, RB)), dim3(RB), 0, s, h.m, t, y, pb, h.FF, f);
 }
 
 void launch_acc(Handle& h, double t, const double* y, const double* pb, double* acc, hipStream_t s) {
@@ -520,11 +523,11 @@ void launch_equilibrium(Handle& h, int count, const double* pc, const double* pw
 void launch_matvec_residual(Handle& h, double t, const double* z, const double* u, const double* s,
                             const double* f, double dh, double* w, bool respect_done, hipStream_t st) {
   const int rd = respect_done ? 1 : 0;
-  hipLaunchKernelGGL(k_mv_props, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, z, u, s, f, h.gs, rd, dh, h.ytmp,
+  hipLaunchKernelGGL(k_mv_props, dim3(nblk(h.n + h.nw, RB)), dim3(RB), 0, st, h.m, z, u, s, f, h.gs, rd, dh, h.ytmp,
                      h.pb2, h.ftmp, w);
   const int nf = FaceTable::of(h.desc).count();
-  hipLaunchKernelGGL(k_faces, dim3(nblk(nf)), dim3(BLK), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.gs, rd);
-  hipLaunchKernelGGL(k_mv_flux, dim3(nblk(h.n + h.nw)), dim3(BLK), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.ftmp, u,
+  hipLaunchKernelGGL(k_faces, dim3(nblk(nf, RB)), dim3(RB), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.gs, rd);
+  hipLaunchKernelGGL(k_mv_flux, dim3(nblk(h.n + h.nw, RB)), dim3(RB), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.ftmp, u,
                      s, f, h.gs, rd, dh, w);
 }
 
