@@ -14,8 +14,8 @@ LIB_PATH = os.path.join(HERE, "libvtsa.so")
 D = C.c_double
 I32 = C.c_int32
 P = C.c_void_p
-SHIFT_NO_REPORT = 2   # vtsa_cons_shift: VTSA_SHIFT_NO_REPORT (include/vtsa.h)
-SHIFT_TRUNCATE_AT_ZERO = 3   # vtsa_cons_shift: VTSA_SHIFT_TRUNCATE_AT_ZERO
+# vtsa_cons_shift mode bits (include/vtsa.h)
+SHIFT_TRUNCATE, SHIFT_AT_ZERO, SHIFT_NO_REPORT = 1, 2, 4
 
 
 class ModelDesc(C.Structure):
@@ -87,6 +87,8 @@ _SIGS = {
     "vtsa_inventories": [P, P, P, P],
     "vtsa_cons_norm2": [P, P, P, P],
     "vtsa_cons_shift": [P, P, P, P, I32, P, P],
+    "vtsa_cons_stash": [P, P, P, I32, P],
+    "vtsa_cons_fetch": [P, I32, P, P, P],
     "vtsa_vec_op": [P, I32, D, D, D, D, P, P, P, P, P, P],
     "vtsa_error_norm": [P, P, P, P, D, C.POINTER(D), P],
     "vtsa_device_count": [C.POINTER(I32)],

@@ -128,6 +128,16 @@ class DeviceOps:
         _abi.call("vtsa_cons", self.h, dv.ptr(R), dv.ptr(s), wr, ws, self.st())
         return np.array(wr[:]), np.array(ws[:])
 
+    def cons_stash(self, R, s, slot):
+        """W R and W s as cons, kept on the device in slot 0..3 (not waited for)."""
+        _abi.call("vtsa_cons_stash", self.h, dv.ptr(R), dv.ptr(s), int(slot), self.st())
+
+    def cons_fetch(self, n):
+        """The stashed (W R, W s) of slots 0..n-1, as two (n, 4) arrays."""
+        wr, ws = (C.c_double * (4 * n))(), (C.c_double * (4 * n))()
+        _abi.call("vtsa_cons_fetch", self.h, int(n), wr, ws, self.st())
+        return np.array(wr[:]).reshape(n, 4), np.array(ws[:]).reshape(n, 4)
+
     def cons_norm2(self, s):
         out = (C.c_double * 4)()
         _abi.call("vtsa_cons_norm2", self.h, dv.ptr(s), out, self.st())
@@ -139,12 +149,8 @@ class DeviceOps:
         truncate="zero": every one that would fall below zero does).  report=False: the plain
         shift, not waited for; returns False."""
         lam = (C.c_double * 4)(*[float(v) for v in lam])
-        if not report:
-            mode = _abi.SHIFT_NO_REPORT
-        elif truncate == "zero":
-            mode = _abi.SHIFT_TRUNCATE_AT_ZERO
-        else:
-            mode = int(bool(truncate))
+        mode = (_abi.SHIFT_TRUNCATE if truncate else 0) | (_abi.SHIFT_AT_ZERO if truncate == "zero" else 0) | \
+            (0 if report else _abi.SHIFT_NO_REPORT)
         rc = _abi.call("vtsa_cons_shift", self.h, dv.ptr(z), dv.ptr(s), lam, mode, dv.ptr(out), self.st())
         return rc == 1
 
@@ -200,26 +206,27 @@ def project_conservation(ops, t, z, f, R, s, dh, rhs_known, cons_tol, iters=3, m
         if np.all(np.abs(wr) <= cons_tol * ws):
             return zc, fc, Rc
         B = np.zeros((4, 4))
-        for l in rows:
-            eps = 1e-2 * ws[l] / d2[l]
+        eps = 1e-2 * ws / np.where(d2 > 0.0, d2, 1.0)
+        for q, l in enumerate(rows):     # the columns' residuals queued back to back, one wait
             lam = np.zeros(4)
-            lam[l] = eps
+            lam[l] = eps[l]
             ops.cons_shift(zc, s, lam, zt, report=False)
             ops.newton_residual(t, zt, dh, rhs_known, s, ft, Rt, norm=False)
-            wl, _ = ops.cons(Rt, s)
-            if not np.all(np.isfinite(wl)):
-                return None
-            B[:, l] = (wl - wr) / eps
+            ops.cons_stash(Rt, s, q)
+        wls, _ = ops.cons_fetch(len(rows))
+        if not np.all(np.isfinite(wls)):
+            return None
+        for q, l in enumerate(rows):
+            B[:, l] = (wls[q] - wr) / eps[l]
         lam = np.zeros(4)
         try:
             lam[rows] = np.linalg.solve(B[np.ix_(rows, rows)], -wr[rows])
         except np.linalg.LinAlgError:
             return None
         zn, fn, Rn = dv.empty(ops.N), dv.empty(ops.N), dv.empty(ops.N)
-        if strict:
-            ops.cons_shift(zc, s, lam, zn, "zero")
-        elif ops.cons_shift(zc, s, lam, zn):        # a concentration or loading would go negative:
-            ops.cons_shift(zc, s, lam, zn, True)    # truncate them as a Newton update does
+        # entries that would go negative are truncated as a Newton update does (an entry that
+        # would not is left as the plain shift leaves it, so no report is needed)
+        ops.cons_shift(zc, s, lam, zn, "zero" if strict else True, report=False)
         rn = ops.newton_residual(t, zn, dh, rhs_known, s, fn, Rn)
         if not _finite(rn):
             return None

@@ -336,15 +336,34 @@ int vtsa_cons_norm2(vtsa_handle* H, const double* s, double* out, vtsa_stream st
   return reduce_weighted(H->h, s, nullptr, 0, 2, out, S(st));
 }
 
-int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, int32_t truncate,
+int vtsa_cons_shift(vtsa_handle* H, const double* z, const double* s, const double* lambda, int32_t mode,
                     double* out, vtsa_stream st) {
   if (!H || !z || !s || !lambda || !out) { set_error("vtsa_cons_shift: null argument"); return -1; }
-  const int mode = (truncate == VTSA_SHIFT_NO_REPORT || truncate == VTSA_SHIFT_TRUNCATE_AT_ZERO) ? truncate
-                                                                                               : (truncate ? 1 : 0);
+  if (mode & ~(VTSA_SHIFT_TRUNCATE | VTSA_SHIFT_AT_ZERO | VTSA_SHIFT_NO_REPORT)) {
+    set_error("vtsa_cons_shift: unknown mode bits");
+    return -2;
+  }
   const int rc = cons_shift_impl(H->h, z, s, lambda, mode, out, S(st));
   if (rc < 0) return rc;
   const int e = finish(S(st));
   return e != 0 ? e : rc;
+}
+
+int vtsa_cons_stash(vtsa_handle* H, const double* R, const double* s, int32_t slot, vtsa_stream st) {
+  if (!H || !R || !s) { set_error("vtsa_cons_stash: null argument"); return -1; }
+  if (slot < 0 || slot >= 4) { set_error("vtsa_cons_stash: slot out of range"); return -2; }
+  cons_stash_impl(H->h, R, s, slot, S(st));
+  return finish(S(st));
+}
+
+int vtsa_cons_fetch(vtsa_handle* H, int32_t n, double* wr, double* ws, vtsa_stream st) {
+  if (!H || !wr || !ws) { set_error("vtsa_cons_fetch: null argument"); return -1; }
+  if (n < 0 || n > 4) { set_error("vtsa_cons_fetch: slot count out of range"); return -2; }
+  double out[32];
+  if (cons_fetch_impl(H->h, n, out, S(st))) return -1;
+  for (int q = 0; q < n; ++q)
+    for (int k = 0; k < 4; ++k) { wr[4 * q + k] = out[8 * q + k]; ws[4 * q + k] = out[8 * q + 4 + k]; }
+  return 0;
 }
 
 int vtsa_inventories(vtsa_handle* H, const double* y, double* out, vtsa_stream st) {

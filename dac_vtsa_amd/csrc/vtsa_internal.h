@@ -110,6 +110,8 @@ void gs_colour_lines(int Nr, int Nth, int nwall_lines, std::vector<int>& order, 
 int prof_resolve(Handle& h);
 // returns 1 when a non-negative c or q entry would fall below -s; truncate = 1: such entries keep
 // 0.1 of their value
+void cons_stash_impl(Handle& h, const double* R, const double* s, int slot, hipStream_t st);
+int cons_fetch_impl(Handle& h, int n, double* out, hipStream_t st);
 int cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], int truncate, double* out,
                     hipStream_t st);
 // out = post^-1 * M^-1 (pre * in * (in_scale ? *in_scale : 1)); optionally stores the scaled

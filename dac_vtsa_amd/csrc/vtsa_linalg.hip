@@ -543,11 +543,29 @@ int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mo
   return 0;
 }
 
-int cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], int truncate, double* out,
+// W R, W s of mode 0 into device slot q (result[16 + 8 q ..]); fetched together by cons_fetch_impl.
+void cons_stash_impl(Handle& h, const double* R, const double* s, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_weighted, dim3(rgrid(h.n + h.nw)), dim3(RBLK), 0, st, h.m, 0, R, s, h.red.partial,
+                     h.red.counter + 3, h.red.result + 16 + 8 * slot);
+}
+
+int cons_fetch_impl(Handle& h, int n, double* out, hipStream_t st) {
+  if (n <= 0) return 0;
+  if (check(hipMemcpyAsync(h.host_scalars + 16, h.red.result + 16, sizeof(double) * 8 * n, hipMemcpyDeviceToHost,
+                           st), "memcpy"))
+    return -1;
+  if (check(hipStreamSynchronize(st), "sync")) return -1;
+  for (int q = 0; q < 8 * n; ++q) out[q] = h.host_scalars[16 + q];
+  return 0;
+}
+
+int cons_shift_impl(Handle& h, const double* z, const double* s, const double lam[4], int mode, double* out,
                     hipStream_t st) {
-  if (truncate == VTSA_SHIFT_NO_REPORT) {   // plain shift; the flag is written but not read
+  // kernel truncate argument: 0 none, 1 below -s, 3 below 0
+  const int truncate = (mode & VTSA_SHIFT_TRUNCATE) ? ((mode & VTSA_SHIFT_AT_ZERO) ? 3 : 1) : 0;
+  if (mode & VTSA_SHIFT_NO_REPORT) {   // the flag is written but not read
     hipLaunchKernelGGL(k_cons_shift, dim3(eblk(h.N)), dim3(256), 0, st, h.m, z, s, lam[0], lam[1], lam[2], lam[3],
-                       out, 0, h.red.counter + 12);
+                       out, truncate, h.red.counter + 12);
     return 0;
   }
   unsigned* flag = h.red.counter + 12;

@@ -160,14 +160,18 @@ int vtsa_cons(vtsa_handle* h, const double* R, const double* s, double* wr, doub
 int vtsa_cons_norm2(vtsa_handle* h, const double* s, double* out, vtsa_stream st);
 /* out = z - S^2 W^T lambda (lambda: 4 host values), the minimum-norm shift in the s-scaled
  * variables that changes the conserved totals W z by -W S^2 W^T lambda.  out may alias z.
- * Returns 1 when a non-negative concentration or loading would fall below -s (one tolerance
- * unit under zero); with truncate = 1 such entries keep 0.1 of their value instead, as a Newton
- * update does (core/solver.py:99-108); with truncate = VTSA_SHIFT_TRUNCATE_AT_ZERO every entry
- * that would fall below zero does (and is reported).  Synchronises the stream, except with
- * truncate = VTSA_SHIFT_NO_REPORT: plain shift, returns 0 without waiting. */
-enum { VTSA_SHIFT_NO_REPORT = 2, VTSA_SHIFT_TRUNCATE_AT_ZERO = 3 };
-int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, int32_t truncate,
+ * mode bits: VTSA_SHIFT_TRUNCATE: a non-negative concentration or loading that would fall below
+ * -s (one tolerance unit under zero) keeps 0.1 of its value instead, as a Newton update does
+ * (core/solver.py:99-108); with VTSA_SHIFT_AT_ZERO as well, any that would fall below zero.
+ * Returns 1 when such an entry exists (truncated or not) and synchronises the stream, except with
+ * VTSA_SHIFT_NO_REPORT: returns 0 without waiting. */
+enum { VTSA_SHIFT_TRUNCATE = 1, VTSA_SHIFT_AT_ZERO = 2, VTSA_SHIFT_NO_REPORT = 4 };
+int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const double* lambda, int32_t mode,
                     double* out, vtsa_stream st);
+/* As vtsa_cons, with W R and W s kept on the device in slot 0..3 (nothing is waited for);
+ * vtsa_cons_fetch returns slots 0..n-1 on the host, wr[4 n] and ws[4 n] (synchronises). */
+int vtsa_cons_stash(vtsa_handle* h, const double* R, const double* s, int32_t slot, vtsa_stream st);
+int vtsa_cons_fetch(vtsa_handle* h, int32_t n, double* wr, double* ws, vtsa_stream st);
 /* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall
  * (core/residual.py:426-443) */
 int vtsa_inventories(vtsa_handle* h, const double* y, double* out, vtsa_stream st);
