@@ -449,8 +449,9 @@ def test_projection_run_completes_small_heating_step():
     """The 4x4x20 heating step from the seeded mid-cycle state (seed 1) with conserve="project"
     completes: either the projection closes its stages, or (as in some roundings of the line
     solves, where projections leave slightly negative concentrations and the step size
-    underflows near t = 938 s) run_step runs the step again with the upstream polish.  The polish
-    leaves stages unclosed here (the reference leaves 100 of 260), so its balance gate is loose."""
+    underflows near t = 938 s) run_step runs the step again with the shift kept at or above zero,
+    then with the upstream polish.  Those close less tightly (the polish leaves stages unclosed
+    here; the reference leaves 100 of 260), so their balance gates are looser."""
     from dac_vtsa_amd.core.grid import Grid
     from dac_vtsa_amd.properties import get_backend
     from dac_vtsa_amd.seeds import seeded_state
@@ -462,12 +463,12 @@ def test_projection_run_completes_small_heating_step():
     st, sr = run_step(seeded_state(g, "midcycle", 1), cfg, g, props, cfg.cycle[2],
                       integrator_kw={"conserve": "project"})
     assert sr.duration == 22400.0
-    fell_back = sr.solver.get("projection_fallback") == 1
+    level = int(sr.solver.get("projection_fallback", 0) or 0)   # 0 none, 1 shift kept >= 0, 2 polish
     mb, eb, _ = balances.check_cycle([sr], props.species)
     m_err = max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE")
     e_err = max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE")
-    print("fallback:", fell_back, "balances:", m_err, e_err)
-    gate = 1e-3 if fell_back else 1e-6
+    print("fallback level:", level, "balances:", m_err, e_err)
+    gate = {0: 1e-6, 1: 1e-4, 2: 1e-3}[level]
     assert m_err <= gate and e_err <= gate
 
 
