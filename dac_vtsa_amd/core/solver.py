@@ -113,6 +113,8 @@ class DeviceOps:
         return rn.value, pn.value, step.value
 
     def gmres(self, t, z, f, s, dh, zs_norm, b, w, rtol, atol, restart):
+        """One GMRES cycle; zs_norm None: max(1, rms(z/s)) formed on the device."""
+        zs_norm = -1.0 if zs_norm is None else zs_norm
         info = _abi.GmresInfo()
         _abi.call("vtsa_gmres", self.h, float(t), dv.ptr(z), dv.ptr(f), dv.ptr(s), float(dh), float(zs_norm),
                   dv.ptr(b), dv.ptr(w), float(rtol), float(atol), int(restart), C.byref(info), self.st())
@@ -280,7 +282,7 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
         it += 1
         ops.scale(z, rtol, s)
         ops.vec(8, b, x=R, y=s)
-        zs_norm = max(1.0, ops.rms(z, s))
+        zs_norm = None   # max(1, rms(z/s)), formed on the device by the GMRES cycle
         floor = (1e-4 if polish else 0.1 * kappa) * np.sqrt(N)
         info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, (1e-3 if polish else eta), floor, krylov_dim)
         if polish and stats is not None:

@@ -272,7 +272,7 @@ int vtsa_precond_setup(vtsa_handle* H, double dh, const double* scale, int32_t k
 int vtsa_precond_apply(vtsa_handle* H, const double* r, double* out, vtsa_stream s) {
   if (!H || !r || !out) { set_error("vtsa_precond_apply: null argument"); return -1; }
   if (!H->h.prec_ready) { set_error("vtsa_precond_apply: no factorisation"); return -2; }
-  launch_prec_apply(H->h, r, nullptr, nullptr, nullptr, nullptr, out, false, 0.0, S(s));
+  launch_prec_apply(H->h, r, nullptr, nullptr, nullptr, nullptr, out, false, S(s));
   return finish(S(s));
 }
 

@@ -19,6 +19,7 @@ struct GmresScalars {
   double cs[MAX_RESTART], sn[MAX_RESTART], S[MAX_RESTART + 1], yv[MAX_RESTART];
   double h_tmp[MAX_RESTART + 1];            // Gram-Schmidt coefficients of the current column
   double h0sq;                              // |w|^2 before orthogonalisation
+  double zs_rms;                            // rms(z/s) of the cycle (the FD step uses max(1, .))
   double sigma;                             // FD step of the current matvec
   double unorm;                             // rms(u) of the current matvec
   double inv_h1;                            // 1 / |w| after orthogonalisation
@@ -117,7 +118,7 @@ int cons_shift_impl(Handle& h, const double* z, const double* s, const double la
 // out = post^-1 * M^-1 (pre * in * (in_scale ? *in_scale : 1)); optionally stores the scaled
 // input to `store_in`; optionally accumulates sum(out^2) and sets sigma in gs (matvec use).
 void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,
-                       const double* post, double* out, bool sigma_for_matvec, double zs_norm, hipStream_t s);
+                       const double* post, double* out, bool sigma_for_matvec, hipStream_t s);
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
                 const double* b, double* w, double rtol, double atol, int restart, vtsa_gmres_info* info,
                 hipStream_t st);
@@ -145,11 +146,11 @@ size_t gs_pairs_count(int ncol, int Nz);   // values of Handle::P
 // Preconditioner apply: out = post^-1 * S0 M~^-1 S0^-1 (pre * scale * in); optional side outputs:
 //   store_in  <- scale * in (normalises a GMRES basis vector in place; may alias in)
 //   red_out   <- rms(out / red_s) (or rms(out))
-//   gs        <- unorm, sigma = sqrt(eps) zs_norm / unorm, zero_u (matvec finite-difference step)
+//   gs        <- unorm, sigma = sqrt(eps) max(1, *zs_rms) / unorm, zero_u (matvec finite-difference step)
 // respect_done: return at once when the GMRES cycle in gs has finished.
 struct ApplyArgs {
   const double* in; const double* in_scale; double* store_in; const double* pre; const double* post; double* out;
-  const double* red_s; double* red_out; GmresScalars* gs; double zs_norm; int respect_done;
+  const double* red_s; double* red_out; GmresScalars* gs; const double* zs_rms; int respect_done;
 };
 void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s);
 

@@ -33,7 +33,12 @@ __global__ void __launch_bounds__(RBLK) k_residual_combine(int N, const double* 
   }
 }
 
+__global__ void k_set_scalar(double* p, double v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
 // generic reduction: mode 0: rms(a/s); mode 1: rms(a)
+
 __global__ void __launch_bounds__(RBLK) k_rms(int N, const double* __restrict__ a, const double* __restrict__ s,
                                               double* __restrict__ partial, unsigned* counter, double* result) {
   __shared__ double sh[RBLK];
@@ -599,7 +604,7 @@ int newton_eval_impl(Handle& h, double t, const double* z, double dh, const doub
   launch_flux(h, t, z, h.pb, f, st);
   hipLaunchKernelGGL(k_residual_combine, dim3(rgrid(h.N)), dim3(RBLK), 0, st, h.N, z, f, dh, rk, s, R,
                      h.red.partial, h.red.counter + 2, h.red.result + 0);
-  ApplyArgs a{R, nullptr, nullptr, nullptr, nullptr, h.vtmp, s, h.red.result + 1, nullptr, 0.0, 0};
+  ApplyArgs a{R, nullptr, nullptr, nullptr, nullptr, h.vtmp, s, h.red.result + 1, nullptr, nullptr, 0};
   apply_launch(h, a, st);
   if (check(hipMemcpyAsync(h.host_scalars, h.red.result, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
     return -1;
@@ -618,7 +623,7 @@ int newton_try_impl(Handle& h, double t, const double* z, const double* dz, doub
   launch_flux(h, t, zt, h.pb, ft, st);
   hipLaunchKernelGGL(k_residual_combine, dim3(rgrid(h.N)), dim3(RBLK), 0, st, h.N, zt, ft, dh, rk, s, Rt,
                      h.red.partial, h.red.counter + 2, h.red.result + 0);
-  ApplyArgs a{Rt, nullptr, nullptr, nullptr, nullptr, h.vtmp, s, h.red.result + 1, nullptr, 0.0, 0};
+  ApplyArgs a{Rt, nullptr, nullptr, nullptr, nullptr, h.vtmp, s, h.red.result + 1, nullptr, nullptr, 0};
   apply_launch(h, a, st);
   if (check(hipMemcpyAsync(h.host_scalars, h.red.result, 3 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
     return -1;
@@ -638,7 +643,7 @@ int error_norm_impl(Handle& h, const double* y, const double* z2, const double* 
   hipLaunchKernelGGL(k_vec_op, dim3(eblk(h.N)), dim3(256), 0, st, 7, h.N, 0, 0.0, 0.0, 0.0, 0.0, y3, z2,
                      (const double*)nullptr, (const double*)nullptr, h.vtmp);
   hipLaunchKernelGGL(k_scale2, dim3(eblk(h.N)), dim3(256), 0, st, h.m, y, z2, rtol, h.vtmp2);
-  ApplyArgs a{h.vtmp, nullptr, nullptr, nullptr, nullptr, h.vtmp, h.vtmp2, h.red.result + 3, nullptr, 0.0, 0};
+  ApplyArgs a{h.vtmp, nullptr, nullptr, nullptr, nullptr, h.vtmp, h.vtmp2, h.red.result + 3, nullptr, nullptr, 0};
   apply_launch(h, a, st);
   if (check(hipMemcpyAsync(h.host_scalars, h.red.result + 3, sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
     return -1;
@@ -668,6 +673,12 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   constexpr int GM_LOOKAHEAD = 3;
   constexpr int NOT_STARTED = -0x7fffffff;
   __atomic_store_n(h.progress, NOT_STARTED, __ATOMIC_RELEASE);
+  // rms(z/s) for the matvec's finite-difference step: given by the caller, or (zs_norm < 0) here
+  if (zs_norm < 0.0)
+    hipLaunchKernelGGL(k_rms, dim3(rgrid(N)), dim3(RBLK), 0, st, N, z, s, h.red.partial, h.red.counter + 2,
+                       &h.gs->zs_rms);
+  else
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, &h.gs->zs_rms, zs_norm);
   hipLaunchKernelGGL(k_gm_init, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial, h.red.counter + 5,
                      h.progress_dev);
   if (check(hipGetLastError(), "k_gm_init")) return -1;
@@ -688,7 +699,7 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
     }
     // u = P^-1 v_col with v_col = w_prev / |w_prev| normalised in place
     const double* vin = col == 0 ? b : Vb + (size_t)col * ld;
-    ApplyArgs a{vin, &h.gs->inv_h1, Vb + (size_t)col * ld, s, s, U, nullptr, nullptr, h.gs, zs_norm, 1};
+    ApplyArgs a{vin, &h.gs->inv_h1, Vb + (size_t)col * ld, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 1};
     apply_launch(h, a, st);
     double* wcol = Vb + (size_t)(col + 1) * ld;
     launch_matvec_residual(h, t, z, U, s, f, dh, wcol, true, st);
@@ -703,7 +714,7 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);
   // true residual r = b - A P^-1 w (one more matvec, as scipy does after the cycle)
   {
-    ApplyArgs a{w, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, zs_norm, 0};
+    ApplyArgs a{w, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 0};
     apply_launch(h, a, st);
   }
   launch_matvec_residual(h, t, z, U, s, f, dh, h.vtmp, false, st);

@@ -717,7 +717,7 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
     if (a.gs) {
       a.gs->unorm = rms;
       a.gs->zero_u = (rms == 0.0) ? 1 : 0;
-      a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * a.zs_norm) / rms;
+      a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * fmax(1.0, *a.zs_rms)) / rms;
     }
   }
 }
@@ -835,7 +835,7 @@ __device__ inline void gs_out_norm(const ApplyArgs& a, double local, int L, int 
   if (a.gs) {
     a.gs->unorm = rms;
     a.gs->zero_u = (rms == 0.0) ? 1 : 0;
-    a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * a.zs_norm) / rms;
+    a.gs->sigma = rms == 0.0 ? 0.0 : (1.4901161193847656e-08 * fmax(1.0, *a.zs_rms)) / rms;
   }
 }
 
@@ -1268,8 +1268,9 @@ void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
 }
 
 void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,
-                       const double* post, double* out, bool sigma_for_matvec, double zs_norm, hipStream_t s) {
-  ApplyArgs a{in, in_scale, store_in, pre, post, out, nullptr, nullptr, sigma_for_matvec ? h.gs : nullptr, zs_norm,
+                       const double* post, double* out, bool sigma_for_matvec, hipStream_t s) {
+  ApplyArgs a{in, in_scale, store_in, pre, post, out, nullptr, nullptr, sigma_for_matvec ? h.gs : nullptr,
+              sigma_for_matvec ? &h.gs->zs_rms : nullptr,
               sigma_for_matvec ? 1 : 0};
   apply_launch(h, a, s);
 }

@@ -130,6 +130,8 @@ int vtsa_precond_apply(vtsa_handle* h, const double* r, double* out, vtsa_stream
  *   A u = (v - dh (f(t, z + sigma v) - f)/sigma)/s,  v = u*s,  sigma = sqrt(eps) zs_norm / rms(u)
  *   P^-1 w = M^-1(w*s)/s
  * solving (A P^-1) w = b with tolerance max(atol, rtol |b|) (core/solver.py:196-224).
+ * The step uses max(1, zs_norm); zs_norm < 0: max(1, rms(z/s)) is formed on the device instead
+ * (no wait before the cycle).
  * On return w holds the GMRES solution (in the preconditioned variable, as upstream). */
 int vtsa_gmres(vtsa_handle* h, double t, const double* z, const double* f, const double* s, double dh,
                double zs_norm, const double* b, double* w, double rtol, double atol, int32_t restart,

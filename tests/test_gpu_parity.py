@@ -366,6 +366,11 @@ def test_gmres_matches_scipy_on_newton_system(dh, rtol):
     wd = dv.to_host(w)
     err = np.max(np.abs(wd - wo)) / np.max(np.abs(wo))
     assert err < 1e-3, err
+    # zs_norm None: the same max(1, rms(z/s)) formed on the device gives the same cycle bit for bit
+    w2 = dv.empty(m.n_core)
+    info2 = ops.gmres(t, yd, f, s, dh, None, dv.to_dev(b), w2, rtol, floor, 30)
+    assert (info2.info, info2.iters) == (info.info, info.iters)
+    assert np.array_equal(dv.to_host(w2), wd)
 
 
 @pytest.mark.parametrize("h", [1.6, 0.5])
