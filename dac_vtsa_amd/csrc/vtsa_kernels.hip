@@ -2,8 +2,9 @@
 //
 // f(t, y) is evaluated in two cell-parallel passes (one thread per cell, z fastest so that
 // a wavefront reads 64 consecutive cells of a z column with unit stride):
-//   k_props : thermodynamic state, gas properties, equilibrium and LDF rates of each cell,
-//             cached as 12 SoA property arrays (96 B/cell) and the loading rows of f;
+//   k_props : thermodynamic state, gas properties, equilibrium and LDF rates of each cell
+//             (transport and equilibrium in two wavefronts of one block), cached as 12 SoA
+//             property arrays (96 B/cell) and the loading rows of f;
 //   k_faces : every face flux once (one thread per face) from the cached properties of the
 //             face's stencil, into a face table;
 //   k_flux  : per cell the telescoping divergence of its 6 faces from the table, sinks and
@@ -30,19 +31,31 @@ __device__ inline void load_state(const double* __restrict__ y, int n, int cell,
 }
 
 // ------------------------------------------------------------------ residual passes
-__global__ void __launch_bounds__(BLK) k_props(Model m, const double* __restrict__ y, double* __restrict__ pb,
-                                               double* __restrict__ f, double* __restrict__ qstar) {
-  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+// Property passes: 64 cells per block of two wavefronts.  Both waves form the cell's state;
+// wave 0 then computes the transport properties, wave 1 the equilibrium loadings and LDF rates,
+// so the two transcendental chains of a cell run on different SIMDs.
+constexpr int PCELLS = 64;
+
+__global__ void __launch_bounds__(2 * PCELLS) k_props(Model m, const double* __restrict__ y, double* __restrict__ pb,
+                                                      double* __restrict__ f, double* __restrict__ qstar) {
+  const int cell = blockIdx.x * PCELLS + (threadIdx.x & (PCELLS - 1));
+  const bool transport = threadIdx.x < PCELLS;   // uniform per wavefront
   const int n = m.n;
   if (cell >= n) return;
   double c[NG], q[NA], e;
   load_state(y, n, cell, c, q, e);
-  Props p;
-  double rate[NA], qs[NA];
-  cell_props(m.d, c, q, e, p, rate, qs);
-  store_props(pb, n, cell, p);
-  if (f) { f[3 * n + cell] = rate[0]; f[4 * n + cell] = rate[1]; }
-  if (qstar) { qstar[cell] = qs[0]; qstar[n + cell] = qs[1]; }
+  CellState cs;
+  cell_state(m.d, c, q, e, cs);
+  if (transport) {
+    Props p;
+    cell_transport(m.d, cs, p);
+    store_props(pb, n, cell, p);
+  } else {
+    double rate[NA], qs[NA];
+    cell_equilibrium(m.d, c, q, cs, rate, qs);
+    if (f) { f[3 * n + cell] = rate[0]; f[4 * n + cell] = rate[1]; }
+    if (qstar) { qstar[cell] = qs[0]; qstar[n + cell] = qs[1]; }
+  }
 }
 
 __device__ inline double wall_row(const Model& m, const double* __restrict__ Tw_all, const Props& o, int j, int k,
@@ -98,37 +111,52 @@ __global__ void __launch_bounds__(BLK) k_flux(Model m, double t, const double* _
 // ------------------------------------------------------------------ matvec residual
 // Finite-difference Jacobian-vector product of the Newton operator (core/solver.py:196-203):
 //   v = u*s, fp = f(t, z + sigma v), w = (v - dh (fp - f)/sigma)/s.
-// Pass 1 forms the perturbed state, its properties and the loading rows of w.
-__global__ void __launch_bounds__(BLK) k_mv_props(Model m, const double* __restrict__ z, const double* __restrict__ u,
-                                                  const double* __restrict__ s, const double* __restrict__ f,
-                                                  const GmresScalars* __restrict__ gs, int respect_done, double dh,
-                                                  double* __restrict__ yp, double* __restrict__ pb,
-                                                  double* __restrict__ fp, double* __restrict__ w) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+// Pass 1 forms the perturbed state, its properties and the loading rows of w.  Blocks
+// 0 .. ceil(n/64)-1 take 64 cells each in two wavefronts (transport | equilibrium, as k_props);
+// the blocks after them form the perturbed wall temperatures, 128 per block.
+__global__ void __launch_bounds__(2 * PCELLS) k_mv_props(Model m, const double* __restrict__ z,
+                                                         const double* __restrict__ u, const double* __restrict__ s,
+                                                         const double* __restrict__ f,
+                                                         const GmresScalars* __restrict__ gs, int respect_done,
+                                                         double dh, double* __restrict__ yp, double* __restrict__ pb,
+                                                         double* __restrict__ fp, double* __restrict__ w) {
   const int n = m.n;
   if (respect_done && gs->done) return;
   const double sig = gs->sigma;
   const bool zero = gs->zero_u != 0;
-  if (tid < n) {
+  const int cblocks = (n + PCELLS - 1) / PCELLS;
+  if ((int)blockIdx.x < cblocks) {
+    const int cell = blockIdx.x * PCELLS + (threadIdx.x & (PCELLS - 1));
+    const bool transport = threadIdx.x < PCELLS;   // uniform per wavefront
+    if (cell >= n) return;
     double yl[NV];
     for (int v = 0; v < NV; ++v) {
-      int idx = v * n + tid;
+      int idx = v * n + cell;
       yl[v] = z[idx] + sig * (u[idx] * s[idx]);
-      yp[idx] = yl[v];
+      if (transport) yp[idx] = yl[v];
     }
-    Props p;
-    double rate[NA], qs[NA];
-    cell_props(m.d, yl, yl + 3, yl[5], p, rate, qs);
-    store_props(pb, n, tid, p);
-    for (int a = 0; a < NA; ++a) {
-      int idx = (3 + a) * n + tid;
-      fp[idx] = rate[a];
-      double vv = u[idx] * s[idx];
-      w[idx] = zero ? 0.0 : (vv - (dh * (rate[a] - f[idx])) / sig) / s[idx];
+    CellState cs;
+    cell_state(m.d, yl, yl + 3, yl[5], cs);
+    if (transport) {
+      Props p;
+      cell_transport(m.d, cs, p);
+      store_props(pb, n, cell, p);
+    } else {
+      double rate[NA], qs[NA];
+      cell_equilibrium(m.d, yl, yl + 3, cs, rate, qs);
+      for (int a = 0; a < NA; ++a) {
+        int idx = (3 + a) * n + cell;
+        fp[idx] = rate[a];
+        double vv = u[idx] * s[idx];
+        w[idx] = zero ? 0.0 : (vv - (dh * (rate[a] - f[idx])) / sig) / s[idx];
+      }
     }
-  } else if (tid < n + m.nw) {
-    int idx = 6 * n + (tid - n);
-    yp[idx] = z[idx] + sig * (u[idx] * s[idx]);
+  } else {
+    const int wi = (blockIdx.x - cblocks) * (2 * PCELLS) + threadIdx.x;
+    if (wi < m.nw) {
+      int idx = 6 * n + wi;
+      yp[idx] = z[idx] + sig * (u[idx] * s[idx]);
+    }
   }
 }
 
@@ -483,7 +511,7 @@ static inline int nblk(long long work, int b = BLK) { return (int)((work + b - 1
 constexpr int RB = 64;
 
 void launch_props(Handle& h, const double* y, double* pb, double* f, double* qstar, hipStream_t s) {
-  hipLaunchKernelGGL(k_props, dim3(nblk(h.n, RB)), dim3(RB), 0, s, h.m, y, pb, f, qstar);
+  hipLaunchKernelGGL(k_props, dim3(nblk(h.n, PCELLS)), dim3(2 * PCELLS), 0, s, h.m, y, pb, f, qstar);
 }
 
 void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s) {
@@ -500,7 +528,7 @@ void launch_diag(Handle& h, double t, const double* y, const double* pb, double*
                  double* uth, double* bnd, double* wall, hipStream_t s) {
   // qstar of the current point: recompute the properties with the q* side output
   double* qs = h.vtmp2;
-  hipLaunchKernelGGL(k_props, dim3(nblk(h.n)), dim3(BLK), 0, s, h.m, y, h.pb, nullptr, qs);
+  hipLaunchKernelGGL(k_props, dim3(nblk(h.n, PCELLS)), dim3(2 * PCELLS), 0, s, h.m, y, h.pb, nullptr, qs);
   hipLaunchKernelGGL(k_diag, dim3(nblk(h.n + h.desc.Nth * h.desc.Nz)), dim3(BLK), 0, s, h.m, t, y, h.pb, qs,
                      cell, uz, ur, uth, bnd, wall);
 }
@@ -523,8 +551,8 @@ void launch_equilibrium(Handle& h, int count, const double* pc, const double* pw
 void launch_matvec_residual(Handle& h, double t, const double* z, const double* u, const double* s,
                             const double* f, double dh, double* w, bool respect_done, hipStream_t st) {
   const int rd = respect_done ? 1 : 0;
-  hipLaunchKernelGGL(k_mv_props, dim3(nblk(h.n + h.nw, RB)), dim3(RB), 0, st, h.m, z, u, s, f, h.gs, rd, dh, h.ytmp,
-                     h.pb2, h.ftmp, w);
+  hipLaunchKernelGGL(k_mv_props, dim3(nblk(h.n, PCELLS) + nblk(h.nw, 2 * PCELLS)), dim3(2 * PCELLS), 0, st, h.m, z, u,
+                     s, f, h.gs, rd, dh, h.ytmp, h.pb2, h.ftmp, w);
   const int nf = FaceTable::of(h.desc).count();
   hipLaunchKernelGGL(k_faces, dim3(nblk(nf, RB)), dim3(RB), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.gs, rd);
   hipLaunchKernelGGL(k_mv_flux, dim3(nblk(h.n + h.nw, RB)), dim3(RB), 0, st, h.m, t, h.ytmp, h.pb2, h.FF, h.ftmp, u,

@@ -182,8 +182,15 @@ __device__ inline double fuller(double T, double P, double sqrtM, double s) {
 // Thermodynamic state, gas properties, stagnant conductivity and the LDF rates of one cell
 // (core/residual.py:218-238 with physics/energy.py:46-56, properties/ideal_gas_backend.py:94-137,
 // physics/heat_transfer.py:23-30,72-77, physics/isotherms.py:262-269, physics/kinetics.py:29-39).
-__device__ inline void cell_props(const vtsa_model_desc& d, const double c[NG], const double q[NA],
-                                  double e, Props& p, double rate[NA], double qstar[NA]) {
+// Split in three so that the transport and equilibrium halves can run in different wavefronts:
+// cell_state (temperature, pressure, mole fractions), cell_transport (mixture viscosity and
+// conductivity, diffusivity, stagnant conductivity) and cell_equilibrium (q* and LDF rates).
+struct CellState {
+  double T, P, Tp, Cpos, y[NG], ycl[NG];
+};
+
+__device__ __forceinline__ void cell_state(const vtsa_model_desc& d, const double c[NG], const double q[NA], double e,
+                                  CellState& cs) {
   double Ctot = (c[0] + c[1]) + c[2];
   double gas_cp = (d.cpg[0] * c[0] + d.cpg[1] * c[1]) + d.cpg[2] * c[2];
   double A = d.eps_t * gas_cp + d.rho_b * (d.cp_s + (d.cpa[0] * q[0] + d.cpa[1] * q[1]));
@@ -198,6 +205,13 @@ __device__ inline void cell_props(const vtsa_model_desc& d, const double c[NG], 
     y[s] = c[s] / Cpos;
     ycl[s] = npclip(y[s], 0.0, 1.0);
   }
+  cs.T = T; cs.P = P; cs.Tp = Tp; cs.Cpos = Cpos;
+  for (int s = 0; s < NG; ++s) { cs.y[s] = y[s]; cs.ycl[s] = ycl[s]; }
+}
+
+__device__ __forceinline__ void cell_transport(const vtsa_model_desc& d, const CellState& cs, Props& p) {
+  const double Tp = cs.Tp, Cpos = cs.Cpos, P = cs.P;
+  const double* ycl = cs.ycl;
   // Wilke viscosity / Mason-Saxena conductivity
   double Tr = Tp / 300.0;
   double mu_i[NG], k_i[NG], sq[NG];
@@ -246,7 +260,14 @@ __device__ inline void cell_props(const vtsa_model_desc& d, const double c[NG], 
                                   0.5 * (B_ + 1.0)) - (B_ - 1.0) / one_m);
     lam0 = kg * ((1.0 - d.zs_s) + d.zs_s * kc);
   }
-  // equilibrium + LDF
+  p.T = cs.T; p.P = P; p.Cpos = Cpos;
+  p.y[0] = cs.y[0]; p.y[1] = cs.y[1]; p.y[2] = cs.y[2];
+  p.mu = mu; p.kg = kg; p.rho = rho; p.cpm = cpm; p.Dm = Dm; p.lam0 = lam0;
+}
+
+__device__ __forceinline__ void cell_equilibrium(const vtsa_model_desc& d, const double c[NG], const double q[NA],
+                                        const CellState& cs, double rate[NA], double qstar[NA]) {
+  const double Tp = cs.Tp;
   double pk0 = (c[0] * d.R) * Tp, pk1 = (c[1] * d.R) * Tp;
   double psat = arden_buck(Tp);
   double qw_eq = q_h2o(pk1, Tp, psat, d);
@@ -258,9 +279,14 @@ __device__ inline void cell_props(const vtsa_model_desc& d, const double c[NG], 
                            : d.k_ldf[j] * 1.0;
     rate[j] = k * (qstar[j] - q[j]);
   }
-  p.T = T; p.P = P; p.Cpos = Cpos;
-  p.y[0] = y[0]; p.y[1] = y[1]; p.y[2] = y[2];
-  p.mu = mu; p.kg = kg; p.rho = rho; p.cpm = cpm; p.Dm = Dm; p.lam0 = lam0;
+}
+
+__device__ inline void cell_props(const vtsa_model_desc& d, const double c[NG], const double q[NA],
+                                  double e, Props& p, double rate[NA], double qstar[NA]) {
+  CellState cs;
+  cell_state(d, c, q, e, cs);
+  cell_transport(d, cs, p);
+  cell_equilibrium(d, c, q, cs, rate, qstar);
 }
 
 __device__ inline void load_props(const double* __restrict__ pb, int n, int cell, Props& p) {
