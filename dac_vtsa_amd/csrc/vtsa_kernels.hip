@@ -121,19 +121,29 @@ __global__ void __launch_bounds__(2 * PCELLS) k_mv_props(Model m, const double* 
                                                          double dh, double* __restrict__ yp, double* __restrict__ pb,
                                                          double* __restrict__ fp, double* __restrict__ w) {
   const int n = m.n;
-  if (respect_done && gs->done) return;
-  const double sig = gs->sigma;
-  const bool zero = gs->zero_u != 0;
   const int cblocks = (n + PCELLS - 1) / PCELLS;
   if ((int)blockIdx.x < cblocks) {
     const int cell = blockIdx.x * PCELLS + (threadIdx.x & (PCELLS - 1));
     const bool transport = threadIdx.x < PCELLS;   // uniform per wavefront
     if (cell >= n) return;
-    double yl[NV];
+    // the cell's 18 inputs are requested together, before the cycle's flags and sigma arrive
+    double zv[NV], uv[NV], sv[NV];
+#pragma unroll
     for (int v = 0; v < NV; ++v) {
-      int idx = v * n + cell;
-      yl[v] = z[idx] + sig * (u[idx] * s[idx]);
-      if (transport) yp[idx] = yl[v];
+      const int idx = v * n + cell;
+      zv[v] = z[idx];
+      uv[v] = u[idx];
+      sv[v] = s[idx];
+    }
+    if (respect_done && gs->done) return;
+    const double sig = gs->sigma;
+    const bool zero = gs->zero_u != 0;
+    double yl[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) yl[v] = zv[v] + sig * (uv[v] * sv[v]);
+    if (transport) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) yp[v * n + cell] = yl[v];
     }
     CellState cs;
     cell_state(m.d, yl, yl + 3, yl[5], cs);
@@ -147,11 +157,13 @@ __global__ void __launch_bounds__(2 * PCELLS) k_mv_props(Model m, const double* 
       for (int a = 0; a < NA; ++a) {
         int idx = (3 + a) * n + cell;
         fp[idx] = rate[a];
-        double vv = u[idx] * s[idx];
-        w[idx] = zero ? 0.0 : (vv - (dh * (rate[a] - f[idx])) / sig) / s[idx];
+        double vv = uv[3 + a] * sv[3 + a];
+        w[idx] = zero ? 0.0 : (vv - (dh * (rate[a] - f[idx])) / sig) / sv[3 + a];
       }
     }
   } else {
+    if (respect_done && gs->done) return;
+    const double sig = gs->sigma;
     const int wi = (blockIdx.x - cblocks) * (2 * PCELLS) + threadIdx.x;
     if (wi < m.nw) {
       int idx = 6 * n + wi;
