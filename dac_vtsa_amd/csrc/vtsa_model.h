@@ -747,25 +747,37 @@ __device__ inline void cell_flux(const Model& m, const Acc& acc, double Tw_self,
 
 // ---------------------------------------------------------------- face-parallel flux
 // The residual's faces, each computed once by one thread into a face table (SoA, 4 rows of nf:
-// F_0, F_1, F_2, H), then summed per cell.  Face index space:
-//   [0, ncol (Nz + 1))                z faces: column (i, j), slot k = the face below cell k
-//                                      (slot Nz: the face above the last cell)
+// F_0, F_1, F_2, H), then summed per cell.  Face index space (col = i Nth + j):
+//   [0, ncol (Nz - 1))                interior z faces: col (Nz - 1) + k - 1 = the face below cell k
+//   [.., + ncol)                      bottom end faces, by col
+//   [.., + ncol)                      top end faces, by col
 //   [.., + (Nr - 1) Nth Nz)           radial faces (i < Nr - 1, j, k)
 //   [.., + Nth > 1 ? n : 0)           theta faces (i, j, k)
+// The end faces run the boundary-condition code; grouped apart, they share wavefronts only
+// with each other instead of putting that code into every z-face wavefront.
 struct FaceTable {
-  int nzf, nrf, ntf;
+  int ncol, nzi, nzf, nrf, ntf;
   __device__ __host__ inline int count() const { return nzf + nrf + ntf; }
   __device__ __host__ static FaceTable of(const vtsa_model_desc& d) {
     FaceTable f;
-    f.nzf = d.Nr * d.Nth * (d.Nz + 1);
+    f.ncol = d.Nr * d.Nth;
+    f.nzi = f.ncol * (d.Nz - 1);
+    f.nzf = f.ncol * (d.Nz + 1);
     f.nrf = (d.Nr - 1) * d.Nth * d.Nz;
     f.ntf = d.Nth > 1 ? d.Nr * d.Nth * d.Nz : 0;
     return f;
   }
+  // kind 0: the face below cell k, kind 3: the face above it, 1: outer r face, 2: theta+ face
   __device__ inline int index(const vtsa_model_desc& d, int kind, int i, int j, int k) const {
-    if (kind == 0 || kind == 3) return (i * d.Nth + j) * (d.Nz + 1) + k + (kind == 3 ? 1 : 0);
-    if (kind == 1) return nzf + (i * d.Nth + j) * d.Nz + k;
-    return nzf + nrf + (i * d.Nth + j) * d.Nz + k;
+    const int col = i * d.Nth + j;
+    if (kind == 0 || kind == 3) {
+      const int zf = k + (kind == 3 ? 1 : 0);   // 0 .. Nz along the column
+      if (zf == 0) return nzi + col;
+      if (zf == d.Nz) return nzi + ncol + col;
+      return col * (d.Nz - 1) + zf - 1;
+    }
+    if (kind == 1) return nzf + col * d.Nz + k;
+    return nzf + nrf + col * d.Nz + k;
   }
 };
 
@@ -773,19 +785,26 @@ template <class Acc>
 __device__ inline void face_by_index(const Model& m, const Acc& acc, const FaceTable& ft, int f, double t,
                                      double F[NG], double& H) {
   const vtsa_model_desc& d = m.d;
-  int kind, i, j, k;
-  if (f < ft.nzf) {
-    const int col = f / (d.Nz + 1);
-    k = f - col * (d.Nz + 1); i = col / d.Nth; j = col - i * d.Nth;
+  int kind, col, k;
+  if (f < ft.nzi) {
+    col = f / (d.Nz - 1);
+    k = f - col * (d.Nz - 1) + 1;
     kind = 0;
-    if (k == d.Nz) { kind = 3; k = d.Nz - 1; }   // the face above the last cell
+  } else if (f < ft.nzi + ft.ncol) {
+    col = f - ft.nzi;
+    k = 0;
+    kind = 0;
+  } else if (f < ft.nzf) {
+    col = f - ft.nzi - ft.ncol;
+    k = d.Nz - 1;
+    kind = 3;   // the face above the last cell
   } else {
     const int g = f < ft.nzf + ft.nrf ? f - ft.nzf : f - ft.nzf - ft.nrf;
     kind = f < ft.nzf + ft.nrf ? 1 : 2;
-    k = g % d.Nz;
-    const int col = g / d.Nz;
-    j = col % d.Nth; i = col / d.Nth;
+    col = g / d.Nz;
+    k = g - col * d.Nz;
   }
+  const int i = col / d.Nth, j = col - i * d.Nth;
   any_face(m, acc, kind, i, j, k, t, F, H);
 }
 
