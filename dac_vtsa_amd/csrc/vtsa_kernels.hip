@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(2 * PCELLS) k_mv_props(Model m, const double* 
     const int cell = blockIdx.x * PCELLS + (threadIdx.x & (PCELLS - 1));
     const bool transport = threadIdx.x < PCELLS;   // uniform per wavefront
     if (cell >= n) return;
-    // the cell's 18 inputs are requested together, before the cycle's flags and sigma arrive
+    // the cell's 18 inputs in one batch of loads (not one round trip per variable)
     double zv[NV], uv[NV], sv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {

@@ -670,7 +670,7 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   // The host queues at most GM_LOOKAHEAD Arnoldi columns beyond the last one the device reports
   // complete (a coherent host word written by k_gm_init / k_gm_finish), and stops queueing once
   // the device reports convergence; queued columns past convergence return at once.
-  constexpr int GM_LOOKAHEAD = 3;
+  constexpr int GM_LOOKAHEAD = 2;
   constexpr int NOT_STARTED = -0x7fffffff;
   __atomic_store_n(h.progress, NOT_STARTED, __ATOMIC_RELEASE);
   // rms(z/s) for the matvec's finite-difference step: given by the caller, or (zs_norm < 0) here
