@@ -299,11 +299,13 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
                 counter[0] += info.iters
         ops.vec(9, ws, x=w, y=s)
         ops.prec_apply(ws, dz)
-        if not _finite(ops.rms(dz)):
-            return z0, False, it + 1, np.inf, new_lu
         lam = 1.0
         for ls in range(4):
             rn_try, pn_try, step = ops.newton_try(t, z, dz, lam, dh, rhs_known, s, zt, ft, Rt)
+            if math.isnan(step):
+                # dz holds a non-finite entry (core/solver.py:241-242); that test evaluates no residual
+                ops.m.nfev -= 1
+                return z0, False, it + 1, np.inf, new_lu
             if not _finite(rn_try):
                 pn_try = np.inf
             if _finite(pn_try) and (pn_try <= (1.0 - 1e-4 * lam) * pn or ls == 3):

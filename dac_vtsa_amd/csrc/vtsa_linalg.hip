@@ -66,9 +66,12 @@ __global__ void __launch_bounds__(RBLK) k_newton_update(int N, int npos, const d
   __shared__ double sh[RBLK];
   __shared__ bool slast;
   double loc = 0.0;
+  bool finite = true;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     double zo = z[i];
-    double v = zo + lam * dz[i];
+    const double d = dz[i];
+    finite = finite && __builtin_isfinite(d);
+    double v = zo + lam * d;
     if (i < npos) {
       double lo = 0.1 * (zo > 0.0 ? zo : (zo != zo ? zo : 0.0));
       v = (v != v || lo != lo) ? v + lo : (v > lo ? v : lo);
@@ -77,6 +80,8 @@ __global__ void __launch_bounds__(RBLK) k_newton_update(int N, int npos, const d
     double q = (v - zo) / s[i];
     loc += q * q;
   }
+  // a non-finite step direction turns the step norm into NaN (an overflowing sum gives +inf)
+  if (!finite) loc = __builtin_nan("");
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;

@@ -150,7 +150,9 @@ int vtsa_newton_eval(vtsa_handle* h, double t, const double* z, double dh, const
  * preconditioner apply.  Used where pn is not needed (the conservation projection's residuals). */
 int vtsa_newton_residual(vtsa_handle* h, double t, const double* z, double dh, const double* rhs_known,
                          const double* s, double* f, double* R, double* rn, vtsa_stream st);
-/* z_try = truncate(z + lam dz, z); f_try, R_try at z_try; rn, pn as above; step = rms((z_try-z)/s) */
+/* z_try = truncate(z + lam dz, z); f_try, R_try at z_try; rn, pn as above; step = rms((z_try-z)/s),
+ * or NaN when dz holds a non-finite entry (the test of core/solver.py:241-242, with no wait of its
+ * own; finite z and s make step NaN in no other case). */
 int vtsa_newton_try(vtsa_handle* h, double t, const double* z, const double* dz, double lam, double dh,
                     const double* rhs_known, const double* s, double* z_try, double* f_try, double* R_try,
                     double* rn, double* pn, double* step, vtsa_stream st);

@@ -414,6 +414,45 @@ def test_newton_stage_matches_oracle(h):
     assert np.sqrt(np.mean(((dv.to_host(zg) - zo) / sc) ** 2)) < 1e-3
 
 
+
+def test_newton_try_step_flags_a_nonfinite_direction():
+    """The trial step's size (core/solver.py:244-253): rms((z_try - z)/s) of the truncated trial
+    point for a finite direction, NaN when the direction holds NaN or +-inf (the upstream
+    np.isfinite(dz) test, core/solver.py:241-242, read from the same wait)."""
+    import math
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    from dac_vtsa_amd.core.solver import DeviceOps
+    z = golden("integ_cfg1.npz")
+    cfg, m = gpu_model(CFG1, 0, state_from(z))
+    y = z["sdirk2/snap_y"][2]
+    t = float(z["sdirk2/snap_t"][2])
+    dh = 0.5
+    ops = DeviceOps(m)
+    N = m.n_core
+    yd = dv.to_dev(y)
+    f = dv.empty(N)
+    m.rhs_dev(t, yd, f)
+    FDJacobian(m).evaluate(t, yd, f)
+    s = ops.scale(yd, 1e-4)
+    ops.precond_setup(dh, s, 1, 0)
+    sh = dv.to_host(s)
+    dz = 3.0 * np.random.default_rng(0).standard_normal(N) * sh
+    zt, ft, Rt = dv.empty(N), dv.empty(N), dv.empty(N)
+    for lam in (1.0, 0.25):
+        rn, pn, step = ops.newton_try(t, yd, dv.to_dev(dz), lam, dh, yd, s, zt, ft, Rt)
+        zth = dv.to_host(zt)
+        lo = 0.1 * np.maximum(y[:5 * m.grid.ncell], 0.0)
+        assert np.array_equal(zth[:5 * m.grid.ncell], np.maximum(y[:5 * m.grid.ncell] + lam * dz[:5 * m.grid.ncell], lo))
+        assert step == pytest.approx(np.sqrt(np.mean(((zth - y) / sh) ** 2)), rel=1e-12)
+        assert math.isfinite(rn) and math.isfinite(pn)
+    for k, bad in ((N // 3, np.nan), (N - 1, np.inf), (5, -np.inf)):
+        d2 = dz.copy()
+        d2[k] = bad
+        _, _, step2 = ops.newton_try(t, yd, dv.to_dev(d2), 1.0, dh, yd, s, zt, ft, Rt)
+        assert math.isnan(step2), (k, bad)
+    m.close()
+
 def test_conservation_projection_closes_the_totals():
     """project_conservation (no upstream counterpart): from a stage converged without the
     conservation test, the projected value closes |W R| <= tol W s for the four totals, moves
