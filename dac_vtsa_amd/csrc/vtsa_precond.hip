@@ -51,21 +51,41 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
     const int X = col * Nz + k;
     const double sr = s0[r * n + X];
     // row r of the block of M~ = S0^-1 (I - dh J) S0 between cell X and cell X + dk
-    auto mrow = [&](int slot, int dk, T* out) {
+    // loaded first (jr: row r of J's block, sc: the scales of the block's columns), formed after
+    auto mload = [&](int slot, int dk, double* jr, double* sc) {
       const double* J = Jc + ((size_t)X * NSLOT + slot) * 36 + r * 6;
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
-        const double v = (slot == S_SELF && r == c ? 1.0 : 0.0) - dh * J[c];
-        out[c] = (T)(v * (s0[c * n + X + dk] / sr));
+        jr[c] = J[c];
+        sc[c] = s0[c * n + X + dk];
+      }
+    };
+    auto mform = [&](int slot, const double* jr, const double* sc, T* out) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const double v = (slot == S_SELF && r == c ? 1.0 : 0.0) - dh * jr[c];
+        out[c] = (T)(v * (sc[c] / sr));
       }
     };
     const int q = k & 1, q1 = (k + 1) & 1;   // ring slots of cells k-2 (and k) and k-1
     T l2[6], l1[6], d[6], u1[6], u2[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) { l2[c] = T(0); l1[c] = T(0); u1[c] = T(0); u2[c] = T(0); }
+    // the cell's five block rows are requested together, one round trip per cell
+    double jr[5][6], sc[5][6];
+    if (zl && k >= 2) mload(S_ZM2, -2, jr[0], sc[0]);
+    if (zl && k >= 1) mload(S_ZM1, -1, jr[1], sc[1]);
+    mload(S_SELF, 0, jr[2], sc[2]);
+    if (zl && k + 1 < Nz) mload(S_ZP1, 1, jr[3], sc[3]);
+    if (zl && k + 2 < Nz) mload(S_ZP2, 2, jr[4], sc[4]);
+    T a2[6], a1[6];
+    if (zl && k >= 2) mform(S_ZM2, jr[0], sc[0], a2);
+    if (zl && k >= 1) mform(S_ZM1, jr[1], sc[1], a1);
+    mform(S_SELF, jr[2], sc[2], d);
+    if (zl && k + 1 < Nz) mform(S_ZP1, jr[3], sc[3], u1);
+    if (zl && k + 2 < Nz) mform(S_ZP2, jr[4], sc[4], u2);
     if (zl && k >= 2) {   // L2 = A_{k,k-2} Dinv_{k-2}
-      T a[6];
-      mrow(S_ZM2, -2, a);
+      const T* a = a2;
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
         T acc = T(0);
@@ -75,8 +95,7 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
       }
     }
     if (zl && k >= 1) {   // L1 = (A_{k,k-1} - L2 U1_{k-2}) Dinv_{k-1}
-      T a[6];
-      mrow(S_ZM1, -1, a);
+      T* a = a1;
       if (k >= 2) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -94,7 +113,7 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
         l1[c] = acc;
       }
     }
-    mrow(S_SELF, 0, d);   // D = A_kk - L1 U1_{k-1} - L2 U2_{k-2}
+    // D = A_kk - L1 U1_{k-1} - L2 U2_{k-2}
     if (zl && k >= 1) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
@@ -114,7 +133,6 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
       }
     }
     if (zl && k + 1 < Nz) {   // U1 = A_{k,k+1} - L1 U2_{k-1}
-      mrow(S_ZP1, 1, u1);
       if (k >= 1) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -125,7 +143,7 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
         }
       }
     }
-    if (zl && k + 2 < Nz) mrow(S_ZP2, 2, u2);   // U2 = A_{k,k+2}
+    // U2 = A_{k,k+2} as loaded
     lds_barrier();   // every lane has read ring slot q (cell k-2) before cell k overwrites it
     T* FX = F + (size_t)X * 180 + r * 6;
     if (active && zl) {
