@@ -67,3 +67,19 @@ WORKLOADS = {
     "midcycle_heating": (10, 16, 50, "midcycle", 2, 22400.0),
     "large_cycle": (20, 32, 100, "small", None, None),
 }
+
+
+def device_qstar(cfg, grid, props):
+    """``qstar_fn`` evaluated by the device isotherm kernels of a throwaway model of the grid."""
+    from .core.residual import TVSAModel
+    shp = grid.shape
+    T = np.full(shp, cfg.feed.T)
+    C = cfg.feed.P / (R_GAS * T)
+    c = np.stack([C * 4e-4, C * 1e-2, C * (1.0 - 4e-4 - 1e-2)])
+    st = CycleState(c=c, q=np.zeros((2,) + shp), T=T, Tw=T[-1].copy())
+    model = TVSAModel(cfg, grid, props, cfg.cycle[0], st)
+
+    def fn(pc, pw, T, qw):
+        return model.equilibrium(pc, pw, T, qw)
+    fn.model = model
+    return fn
