@@ -1,31 +1,39 @@
-"""Benchmark: simulated seconds per wall second of the implicit step engine on one MI355X.
+"""Benchmark: cycle simulated seconds per wall second of the implicit step engine on one MI355X.
 
-Workload (the metric's single-GPU configuration): grid Nr=10, Ntheta=16, Nz=50 (8,000 cells,
-48,800 unknowns), seeded mid-cycle state (seed 0: T~U(288,373) K, P~U(1.5e4,1.013e5) Pa,
-y_CO2~U(0,0.9), y_H2O~U(0,0.05), q_CO2~U(0,1.6), q_H2O~U(0,6) mol/kg), one full heating step of
-22,400 s (jacket 373.15 K, product vacuum pump at 1.5e4 Pa), FP64 residual / Jacobian / Newton /
-GMRES with the line Gauss-Seidel preconditioner (``--precision fp32`` stores its factors in FP32).
+Workload (the metric's configuration): grid Nr=20, Ntheta=32, Nz=100 (64,000 cells, 387,200
+unknowns), the default cycle adsorption 8,000 s -> blowdown 60 s -> heating 22,400 s -> cooling
+400 s (30,860 s), from the clean bed (feed gas at 288.15 K and 1 atm over bare sorbent, the
+upstream default ``run.initial_state``).  FP64 residual, FD Jacobian, Newton and GMRES(60)
+(Arnoldi orthogonalisation in FP64), line Gauss-Seidel preconditioner (2 symmetric sweeps of the
+z-line block LU), conservation projection, conservation_tol 1e-5 so that every step's mass and
+energy balance closes below 1e-6.
 
-One bench "step" = one complete heating step through the drop-in ``run_step`` (outlet recording
-and balances included, as upstream), from the same seeded state.  value = steps x 22,400 s /
-wall seconds of the timed region.
+One bench "step" is one window of 1,543 s of simulated time (1/20 of the cycle) of the cycle
+schedule run continuously from the clean bed: ``--steps 20`` times exactly one full cycle.  The
+runs that are timed, each over the same K windows from the same bed:
 
-Also reported: the per-step mass/energy balance errors; the roofline of the dominant kernel, the
-line Gauss-Seidel half-sweep, from every 8th of its launches in the timed region bracketed by HIP
-events on its stream (algorithmic bytes per launch / average launch time); the residual kernel
-pair timed the same way as a secondary figure; a CPU baseline: the NumPy/SciPy oracle (same
-algorithms as upstream, 1 thread) on the same state for a bounded wall-clock sample, with the GPU
-wall time over the same simulated window; one full cycle (``cycle``: adsorption, blowdown,
-heating, cooling, 30,860 s) on the same grid; and the learned form (``learned_fp32``): the same step
-with the learned stage guess and FP32 preconditioner factors, the network trained on the device
-from a plain run of another seeded state, its rate against the plain FP64 rate, its balances and
-its end-field differences from the plain FP64 run.
+* value: the learned + FP32 form -- the learned stage guess (network trained on the device from a
+  plain FP64 run of another initial bed, the feed-equilibrium one) and FP32 preconditioner
+  factors; Newton, residual, Jacobian and GMRES stay FP64 and decide every step;
+* ``plain_fp64``: the plain method with FP64 factors, and ``vs_plain_fp64`` = value / its rate;
+* ``fp32``: FP32 factors without the guess.
+
+Per-field errors (c_CO2, c_H2O, c_N2, q_CO2, q_H2O, T, Tw) of the FP32 and learned runs against
+the plain FP64 run are taken at the end of the last cycle step all runs completed.  The roofline
+is that of the dominant kernel (the line Gauss-Seidel half-sweep), from every 8th of its launches
+in the value's timed region bracketed by HIP events on its stream.  The CPU baseline is the
+NumPy/SciPy oracle (the upstream algorithms) on one host thread: its unit costs measured on this
+workload within a bounded budget, extrapolated with the operation counts of the plain FP64 run
+(the extrapolation is labelled in the line).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import multiprocessing as mp
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -35,119 +43,92 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_gs_half launch on this workload, from two rocprofv3 counter passes over one bench
-# step (FETCH_SIZE x 1.9993 + WRITE_SIZE x 1.0000 KiB, the factors from a 64 MiB device copy):
-# 5643.4 x 1.9993 + 373.1 KiB.  Re-measure when the kernel changes.
-TRAFFIC_BYTES_PER_LAUNCH = (5643.4 * 1.9993 + 373.1) * 1024
 FP64_PEAK_TFLOPS = 78.6
-WORKLOAD = {"grid.Nr": 10, "grid.Ntheta": 16, "grid.Nz": 50, "operation.P_vacuum": 1.5e4}
-STEP_INDEX = 2          # heating in the default cycle
-SEED = 0
-
-
-def setup(precision, overrides=None):
-    from dac_vtsa_amd.config import load_config
-    from dac_vtsa_amd.core.grid import Grid
-    from dac_vtsa_amd.properties import get_backend
-    from dac_vtsa_amd.seeds import seeded_state
-    cfg = load_config(override_dict={**WORKLOAD, **(overrides or {})})
-    grid = Grid.from_config(cfg)
-    props = get_backend(cfg)
-    state = seeded_state(grid, "midcycle", SEED)
-    return cfg, grid, props, state
+GRID = (20, 32, 100)
+CYCLE_S = 30860.0
+WINDOW_S = CYCLE_S / 20.0
+# HBM bytes per launch of the dominant kernel on this workload from two rocprofv3 counter passes
+# (FETCH_SIZE and WRITE_SIZE, scaled by a 64 MiB device copy); None until measured for the
+# current kernel.
+TRAFFIC_BYTES_PER_LAUNCH = None
+METRIC = "cycle sim-s/wall-s, 1 GPU: learned+FP32 vs plain FP64 at MB/EB err <=1e-6"
 
 
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def one_step(cfg, grid, props, state, precision, sweeps=2, conserve="polish"):
-    from dac_vtsa_amd.steps import run_step
-    return run_step(state, cfg, grid, props, cfg.cycle[STEP_INDEX], logger=_log, progress_s=20.0,
-                    integrator_kw={"precision": precision, "gs_sweeps": sweeps, "conserve": conserve})
+def make_cfg(extra=None):
+    from dac_vtsa_amd.config import load_config
+    nr, nth, nz = GRID
+    over = {"grid.Nr": nr, "grid.Ntheta": nth, "grid.Nz": nz, "solver.conservation_tol": 1e-5}
+    over.update(extra or {})
+    return load_config(override_dict=over)
 
 
-def copy_probe(mib=64, repeat=5):
-    """Calibration for the HBM traffic counters: `repeat` device-to-device copies of `mib` MiB
-    (the runtime's copy kernel reads and writes exactly that many bytes per dispatch)."""
-    from dac_vtsa_amd import device as dv
-    n = mib * (1 << 20) // 8
-    a, b = dv.zeros(n), dv.empty(n)
-    for _ in range(repeat):
-        b.copy_from(a)
-    dv.synchronize()
-    print(json.dumps({"copy_probe_bytes_per_dispatch": n * 8, "dispatches": repeat}), flush=True)
+def setup(extra=None, initial="clean"):
+    from dac_vtsa_amd.core.grid import Grid
+    from dac_vtsa_amd.core.state import initial_state
+    from dac_vtsa_amd.properties import get_backend
+    cfg = make_cfg({**(extra or {}), "run.initial_state": initial})
+    grid = Grid.from_config(cfg)
+    props = get_backend(cfg)
+    return cfg, grid, props, initial_state(cfg, grid, props)
 
 
-def residual_roofline(cfg, grid, props, state, n_launch=200):
-    """Average duration of one residual evaluation (k_props + k_flux, the kernel pair of every
-    line-search point and error estimate), timed with HIP events recorded by libvtsa on the
-    stream it launches on, and its algorithmic traffic (read y, write f)."""
-    import ctypes as C
-    from dac_vtsa_amd import _abi
-    from dac_vtsa_amd import device as dv
-    from dac_vtsa_amd.core.residual import TVSAModel
-    m = TVSAModel(cfg, grid, props, cfg.cycle[STEP_INDEX], state)
-    y = dv.to_dev(m.pack(state))
-    f = dv.empty(m.n_core)
-    m.rhs_dev(0.0, y, f)
-    dv.synchronize()
-    ms = C.c_double()
-    _abi.call("vtsa_time_rhs", m.handle, 0.0, dv.ptr(y), dv.ptr(f), n_launch, C.byref(ms), dv.stream())
-    avg_s = ms.value / 1e3 / n_launch
-    n, nw = grid.ncell, m.n_w
-    algo_bytes = 96 * n + 16 * nw          # read y, write f (FP64)
-    m.close()
-    return avg_s, algo_bytes
+def run_windows(cfg, grid, props, state, n_windows, integrator_kw, log=None):
+    """The cycle schedule from ``state`` for n_windows x WINDOW_S simulated seconds, the bed handed
+    from step to step in HBM-resident runs of ``run_step``; the last step ends at the first
+    accepted time step reaching the window boundary.  Returns the simulated seconds advanced, the
+    step results and the end state of every completed cycle step."""
+    from dac_vtsa_amd.steps import RUNNERS
+    t_goal = n_windows * WINDOW_S
+    t_done, cycle, T_jac = 0.0, 1, None
+    results, ends = [], []
+    while t_done < t_goal * (1.0 - 1e-12):
+        for step in cfg.cycle:
+            left = t_goal - t_done
+            dur = float(step["duration"])
+            stop = left if left < dur * (1.0 - 1e-12) else None
+            state, sr = RUNNERS[step["type"]](state, cfg, grid, props, step=step, cycle_idx=cycle,
+                                              T_jacket_prev=T_jac, logger=log, integrator_kw=integrator_kw,
+                                              stop_at=stop, progress_s=60.0)
+            t_done += sr.duration
+            T_jac = sr.T_jacket if sr.jacket_on else T_jac
+            results.append(sr)
+            if stop is None:
+                ends.append((cycle, step["name"], state))
+            if t_done >= t_goal * (1.0 - 1e-12):
+                break
+        cycle += 1
+    return t_done, results, ends
 
 
-def gpu_window(cfg, grid, props, state, t_end, precision, sweeps=2, conserve="polish"):
-    """GPU wall seconds for the first t_end simulated seconds of the step (same integrator)."""
-    from dac_vtsa_amd import device as dv
-    from dac_vtsa_amd.core.residual import TVSAModel
-    from dac_vtsa_amd.core.solver import TRBDF2
-    step = cfg.cycle[STEP_INDEX]
-    m = TVSAModel(cfg, grid, props, step, state)
-    integ = TRBDF2(m, cfg, h_max=cfg.solver.h_max_fraction * float(step["duration"]), precision=precision,
-                   gs_sweeps=sweeps, conserve=conserve)
-    dv.synchronize()
-    t0 = time.time()
-    integ.integrate(m.pack(state), t_end, np.array([t_end]))
-    dv.synchronize()
-    wall = time.time() - t0
-    m.close()
-    return wall
+def balance_errors(results, species):
+    """Largest relative mass and energy balance error over every step (upstream definitions,
+    postprocessing/reporting.py:113-140 with the validation/conservation_checks.py floors)."""
+    from dac_vtsa_amd import balances
+    mb_max = eb_max = 0.0
+    by_cycle = {}
+    for sr in results:
+        by_cycle.setdefault(sr.cycle, []).append(sr)
+    for steps in by_cycle.values():
+        mb, eb, _ = balances.check_cycle(steps, species)
+        mb_max = max([mb_max] + [abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE"])
+        eb_max = max([eb_max] + [abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE"])
+    return mb_max, eb_max
 
 
-def cpu_baseline(cfg, state, budget_s, sr):
-    """The oracle (NumPy/SciPy, the upstream algorithms) on 1 host thread, two windows of
-    budget_s / 2 wall seconds each: from the start of the heating step, and from the GPU
-    trajectory's snapshot nearest mid-step (with the GPU's step size there as the first step).
-    Returns [(t_start, simulated seconds, wall seconds, accepted steps)] per window."""
-    from oracle.model import OracleModel
-    from oracle.runner import make
-    from oracle.solver import OracleTRBDF2
-    g, gas = make(cfg)
-    step = cfg.cycle[STEP_INDEX]
-    dur = float(step["duration"])
-    mid = min(sr.snapshots, key=lambda sn: abs(sn["t_step"] - 0.5 * dur))
-    t_mid = float(mid["t_step"])
-    h_mid = min(sr.steps_taken, key=lambda th: abs(th[0] - t_mid))[1]
-    out = []
-    for t0, st, h0 in ((0.0, state, None), (t_mid, mid["state"], h_mid)):
-        m = OracleModel(cfg, g, gas, step, state)     # the step's model is built from its initial state
-        integ = OracleTRBDF2(m, cfg, h_max=cfg.solver.h_max_fraction * dur, h_init=h0)
-        w0 = time.time()
-        res = integ.integrate(m.pack(st), dur, np.array([dur]), stop_fn=lambda t, y: time.time() - w0,
-                              stop_value=0.5 * budget_s, t0=t0)
-        out.append((t0, res.t - t0, time.time() - w0, res.n_steps))
-    return out
+def solver_totals(results):
+    keys = ("steps", "rejected", "newton_fail", "jacobians", "lu", "gmres_iters", "nfev", "cons_projected",
+            "balance_unclosed", "unclosed_rejected", "guess_used", "guess_kept_predictor")
+    return {k: sum(int(sr.solver.get(k) or 0) for sr in results) for k in keys}
 
 
 def field_errors(st, ref, cfg):
-    """Per field (c_CO2, c_H2O, c_N2, q_CO2, q_H2O, T, Tw) against a reference end state: the max
-    relative difference where |ref| is above both 1e-3 of the field's maximum and atol, and the max difference
-    in integration-tolerance units |a - b| / (atol + rtol |b|) (atol of the configuration)."""
+    """Per field: the max relative difference where |ref| exceeds both 1e-3 of the field's maximum
+    and its absolute tolerance, and the max difference in integration-tolerance units
+    |a - b| / (atol + rtol |b|)."""
     s = cfg.solver
     fields = [("c_CO2", st.c[0], ref.c[0], s.atol_c_co2), ("c_H2O", st.c[1], ref.c[1], s.atol_c_h2o),
               ("c_N2", st.c[2], ref.c[2], s.atol_c_inert), ("q_CO2", st.q[0], ref.q[0], s.atol_q),
@@ -161,126 +142,196 @@ def field_errors(st, ref, cfg):
     return {"max_rel": rel, "max_tol_units": tol}
 
 
-def cycle_leg(cfg, grid, props, state, args):
-    """One full adsorption -> blowdown -> heating -> cooling cycle (30,860 s) on the same grid from
-    the same seeded state, plain FP64, through the cycle driver (vtsa_cycle.py:82-84 order)."""
+def timed_leg(name, cfg, grid, props, state, n_windows, kw):
     from dac_vtsa_amd import device as dv
-    from dac_vtsa_amd.cycle import run_cycle
-    from dac_vtsa_amd import balances
-    kw = {"integrator_kw": {"precision": "fp64", "gs_sweeps": args.sweeps, "conserve": args.conserve}}
     dv.synchronize()
     t0 = time.time()
-    _, steps, _ = run_cycle(state, cfg, grid, props, 1, **kw)
+    sim, results, ends = run_windows(cfg, grid, props, state, n_windows, kw)
     dv.synchronize()
     wall = time.time() - t0
-    sim = sum(s.duration for s in steps)
-    mb, eb, _ = balances.check_cycle(steps, props.species)
-    _log(f"cycle: {sim:.0f} s simulated in {wall:.2f} s")
+    mb, eb = balance_errors(results, props.species)
+    tot = solver_totals(results)
+    _log(f"{name}: {sim:.0f} s simulated in {wall:.2f} s ({sim / wall:.1f} sim-s/wall-s), {tot['steps']} steps, "
+         f"{tot['gmres_iters']} GMRES iterations, balances {mb:.2e} / {eb:.2e}")
+    per_step = {}
+    for sr in results:
+        d = per_step.setdefault(sr.name, {"sim_s": 0.0, "wall_s": 0.0, "accepted": 0, "gmres_iters": 0})
+        d["sim_s"] += sr.duration
+        d["wall_s"] += sr.wall_time
+        d["accepted"] += int(sr.solver["steps"])
+        d["gmres_iters"] += int(sr.solver.get("gmres_iters") or 0)
     return {"value": sim / wall, "unit": "sim-s/wall-s", "sim_s": sim, "wall_s": wall,
-            "max_mass_rel_err": max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE"),
-            "max_energy_rel_err": max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE"),
-            "steps": {s.name: {"sim_s": s.duration, "wall_s": s.wall_time, "accepted": s.solver["steps"],
-                               "gmres_iters": s.solver.get("gmres_iters")} for s in steps}}
+            "balance": {"max_mass_rel_err": mb, "max_energy_rel_err": eb}, "solver": tot,
+            "steps": per_step}, ends
 
 
-def batch_leg(args):
-    """Independent sensitivity cases (grid N_r 4-12, N_theta 1-16, N_z 20-100, T_heat, P_vacuum and
-    feed RH drawn per case) run concurrently, one HIP stream per worker thread: one heating step
-    of 22,400 s each from a seeded mid-cycle state.  value = total simulated seconds / wall."""
-    from dac_vtsa_amd.batch import draw_cases, run_batch
-    cases = draw_cases(args.batch, seed=0)
-    r = run_batch(cases, workers=args.workers, log=_log, mode="heating", state_kind="midcycle",
-                  base_overrides={"solver.conservation_tol": args.cons_tol},
-                  integrator_kw={"gs_sweeps": args.sweeps, "conserve": args.conserve})
-    ok = [c for c in r["cases"] if c["ok"]]
-    return {"value": r["value"], "unit": "sim-s/wall-s (all cases)", "cases": len(cases), "workers": args.workers,
-            "wall_s": r["wall_s"], "sim_s": r["sim_s"], "failed": r["failed"],
-            "cells_range": [min(c["cells"] for c in r["cases"]), max(c["cells"] for c in r["cases"])],
-            "max_mass_rel_err": max((c["max_mass_rel_err"] for c in ok), default=None),
-            "max_energy_rel_err": max((c["max_energy_rel_err"] for c in ok), default=None),
-            "projection_fallback_cases": sum(1 for c in ok if c.get("projection_fallback")),
-            "unclosed_stages": sum(c.get("balance_unclosed", 0) for c in ok)}
-
-
-def learned_leg(cfg, grid, props, state, args, st_plain, plain_value):
-    """Learned stage guess + FP32 preconditioner on the same heating step.  The network is trained
-    on the device from a plain FP64 run of another seeded state (``--train-seed``): that run
-    records every converged stage, 20 epochs of Adam follow, the output is shrunk per variable on
-    held-out stages; then the seed-0 step is timed with the guess and FP32 factors."""
+def train_network(args, base_kw, log=_log):
+    """Record the stages of a plain FP64 run of the cycle from the feed-equilibrium bed (every
+    ``--record-stride``-th stage) and train the stage network on the device."""
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.learned import StageDataset, StageNet
-    from dac_vtsa_amd.seeds import seeded_state
-    from dac_vtsa_amd.steps import run_step
-    step = cfg.cycle[STEP_INDEX]
-    dur = float(step["duration"])
-    n = grid.ncell
-    base = {"gs_sweeps": args.sweeps, "conserve": args.conserve}
-    ds = StageDataset(n, 700)
+    cfg, grid, props, st = setup(initial="feed_equilibrium")
+    ds = StageDataset(grid.ncell, args.record_stages, stride=args.record_stride)
     t0 = time.time()
-    run_step(seeded_state(grid, "midcycle", args.train_seed), cfg, grid, props, step,
-             integrator_kw={**base, "precision": "fp64", "record": ds})
+    # untimed: the first --warmup windows of this run are the bench's warm-up of the hot path
+    n_win = max(args.train_windows, args.warmup)
+    run_windows(cfg, grid, props, st, n_win, {**base_kw, "precision": "fp64", "record": ds})
     dv.synchronize()
     t_rec = time.time() - t0
-    net = StageNet(n, norm=ds.norm(), seed=0)
+    net = StageNet(grid.ncell, norm=ds.norm(), seed=0)
     hist = ds.train(net, epochs=args.epochs, lr=1e-3, lr_final=1e-4)
     ep, eb, eg = ds.errors(net, hist["val_ids"])
-    _log(f"learned: {ds.count} stages recorded in {t_rec:.2f} s, trained in {hist['wall_s']:.2f} s "
-         f"({hist['steps']} Adam steps), held-out val loss {hist['val'][-1]:.3g}")
-    kw = {**base, "precision": "fp32", "guess": net}
-    for _ in range(args.warmup):
-        run_step(state, cfg, grid, props, step, integrator_kw=kw)
+    log(f"learned: {ds.count} stages recorded in {t_rec:.2f} s, trained in {hist['wall_s']:.2f} s "
+        f"({hist['steps']} Adam steps), held-out val loss {hist['val'][-1]:.3g}")
+    info = {"initial_bed": "feed_equilibrium", "windows": n_win, "stages": ds.count,
+            "record_stride": args.record_stride, "rows": ds.count * grid.ncell, "record_s": t_rec,
+            "train_s": hist["wall_s"], "adam_steps": hist["steps"], "epochs": args.epochs,
+            "val_loss": hist["val"][-1],
+            "heldout_rms_err_tol_units": {"predictor": ep.mean(0).tolist(), "preconditioned_step": eb.mean(0).tolist(),
+                                          "learned_guess": eg.mean(0).tolist()}}
+    return net, info
+
+
+def residual_roofline(cfg, grid, props, state, n_launch=200):
+    """Average duration of one residual evaluation f(t, y) (k_props + k_flux), timed with HIP events
+    recorded on the stream it is launched on, and its algorithmic traffic (read y, write f)."""
+    import ctypes as C
+    from dac_vtsa_amd import _abi
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.residual import TVSAModel
+    m = TVSAModel(cfg, grid, props, cfg.cycle[2], state)
+    y = dv.to_dev(m.pack(state))
+    f = dv.empty(m.n_core)
+    m.rhs_dev(0.0, y, f)
     dv.synchronize()
+    ms = C.c_double()
+    _abi.call("vtsa_time_rhs", m.handle, 0.0, dv.ptr(y), dv.ptr(f), n_launch, C.byref(ms), dv.stream())
+    avg_s = ms.value / 1e3 / n_launch
+    algo = 96 * grid.ncell + 16 * m.n_w
+    m.close()
+    return {"kernels": "k_props + k_flux", "avg_launch_us": avg_s * 1e6, "algo_bytes_per_launch": algo,
+            "achieved_GBs": algo / avg_s / 1e9}
+
+
+def copy_probe(mib=64, repeat=5):
+    """Calibration of the HBM traffic counters: `repeat` device-to-device copies of `mib` MiB."""
+    from dac_vtsa_amd import device as dv
+    n = mib * (1 << 20) // 8
+    a, b = dv.zeros(n), dv.empty(n)
+    for _ in range(repeat):
+        b.copy_from(a)
+    dv.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_dispatch": n * 8, "dispatches": repeat}), flush=True)
+
+
+def _cpu_units(queue, budget_s):
+    """Child process: the oracle's unit costs on the workload, one host thread."""
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        from oracle.jacobian import OracleFDJacobian
+        from oracle.model import OracleModel
+        from oracle.runner import make, initial_state
+        cfg = make_cfg()
+        g, gas = make(cfg)
+        st = initial_state(cfg, g, gas)
+        m = OracleModel(cfg, g, gas, cfg.cycle[2], st)
+        y = m.pack(st)
+        out = {}
+        t0 = time.time()
+        f, _ = m.rhs(0.0, y, want_acc=False)
+        t0 = time.time()
+        for _ in range(3):
+            f, _ = m.rhs(0.0, y, want_acc=False)
+        out["rhs_s"] = (time.time() - t0) / 3
+        queue.put(dict(out))
+        t0 = time.time()
+        jac = OracleFDJacobian(m)
+        out["pattern_s"] = time.time() - t0
+        out["colours"] = jac.ngroups
+        t0 = time.time()
+        J = jac(0.0, y, f)
+        out["jacobian_s"] = time.time() - t0
+        queue.put(dict(out))
+        D = 1.0 - 1.0 / np.sqrt(2.0)
+        M = (sp.identity(J.shape[0], format="csc") - (D * 100.0) * jac.theta_block(J)).tocsc()
+        t0 = time.time()
+        out["splu_started"] = True
+        queue.put(dict(out))
+        lu = spla.splu(M, permc_spec="COLAMD")
+        out["splu_s"] = time.time() - t0
+        t0 = time.time()
+        lu.solve(f)
+        out["lu_solve_s"] = time.time() - t0
+        queue.put(dict(out))
+
+
+def cpu_baseline(budget_s, counts, sim_s):
+    """Oracle unit costs (one host thread, bounded by budget_s; a factorisation still running at
+    the budget is reported as at least that long) x the operation counts of the plain FP64 GPU
+    run, as an extrapolated CPU time for the same simulated window."""
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    p = ctx.Process(target=_cpu_units, args=(q, budget_s), daemon=True)
     t0 = time.time()
-    for _ in range(args.steps):
-        st, sr = run_step(state, cfg, grid, props, step, integrator_kw=kw)
-    dv.synchronize()
-    rate = args.steps * dur / (time.time() - t0)
-    mb, eb_ = balance_errors(sr, props.species)
-    _log(f"learned+fp32: {rate:.0f} sim-s/wall-s ({rate / plain_value:.2f}x plain fp64)")
-    return {
-        "value": rate, "unit": "sim-s/wall-s", "vs_plain_fp64": rate / plain_value,
-        "balance": {"max_mass_rel_err": mb, "max_energy_rel_err": eb_},
-        "field_diff_vs_plain_fp64": field_errors(st, st_plain, cfg),
-        "solver": {k: sr.solver.get(k) for k in ("steps", "rejected", "gmres_iters", "guess_used",
-                                                 "guess_kept_predictor")},
-        "training": {"train_seed": args.train_seed, "stages": ds.count, "rows": ds.count * n,
-                     "record_s": t_rec, "train_s": hist["wall_s"], "adam_steps": hist["steps"],
-                     "epochs": args.epochs, "val_loss": hist["val"][-1], "gain": hist["gain"],
-                     "heldout_rms_err_tol_units": {"predictor": ep.mean(0).tolist(),
-                                                   "preconditioned_step": eb.mean(0).tolist(),
-                                                   "learned_guess": eg.mean(0).tolist()}},
-    }
-
-
-def balance_errors(sr, species):
-    from dac_vtsa_amd import balances
-    mb, eb, _ = balances.check_cycle([sr], species)
-    return (max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE"),
-            max(abs(r["relative_error"]) for r in eb if r["step"] != "CYCLE"))
+    p.start()
+    units = {}
+    while time.time() - t0 < budget_s and (p.is_alive() or not q.empty()):
+        try:
+            units = q.get(timeout=1.0)
+        except Exception:
+            pass
+        if "lu_solve_s" in units:
+            break
+    elapsed = time.time() - t0
+    if p.is_alive():
+        p.terminate()
+        p.join(5)
+    if "splu_started" in units and "splu_s" not in units:
+        units["splu_s_lower_bound"] = elapsed - sum(units.get(k, 0.0) for k in ("pattern_s", "jacobian_s")) \
+            - 4 * units.get("rhs_s", 0.0)
+    rhs = units.get("rhs_s")
+    if rhs is None:
+        return None
+    lu = units.get("splu_s", units.get("splu_s_lower_bound", 0.0))
+    solve = units.get("lu_solve_s", 0.0)
+    est = (counts["nfev"] * rhs + counts["jacobians"] * units.get("jacobian_s", 90 * rhs)
+           + counts["lu"] * lu + (counts["gmres_iters"] + 2 * counts["steps"]) * solve + units.get("pattern_s", 0.0))
+    try:
+        cpu_model = next(line.split(":", 1)[1].strip() for line in open("/proc/cpuinfo") if line.startswith("model name"))
+    except Exception:
+        cpu_model = platform.processor()
+    lower = "splu_s" not in units
+    return {"value": sim_s / est, "unit": "sim-s/wall-s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (NumPy/SciPy restatement of the upstream algorithms) on 1 host thread of "
+                       f"'{cpu_model}' (nproc {os.cpu_count()}), unit costs measured in {elapsed:.0f} s on this "
+                       f"workload: rhs {rhs:.3f} s, coloured FD Jacobian {units.get('jacobian_s', float('nan')):.2f} s, "
+                       f"theta-block SuperLU factor {'>=' if lower else ''}{lu:.1f} s, LU solve "
+                       f"{solve:.3f} s; EXTRAPOLATED to {sim_s:.0f} simulated s with the plain FP64 GPU run's "
+                       f"counts ({counts['nfev']} rhs, {counts['jacobians']} Jacobians, {counts['lu']} "
+                       f"factorisations, {counts['gmres_iters']} GMRES iterations): {est:.0f} s"
+                       + (" (a lower bound: the factorisation did not finish within the budget)" if lower else "")),
+            "units": units}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
-    ap.add_argument("--cpu-sample-s", type=float, default=20.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--steps", type=int, default=20, help="windows of 1,543 simulated s (20 = one cycle)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=2, help="line Gauss-Seidel sweeps of the preconditioner")
-    ap.add_argument("--cons-tol", type=float, default=1e-5,
-                    help="solver.conservation_tol (upstream default 1e-3; the metric needs balances <= 1e-6)")
-    ap.add_argument("--rtol", type=float, default=None, help="solver.rtol (default: config)")
-    ap.add_argument("--conserve", choices=("polish", "project"), default="project",
-                    help="stage conservation: upstream polish, or the conservation projection first")
+    ap.add_argument("--krylov", type=int, default=60, help="GMRES restart length")
+    ap.add_argument("--cpu-budget-s", type=float, default=40.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-only leg")
+    ap.add_argument("--no-learned", action="store_true", help="value = FP32 factors without the learned guess")
+    ap.add_argument("--train-windows", type=int, default=20)
+    ap.add_argument("--record-stages", type=int, default=400)
+    ap.add_argument("--record-stride", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--copy-probe", action="store_true", help="only run the traffic-counter calibration copy")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess + FP32 preconditioner leg")
-    ap.add_argument("--no-cycle", action="store_true", help="skip the full-cycle leg")
-    ap.add_argument("--batch", type=int, default=0,
-                    help="also run this many independent sensitivity cases concurrently (heating step each)")
-    ap.add_argument("--workers", type=int, default=16, help="worker threads (HIP streams) of the batch leg")
-    ap.add_argument("--train-seed", type=int, default=1, help="seed of the state whose run trains the network")
-    ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -289,80 +340,71 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (independent cases would shard "
               "one group per GPU with no communication; no multi-GPU run is part of this build)", file=sys.stderr)
         return 2
-    from dac_vtsa_amd import device as dv
-    over = {}
-    if args.cons_tol is not None:
-        over["solver.conservation_tol"] = args.cons_tol
-    if args.rtol is not None:
-        over["solver.rtol"] = args.rtol
-    cfg, grid, props, state = setup(args.precision, over)
-    dur = float(cfg.cycle[STEP_INDEX]["duration"])
-    print(f"[bench] workload 10x16x50 heating {dur:.0f} s, precision {args.precision}", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        t0 = time.time()
-        one_step(cfg, grid, props, state, args.precision, args.sweeps, args.conserve)
-        print(f"[bench] warmup {i + 1}/{args.warmup}: {time.time() - t0:.2f} s", file=sys.stderr, flush=True)
     from dac_vtsa_amd.core import residual
+    base_kw = {"gs_sweeps": args.sweeps, "conserve": "project", "krylov_dim": args.krylov}
+    cfg, grid, props, bed = setup()
+    _log(f"workload {GRID[0]}x{GRID[1]}x{GRID[2]} cycle from the clean bed, {args.steps} windows of {WINDOW_S:.0f} s")
+    net, train = (None, None)
+    if args.no_learned:
+        cfg_w, grid_w, props_w, bed_w = setup(initial="feed_equilibrium")
+        t0 = time.time()
+        run_windows(cfg_w, grid_w, props_w, bed_w, args.warmup, {**base_kw, "precision": "fp64"})
+        _log(f"warm-up: {args.warmup} windows in {time.time() - t0:.2f} s")
+    else:
+        net, train = train_network(args, base_kw)
+    main_kw = {**base_kw, "precision": "fp32"}
+    if net is not None:
+        main_kw["guess"] = net
     residual.reset_kernel_profile(8)
-    dv.synchronize()
-    t0 = time.time()
-    results = []
-    for i in range(args.steps):
-        st_plain, sr = one_step(cfg, grid, props, state, args.precision, args.sweeps, args.conserve)
-        results.append(sr)
-        print(f"[bench] step {i + 1}/{args.steps}: {sr.solver['steps']} accepted steps, "
-              f"{sr.solver['rejected']} rejected, {sr.solver.get('gmres_iters', 0)} GMRES iterations",
-              file=sys.stderr, flush=True)
-    dv.synchronize()
-    elapsed = time.time() - t0
+    main, ends_main = timed_leg("learned+fp32" if net is not None else "fp32", cfg, grid, props, bed, args.steps,
+                                main_kw)
     prof = dict(residual.KERNEL_PROFILE)
     residual.reset_kernel_profile(0)
-    value = args.steps * dur / elapsed
-    mb, eb = balance_errors(results[-1], props.species)
     avg_s = prof["ms"] / 1e3 / max(prof["launches"], 1)
-    bytes_per_launch = prof["bytes"] / max(prof["launches"], 1)
-    achieved = bytes_per_launch / avg_s / 1e9
-    res_s, res_bytes = residual_roofline(cfg, grid, props, state)
+    bpl = prof["bytes"] / max(prof["launches"], 1)
+    achieved = bpl / avg_s / 1e9 if avg_s > 0 else 0.0
     line = {
-        "metric": "cycle sim-s/wall-s, 1 GPU: learned+FP32 vs plain FP64 at MB/EB err <=1e-6",
-        "value": value, "unit": "sim-s/wall-s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 preconditioner)",
-        "data": "synthetic seeded mid-cycle state (seed 0)",
-        "config": {"workload": "heating step 22400 s, grid 10x16x50 (48800 unknowns), P_vac 1.5e4 Pa",
-                   "preconditioner": f"zline_gs ({args.sweeps} sweep{'s' if args.sweeps > 1 else ''})", "precision": args.precision,
-                   "solver": "sdirk2 Newton-GMRES(30)", "conserve": args.conserve, "rtol": cfg.solver.rtol,
-                   "conservation_tol": cfg.solver.conservation_tol},
-        "balance": {"max_mass_rel_err": mb, "max_energy_rel_err": eb},
-        "solver": {k: results[-1].solver.get(k) for k in ("steps", "rejected", "newton_fail", "jacobians", "lu",
-                                                          "gmres_iters", "gmres_polish", "balance_unclosed",
-                                                          "cons_projected", "nfev")},
+        "metric": METRIC, "value": main["value"], "unit": "sim-s/wall-s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * main["wall_s"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (fp32 preconditioner factors)",
+        "data": "synthetic: clean bed (feed gas over bare sorbent); network trained on the simulator's own stages",
+        "config": {"workload": f"full TVSA cycle (adsorption 8000 s, blowdown 60 s, heating 22400 s, cooling 400 s) "
+                               f"on grid {GRID[0]}x{GRID[1]}x{GRID[2]} (387200 unknowns), clean bed",
+                   "step": f"{WINDOW_S:.0f} simulated s of the cycle schedule (20 steps = one cycle)",
+                   "form": "learned stage guess + FP32 preconditioner factors" if net is not None else "FP32 factors",
+                   "solver": f"sdirk2 Newton-GMRES({args.krylov}), line Gauss-Seidel ({args.sweeps} sweeps), projection",
+                   "rtol": cfg.solver.rtol, "conservation_tol": cfg.solver.conservation_tol},
+        "balance": main["balance"], "solver": main["solver"], "steps_detail": main["steps"],
         "roofline": {"bound": "hbm", "kernel": "k_gs_half (line Gauss-Seidel half-sweep)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "avg_launch_us": avg_s * 1e6, "algo_bytes_per_launch": bytes_per_launch,
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH, "avg_launch_us": avg_s * 1e6, "algo_bytes_per_launch": bpl,
                      "sampled_launches": prof["launches"]},
-        "residual_pair": {"kernels": "k_props + k_flux", "avg_launch_us": res_s * 1e6,
-                          "algo_bytes_per_launch": res_bytes, "achieved_GBs": res_bytes / res_s / 1e9},
     }
-    if not args.no_cycle:
-        line["cycle"] = cycle_leg(cfg, grid, props, state, args)
-    if args.batch > 0:
-        line["batch"] = batch_leg(args)
-    if not args.no_learned:
-        line["learned_fp32"] = learned_leg(cfg, grid, props, state, args, st_plain, value)
-    if not args.no_cpu:
-        from threadpoolctl import threadpool_limits
-        with threadpool_limits(1):
-            wins = cpu_baseline(cfg, state, args.cpu_sample_s, results[-1])
-        (_, s0, w0, n0), (tm, s1, w1, n1) = wins
-        g_wall = gpu_window(cfg, grid, props, state, s0, args.precision, args.sweeps, args.conserve)
-        line["cpu_baseline"] = {"value": s1 / w1, "unit": "sim-s/wall-s", "cores": 1, "kind": "port",
-                                "sample": f"oracle on 1 host thread, two windows of the same heating step: "
-                                          f"from t={tm:.0f} s (GPU snapshot, mid-step) {s1:.3g} s simulated in "
-                                          f"{w1:.1f} s ({n1} steps, this rate is the value); from t=0 {s0:.3g} s "
-                                          f"in {w0:.1f} s ({n0} steps, {s0 / w0:.3g} sim-s/wall-s; the GPU runs "
-                                          f"that first window in {g_wall:.3f} s)"}
+    if train is not None:
+        line["training"] = train
+    ends = {"main": ends_main}
+    if not args.no_plain:
+        plain, ends["plain"] = timed_leg("plain fp64", cfg, grid, props, bed, args.steps, {**base_kw, "precision": "fp64"})
+        line["plain_fp64"] = plain
+        line["vs_plain_fp64"] = main["value"] / plain["value"]
+    if not args.no_fp32 and net is not None:
+        fp32, ends["fp32"] = timed_leg("fp32", cfg, grid, props, bed, args.steps, {**base_kw, "precision": "fp32"})
+        line["fp32"] = fp32
+    if "plain" in ends and ends["plain"]:
+        key = min(len(e) for e in ends.values())
+        if key:
+            ref = ends["plain"][key - 1]
+            line["field_err_vs_plain_fp64"] = {"at": f"cycle {ref[0]} end of {ref[1]}"}
+            for nm in ("main", "fp32"):
+                if nm in ends:
+                    line["field_err_vs_plain_fp64"]["learned_fp32" if nm == "main" and net is not None else
+                                                     ("fp32" if nm == "fp32" else "value_run")] = \
+                        field_errors(ends[nm][key - 1][2], ref[2], cfg)
+    line["residual"] = residual_roofline(cfg, grid, props, bed)
+    if not args.no_cpu and "plain_fp64" in line:
+        cb = cpu_baseline(args.cpu_budget_s, line["plain_fp64"]["solver"], line["plain_fp64"]["sim_s"])
+        if cb is not None:
+            line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
     return 0
 

@@ -16,7 +16,7 @@ import numpy as np
 
 from .. import _abi
 from .. import device as dv
-from ..properties import R_GAS
+from ..properties import R_GAS, conductivity_terms, fuller_pair, viscosity_terms, wilke_tables
 from ..valves import KIND_CODE, end_conditions, feed_molar_flux, feed_profile, humid_air_composition
 
 LIMITERS = {"van_albada": 0, "van_albada_tvd": 1, "van_leer": 2, "minmod": 3, "upwind": 4}
@@ -195,21 +195,27 @@ class TVSAModel:
         sb = cfg.sorbent
         d.R, d.eps_b, d.eps_t, d.rho_b, d.cp_s, d.T_ref = R_GAS, sb.bed_voidage, sb.total_voidage, self.rho_b, \
             sb.cp_solid, cfg.thermo.T_ref
+        # only the attributes every upstream property backend has (species, cp, M) are read from
+        # props; the pure-component correlations and mixing-rule tables are this package's own
+        species = list(pr.species)
+        cp, M = np.asarray(pr.cp, dtype=float), np.asarray(pr.M, dtype=float)
+        visc, cond = viscosity_terms(species), conductivity_terms(species)
+        wden, wmq = wilke_tables(M)
         for s in range(3):
-            d.cpg[s], d.M[s] = pr.cp[s], pr.M[s]
-            f1, a1, b1, f2, a2, b2, two = pr.viscosity_terms()[s]
+            d.cpg[s], d.M[s] = cp[s], M[s]
+            f1, a1, b1, f2, a2, b2, two = visc[s]
             d.mu_f1[s], d.mu_a1[s], d.mu_b1[s], d.mu_f2[s], d.mu_a2[s], d.mu_b2[s], d.mu_two[s] = \
                 f1, a1, b1, f2, a2, b2, two
-            c1, n1, c2, n2 = pr.conductivity_terms()[s]
+            c1, n1, c2, n2 = cond[s]
             d.k_c1[s], d.k_n1[s], d.k_c2[s], d.k_n2[s] = c1, n1, c2, n2
             for t in range(3):
-                d.wilke_mq[s][t] = pr.mass_q[s, t]
-                d.wilke_den[s][t] = pr.wilke_den[s, t]
+                d.wilke_mq[s][t] = wmq[s, t]
+                d.wilke_den[s][t] = wden[s, t]
         th = cfg.thermo
         d.cpa[0], d.cpa[1] = th.cp_ads_co2, th.cp_ads_h2o
         d.dHa[0], d.dHa[1] = th.dH_ads_co2, th.dH_ads_h2o
-        d.fuller_sqrtM_air, d.fuller_s_air = pr.fuller_pair("CO2", "AIR")
-        d.fuller_sqrtM_h2o, d.fuller_s_h2o = pr.fuller_pair("CO2", "H2O")
+        d.fuller_sqrtM_air, d.fuller_s_air = fuller_pair("CO2", "AIR")
+        d.fuller_sqrtM_h2o, d.fuller_s_h2o = fuller_pair("CO2", "H2O")
         eps, dp = sb.bed_voidage, sb.particle_diameter
         d.lam_const, d.lam_s = tr.bed_conductivity_constant, sb.particle_conductivity
         d.zs_B = tr.zs_shape_factor * ((1.0 - eps) / eps) ** (10.0 / 9.0)

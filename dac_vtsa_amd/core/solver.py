@@ -255,9 +255,9 @@ def _unclosed(stats, z0, best, it, rate, new_lu, reject):
     return (z0, False, it + 1, rate, new_lu) if reject else (best[1], True, it + 1, rate, new_lu)
 
 
-def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_dim=30, eta=1e-2, counter=None,
-                  cons_tol=0.0, stats=None, trace=None, refresh=None, project=False, strict=False,
-                  reject_unclosed=False):
+def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_dim=30, eta=1e-2, counter=None,
+                      cons_tol=0.0, stats=None, trace=None, refresh=None, project=False, strict=False,
+                      reject_unclosed=False):
     """Inexact Newton-Krylov stage solve on the device (core/solver.py:154-294).
 
     ``project`` (no upstream counterpart): when the converged value misses the conservation
@@ -352,6 +352,112 @@ def newton_krylov(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, krylov_d
     return z, False, it + 1, 1.0, new_lu
 
 
+def newton_modified_dev(ops, t, z0, rhs_known, dh, rtol, max_iter, kappa, fail_rate=0.9, cons_tol=0.0):
+    """Modified Newton on the device with the handle's fixed preconditioner in the place of the
+    upstream stored LU (core/solver.py:111-151): z <- trunc(z - M^-1 R(z)), the step norm
+    rms((z_new - z_old) / s(z_new)), no acceptance on the first iteration, a contraction test
+    against fail_rate, convergence when rate/(1 - rate) |dz| <= kappa, and, with a conservation
+    tolerance, one more residual to test the balances before returning.
+
+    z0, rhs_known: device vectors.  Returns (z, ok, iterations, rate)."""
+    N = ops.N
+    z = z0.clone()
+    f, R, s, r = dv.empty(N), dv.empty(N), dv.empty(N), dv.empty(N)
+    zn, fn, Rn, diff, sn = dv.empty(N), dv.empty(N), dv.empty(N), dv.empty(N), dv.empty(N)
+    ops.scale(z, rtol, s)
+    rn = ops.newton_residual(t, z, dh, rhs_known, s, f, R)
+    nrm_prev = None
+    rate = 0.0
+    done = False
+    for it in range(max_iter + 2):
+        if not _finite(rn):
+            return z0, False, it, np.inf
+        if done:
+            if cons_ok(ops, cons_tol, R, ops.scale(z, rtol, s)) or it >= max_iter + 1:
+                return z, True, it, rate
+        elif it >= max_iter:
+            break
+        ops.prec_apply(R, r)                                # M^-1 R; the update is -M^-1 R
+        rn_new, _, _ = ops.newton_try(t, z, r, -1.0, dh, rhs_known, s, zn, fn, Rn)
+        ops.vec(7, diff, x=z, y=zn)
+        ops.scale(zn, rtol, sn)
+        nrm = ops.rms(diff, sn)
+        z, zn = zn, z
+        f, fn = fn, f
+        R, Rn = Rn, R
+        s, sn = sn, s
+        # the trial point's residual was formed with the previous scale; the next test uses R
+        # itself, and its norm only for the finiteness test of the next iteration
+        rn = rn_new
+        if not _finite(nrm):
+            return z0, False, it + 1, np.inf
+        if done:
+            continue
+        if nrm_prev is not None:
+            rate = nrm / max(nrm_prev, 1e-300)
+            if rate >= fail_rate:
+                return z, False, it + 1, rate
+            if rate / (1.0 - rate) * nrm <= kappa:
+                if cons_tol <= 0.0:
+                    return z, True, it + 1, rate
+                done = True
+        elif nrm < 1e-8:
+            if cons_tol <= 0.0:
+                return z, True, it + 1, 0.0
+            done = True
+        nrm_prev = nrm
+    return z, False, max_iter, rate
+
+
+def _upstream_args(model, lu, scale_fn, pos, cons):
+    """DeviceOps and rtol for the upstream-signature stage solvers, checking that what the caller
+    passes is what the device computes: scale_fn(y) = atol + rtol |y| with the model's atol, the
+    positivity set pos = all concentrations and loadings, cons = the model's conservation matrix,
+    and lu the handle's current device factorisation."""
+    if not isinstance(lu, Factorisation):
+        raise TypeError("lu must be the Factorisation of this model's device preconditioner (TRBDF2._factor)")
+    N = model.n_core
+    ones = np.ones(N)
+    s0, s1, s2 = (np.asarray(scale_fn(a * ones), dtype=float) for a in (0.0, 1.0, 2.0))
+    rt = s1 - s0
+    rtol = float(rt[0])
+    if not (np.allclose(s0, model.atol, rtol=1e-12, atol=0.0) and np.allclose(rt, rtol, rtol=1e-12, atol=0.0)
+            and np.allclose(s2 - s1, rt, rtol=1e-9, atol=0.0)):
+        raise ValueError("scale_fn must be atol + rtol |y| with the model's atol (the device forms it)")
+    if pos is not None and not np.array_equal(np.asarray(pos), np.arange(model.off_e)):
+        raise ValueError("the device truncates exactly the concentrations and loadings (pos = arange(off_e))")
+    if cons is not None:
+        W = model.conservation_matrix()
+        if cons.shape != W.shape or abs(cons - W).max() > 0.0:
+            raise ValueError("cons must be the model's conservation_matrix() (the device applies its own)")
+    return DeviceOps(model), rtol
+
+
+def newton_krylov(model, t, z0, rhs_known, dh, lu, scale_fn, max_iter, kappa, krylov_dim=30, eta=1e-2,
+                  counter=None, pos=None, cons=None, cons_tol=0.0, stats=None, trace=None, refresh=None):
+    """Upstream signature (core/solver.py:154-156): NumPy vectors in and out, the stage solved on the
+    device by newton_krylov_dev.  ``refresh(z)`` receives the iterate as a device vector and
+    returns the new Factorisation.  Returns (z, ok, iterations, rate, new_lu_or_None)."""
+    ops, rtol = _upstream_args(model, lu, scale_fn, pos, cons)
+    z, ok, it, rate, new_lu = newton_krylov_dev(ops, t, dv.to_dev(np.asarray(z0, dtype=float)),
+                                                dv.to_dev(np.asarray(rhs_known, dtype=float)), dh, lu, rtol, max_iter,
+                                                kappa, krylov_dim=krylov_dim, eta=eta, counter=counter,
+                                                cons_tol=cons_tol if cons is not None else 0.0, stats=stats,
+                                                trace=trace, refresh=refresh)
+    return dv.to_host(z).copy(), ok, it, rate, new_lu
+
+
+def newton_modified(model, t, z0, rhs_known, dh, lu, scale_fn, max_iter, kappa, fail_rate=0.9, pos=None, cons=None,
+                    cons_tol=0.0):
+    """Upstream signature (core/solver.py:111-112) over newton_modified_dev.  Returns (z, ok,
+    iterations, rate)."""
+    ops, rtol = _upstream_args(model, lu, scale_fn, pos, cons)
+    z, ok, it, rate = newton_modified_dev(ops, t, dv.to_dev(np.asarray(z0, dtype=float)),
+                                          dv.to_dev(np.asarray(rhs_known, dtype=float)), dh, rtol, max_iter, kappa,
+                                          fail_rate=fail_rate, cons_tol=cons_tol if cons is not None else 0.0)
+    return dv.to_host(z).copy(), ok, it, rate
+
+
 class TRBDF2:
     """Two-stage L-stable implicit integrator (core/solver.py:297-511) on the GPU.
 
@@ -385,11 +491,14 @@ class TRBDF2:
         self.stats = {"modified_fail": 0, "krylov_ok": 0, "krylov_fail": 0}
         self._gm = [0]
         self.scheme = s.scheme
+        if s.newton_mode not in ("krylov", "hybrid", "modified"):
+            raise ValueError(f"solver.newton_mode = {s.newton_mode!r}")
         self.mode = s.newton_mode
-        if self.mode != "krylov":
-            # the fixed-LU modified Newton needs an exact factorisation; with the approximate
-            # device preconditioner the Krylov stage solve is always used
-            self.mode = "krylov"
+        if self.mode == "modified":
+            # the device preconditioner is never an exact factorisation of I - D h J, so the upstream
+            # rule for an approximate (theta-block) factor applies: modified Newton with the
+            # Krylov fallback (core/solver.py:328-329)
+            self.mode = "hybrid"
         self.gmres_refresh = s.gmres_refresh
         self.log_refactor = np.log(s.refactor_ratio)
         self.cons_tol = float(s.conservation_tol or 0.0)
@@ -460,6 +569,16 @@ class TRBDF2:
         z_pred = z0
         self._recording = False
         z0 = self._start(t, z0, rhs_known, dh, y_ref, stage)
+        if self.mode == "hybrid":
+            # modified Newton first (4 iterations, contraction limit 0.5), Krylov on failure
+            # (core/solver.py:340-347)
+            z, ok, it, rate = newton_modified_dev(self.ops, t, z0, rhs_known, dh, self.rtol, 4, self.kappa,
+                                                  fail_rate=0.5, cons_tol=self.cons_tol)
+            if ok:
+                if self._recording:
+                    self.record.commit(self.m.handle, z, z_pred, self._gP, self._s_ref)
+                return z, ok, rate
+            self.stats["modified_fail"] += 1
 
         def refresh(zc):
             fc = dv.empty(self.ops.N)
@@ -471,7 +590,7 @@ class TRBDF2:
             return self._factor(zc, h)
 
         self._new_J = None
-        z, ok, it, rate, new_lu = newton_krylov(self.ops, t, z0, rhs_known, dh, lu, self.rtol, self.max_iter,
+        z, ok, it, rate, new_lu = newton_krylov_dev(self.ops, t, z0, rhs_known, dh, lu, self.rtol, self.max_iter,
                                                 self.kappa, krylov_dim=self.krylov_dim, counter=self._gm,
                                                 cons_tol=self.cons_tol, stats=self.stats,
                                                 refresh=refresh if (self.newton_refresh and not self._fresh) else None,

@@ -231,6 +231,7 @@ int vtsa_equilibrium(vtsa_handle* H, int32_t count, const double* pc, const doub
                      const double* qw, double* qc, double* qwe, vtsa_stream s) {
   if (!H || count < 0) { set_error("vtsa_equilibrium: bad argument"); return -1; }
   if (count == 0) return 0;
+  if (!pc || !pw || !T || !qw || !qc || !qwe) { set_error("vtsa_equilibrium: null argument"); return -1; }
   launch_equilibrium(H->h, count, pc, pw, T, qw, qc, qwe, S(s));
   return finish(S(s));
 }
@@ -373,7 +374,12 @@ int vtsa_inventories(vtsa_handle* H, const double* y, double* out, vtsa_stream s
 
 int vtsa_vec_op(vtsa_handle* H, int32_t op, double a, double b, double c, double e, const double* x, const double* y,
                 const double* z, const double* w, double* out, vtsa_stream st) {
-  if (!H || !out || op < 0 || op > 9) return -1;
+  if (!H || !out || op < 0 || op > 9) { set_error("vtsa_vec_op: bad argument"); return -1; }
+  // every op reads x and y; op 5 (the third-order quadrature) reads z and w as well
+  if (!x || !y || (op == 5 && (!z || !w))) {
+    set_error("vtsa_vec_op: null input for this op");
+    return -1;
+  }
   vec_op_impl(H->h, op, 5 * H->h.n, a, b, c, e, x, y, z, w, out, S(st));
   return finish(S(st));
 }

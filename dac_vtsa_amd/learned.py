@@ -120,8 +120,10 @@ class StageNet:
 class StageDataset:
     """Device-resident training rows of recorded stages on one grid (n cells per stage)."""
 
-    def __init__(self, n: int, capacity: int, n_in: int = 32):
+    def __init__(self, n: int, capacity: int, n_in: int = 32, stride: int = 1):
+        """capacity stages of n cells; every stride-th stage offered to ``begin`` is recorded."""
         self.n, self.capacity, self.n_in = int(n), int(capacity), n_in
+        self.stride, self.offered = max(1, int(stride)), 0
         self.feat = dv.empty(self.capacity * self.n * n_in, np.float32)
         self.targ = dv.empty(self.capacity * self.n * 8, np.float32)
         self.count = 0
@@ -132,7 +134,8 @@ class StageDataset:
     def begin(self, handle, y, z_pred, s, r, h, stage):
         """Features of the next stage (slot kept only if ``commit`` follows); r is the
         preconditioned stage residual M^-1 R(z_pred)."""
-        if self.count >= self.capacity:
+        self.offered += 1
+        if self.count >= self.capacity or (self.offered - 1) % self.stride:
             return False
         f, _ = self.slot(self.count)
         _abi.call("vtsa_mlp_features", handle, dv.ptr(y), dv.ptr(z_pred), dv.ptr(s), dv.ptr(r), float(np.log(h)),

@@ -31,6 +31,43 @@ def species_for(cfg):
     return ["CO2", "H2O", "N2"] if cfg.components.water else ["CO2", "N2"]
 
 
+def viscosity_terms(species):
+    """Per species: (f1, a1, b1, f2, a2, b2, two) with mu = f1 (a1 T + b1) [+ f2 (a2 T + b2)]."""
+    out = []
+    for s in species:
+        if s == "N2":
+            (c1, f1), (c2, f2) = AIR
+            out.append((f1, *MU_LIN[c1], f2, *MU_LIN[c2], 1))
+        else:
+            out.append((1.0, *MU_LIN[s], 0.0, 0.0, 0.0, 0))
+    return out
+
+
+def conductivity_terms(species):
+    """Per species: (c1, n1, c2, n2) with k = c1 (T/300)^n1 [+ c2 (T/300)^n2]."""
+    out = []
+    for s in species:
+        if s == "N2":
+            (a, fa), (b, fb) = AIR
+            out.append((fa * K_POW[a][0], K_POW[a][1], fb * K_POW[b][0], K_POW[b][1]))
+        else:
+            out.append((K_POW[s][0], K_POW[s][1], 0.0, 0.0))
+    return out
+
+
+def wilke_tables(M):
+    """(sqrt(8 (1 + M_i/M_j)), (M_j/M_i)^(1/4)) of Wilke's mixing rule from the molar masses."""
+    M = np.asarray(M, dtype=float)
+    Mi, Mj = M[:, None], M[None, :]
+    return np.sqrt(8.0 * (1.0 + Mi / Mj)), (Mj / Mi) ** 0.25
+
+
+def fuller_pair(a, b):
+    """(sqrt(M_ab), Sv_a^(1/3) + Sv_b^(1/3)) of the Fuller correlation."""
+    Mab = 2.0 / (1.0 / MOLAR_MASS_G[a] + 1.0 / MOLAR_MASS_G[b])
+    return float(np.sqrt(Mab)), DIFFUSION_VOLUME[a] ** (1 / 3) + DIFFUSION_VOLUME[b] ** (1 / 3)
+
+
 class GasProps:
     """Constant tables of the humid-air mixture ``[CO2, H2O, N2]``."""
 
@@ -44,9 +81,7 @@ class GasProps:
         self.M = np.array([MOLAR_MASS[s] for s in self.species])
         self.cp = np.array([self._cp1(s, cp_reference_T) for s in self.species])
         self.i_co2, self.i_h2o, self.i_inert = 0, 1, 2
-        Mi, Mj = self.M[:, None], self.M[None, :]
-        self.wilke_den = np.sqrt(8.0 * (1.0 + Mi / Mj))
-        self.mass_q = (Mj / Mi) ** 0.25
+        self.wilke_den, self.mass_q = wilke_tables(self.M)
 
     @staticmethod
     def _cp1(s, T):
@@ -58,32 +93,12 @@ class GasProps:
         return arden_buck(T)
 
     def viscosity_terms(self):
-        """Per species: (f1, a1, b1, f2, a2, b2, two) with mu = f1 (a1 T + b1) [+ f2 (a2 T + b2)]."""
-        out = []
-        for s in self.species:
-            if s == "N2":
-                (c1, f1), (c2, f2) = AIR
-                out.append((f1, *MU_LIN[c1], f2, *MU_LIN[c2], 1))
-            else:
-                out.append((1.0, *MU_LIN[s], 0.0, 0.0, 0.0, 0))
-        return out
+        return viscosity_terms(self.species)
 
     def conductivity_terms(self):
-        """Per species: (c1, n1, c2, n2) with k = c1 (T/300)^n1 [+ c2 (T/300)^n2]."""
-        out = []
-        for s in self.species:
-            if s == "N2":
-                (a, fa), (b, fb) = AIR
-                out.append((fa * K_POW[a][0], K_POW[a][1], fb * K_POW[b][0], K_POW[b][1]))
-            else:
-                out.append((K_POW[s][0], K_POW[s][1], 0.0, 0.0))
-        return out
+        return conductivity_terms(self.species)
 
-    @staticmethod
-    def fuller_pair(a, b):
-        """(sqrt(M_ab), Sv_a^(1/3) + Sv_b^(1/3)) of the Fuller correlation."""
-        Mab = 2.0 / (1.0 / MOLAR_MASS_G[a] + 1.0 / MOLAR_MASS_G[b])
-        return float(np.sqrt(Mab)), DIFFUSION_VOLUME[a] ** (1 / 3) + DIFFUSION_VOLUME[b] ** (1 / 3)
+    fuller_pair = staticmethod(fuller_pair)
 
 
 def get_backend(cfg, species=None):

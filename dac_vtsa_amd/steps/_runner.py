@@ -53,11 +53,14 @@ def _end_summary(model, d, end, c_feed_co2):
 
 def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle0: float = 0.0,
              T_jacket_prev: Optional[float] = None, logger=None, snapshots: Optional[int] = None,
-             record_outlet: bool = True, integrator_kw: Optional[Dict] = None, progress_s: float = 30.0):
+             record_outlet: bool = True, integrator_kw: Optional[Dict] = None, progress_s: float = 30.0,
+             stop_at: Optional[float] = None):
     """One cycle step on the GPU -> (state_out, StepResult) (steps/_runner.py:87-188).
 
-    ``record_outlet`` and ``integrator_kw`` (preconditioner kind/precision) have no upstream
-    counterpart; the defaults reproduce the upstream behaviour."""
+    ``record_outlet``, ``integrator_kw`` (preconditioner kind/precision) and ``stop_at`` (end the
+    step at the first accepted time step reaching this step time, as a stop criterion does; only
+    for steps without one of their own) have no upstream counterpart; the defaults reproduce the
+    upstream behaviour."""
     log = logger or (lambda *a, **k: None)
     model = TVSAModel(cfg, grid, props, step, state_in, T_jacket_prev)
     y0 = model.pack(state_in)
@@ -110,6 +113,8 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
     if stop:
         stop_fn, stop_val = model.stop_metric, stop["value"]
         direction = -1 if stop["criterion"] == "min_temperature" else 1
+    elif stop_at is not None:
+        stop_fn, stop_val = (lambda t, y: t), float(stop_at)
     wall0 = time.time()
     if cfg.solver.method != "trbdf2":
         raise RuntimeError(f"solver.method = {cfg.solver.method}: only the implicit trbdf2 path runs on the GPU")

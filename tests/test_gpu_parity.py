@@ -381,7 +381,7 @@ def test_newton_stage_matches_oracle(h):
     the iterates differ; the converged stage values agree to the Newton tolerance."""
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.core.jacobian import FDJacobian
-    from dac_vtsa_amd.core.solver import DeviceOps, newton_krylov
+    from dac_vtsa_amd.core.solver import DeviceOps, newton_krylov_dev as newton_krylov
     import oracle.solver as osv
     from oracle.jacobian import OracleFDJacobian
     import scipy.sparse as sp
@@ -413,6 +413,79 @@ def test_newton_stage_matches_oracle(h):
     sc = om.atol + 1e-4 * np.abs(zo)
     assert np.sqrt(np.mean(((dv.to_host(zg) - zo) / sc) ** 2)) < 1e-3
 
+
+
+def test_upstream_signature_stage_solvers():
+    """newton_krylov and newton_modified with the upstream signatures (core/solver.py:111-112,
+    154-156: model, NumPy vectors, lu, scale_fn, pos, cons): the Krylov wrapper returns exactly the
+    device-vector solve's result; modified Newton on the device preconditioner converges to the
+    stage value the oracle's Newton-Krylov reaches with the upstream theta-block SuperLU."""
+    from dac_vtsa_amd import device as dv
+    from dac_vtsa_amd.core.jacobian import FDJacobian
+    from dac_vtsa_amd.core.solver import (DeviceOps, Factorisation, newton_krylov, newton_krylov_dev,
+                                          newton_modified)
+    import oracle.solver as osv
+    from oracle.jacobian import OracleFDJacobian
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    z = golden("integ_cfg1.npz")
+    over = {**CFG1, "solver.newton_kappa": 1e-4}
+    st = state_from(z)
+    cfg, m = gpu_model(over, 0, st)
+    cfg, om = oracle_model(over, 0, st)
+    y = z["sdirk2/snap_y"][2]
+    t0 = float(z["sdirk2/snap_t"][2])
+    h = 0.5
+    dh = osv.D * h
+    t = t0 + osv.G_SD * h
+    ops = DeviceOps(m)
+    yd = dv.to_dev(y)
+    f0 = dv.empty(m.n_core)
+    m.rhs_dev(t0, yd, f0)
+    FDJacobian(m).evaluate(t0, yd, f0)
+    ops.precond_setup(dh, ops.scale(yd, 1e-4), 2, 0, 2)
+    lu = Factorisation(h)
+    scale = lambda v: m.atol + 1e-4 * np.abs(v)   # noqa: E731
+    pos, W = np.arange(m.off_e), m.conservation_matrix()
+    zk, ok, it, rate, _ = newton_krylov(m, t, y, y, dh, lu, scale, 7, 1e-4, pos=pos, cons=W, cons_tol=1e-3)
+    zd, okd, itd, rated, _ = newton_krylov_dev(ops, t, yd, yd, dh, lu, 1e-4, 7, 1e-4, cons_tol=1e-3)
+    assert ok and okd and it == itd
+    assert np.array_equal(zk, dv.to_host(zd))
+    zm, okm, itm, ratem = newton_modified(m, t, y, y, dh, lu, scale, 7, 1e-4, fail_rate=0.9, pos=pos, cons=W,
+                                          cons_tol=1e-3)
+    assert okm, (itm, ratem)
+    oj = OracleFDJacobian(om)
+    J = oj(t0, y)
+    olu = spla.splu((sp.identity(len(y), format="csc") - dh * oj.theta_block(J)).tocsc(), permc_spec="COLAMD")
+    zo, oko, _, _, _ = osv.newton_krylov(om, t, y, y, dh, olu, scale, 7, 1e-4, pos=pos, cons=om.conservation_matrix(),
+                                         cons_tol=1e-3)
+    assert oko
+    sc = om.atol + 1e-4 * np.abs(zo)
+    assert np.sqrt(np.mean(((zm - zo) / sc) ** 2)) < 1e-2
+    with pytest.raises(ValueError):
+        newton_krylov(m, t, y, y, dh, lu, lambda v: 2.0 * m.atol + 1e-4 * np.abs(v), 7, 1e-4)
+    with pytest.raises(TypeError):
+        newton_modified(m, t, y, y, dh, olu, scale, 7, 1e-4)
+
+
+def test_hybrid_mode_follows_the_golden_trajectory():
+    """solver.newton_mode = hybrid (core/solver.py:339-368): modified Newton on the device
+    preconditioner, Newton-Krylov when it fails; 10 s of adsorption from the seeded 4x4x20 state
+    against the upstream hybrid run (whose theta-block factor is approximate too): both branches
+    run and the end state is within the controller's tolerance of the upstream one."""
+    from dac_vtsa_amd.core.solver import TRBDF2
+    zh = golden("integ_cfg1_hybrid.npz")
+    st = state_from(golden("integ_cfg1.npz"))
+    cfg, m = gpu_model({**CFG1, "solver.newton_mode": "hybrid"}, 0, st)
+    y0 = m.pack(st)
+    integ = TRBDF2(m, cfg, h_max=4.0)
+    assert integ.mode == "hybrid"
+    res = integ.integrate(y0, 10.0, np.linspace(0, 10.0, 3))
+    assert integ.stats["modified_fail"] > 0 and integ.stats["krylov_ok"] > 0
+    assert res.n_krylov > 0
+    sc = m.atol + 1e-4 * np.abs(zh["y"])
+    e = (res.y - zh["y"]) / sc
+    assert np.sqrt(np.mean(e * e)) < 1.0, np.sqrt(np.mean(e * e))
 
 
 def test_newton_try_step_flags_a_nonfinite_direction():
@@ -459,7 +532,7 @@ def test_conservation_projection_closes_the_totals():
     at most one tolerance unit (rms over s) and keeps c and q at >= 0.1 of their values."""
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.core.jacobian import FDJacobian
-    from dac_vtsa_amd.core.solver import DeviceOps, newton_krylov, project_conservation
+    from dac_vtsa_amd.core.solver import DeviceOps, newton_krylov_dev as newton_krylov, project_conservation
     import oracle.solver as osv
     z = golden("integ_cfg1.npz")
     cfg, m = gpu_model(CFG1, 0, state_from(z))

@@ -44,6 +44,29 @@ def test_descriptor_layout_matches_header():
     assert vals[1:] == [getattr(_abi.ModelDesc, f).offset for f in fields]
 
 
+def test_null_inputs_are_rejected_before_any_device_access():
+    """vtsa_vec_op and vtsa_equilibrium return -1 with a message for null inputs; the handle is a
+    dummy non-null pointer that the checks must not dereference."""
+    from dac_vtsa_amd import _abi
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libvtsa.so not built (run __graft_entry__.build())")
+    L = _abi.lib()
+    dummy = C.create_string_buffer(64)
+    H = C.cast(dummy, C.c_void_p)
+    buf = (C.c_double * 4)()
+    p = C.cast(buf, C.c_void_p)
+    vec = L.vtsa_vec_op
+    for op, args in ((0, (None, p, None, None)), (7, (p, None, None, None)), (5, (p, p, None, p)),
+                     (5, (p, p, p, None))):
+        assert vec(H, op, C.c_double(1.0), C.c_double(0.0), C.c_double(0.0), C.c_double(0.0), *args, p, None) == -1
+        assert b"vtsa_vec_op" in L.vtsa_last_error()
+    for k in range(6):
+        ptrs = [p] * 6
+        ptrs[k] = None
+        assert L.vtsa_equilibrium(H, 4, *ptrs, None) == -1
+        assert b"null argument" in L.vtsa_last_error()
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "dac_vtsa_amd")
     for dirpath, _, files in os.walk(pkg):
@@ -168,3 +191,33 @@ def test_buffer_cache_finalises_inside_its_own_critical_section(monkeypatch):
     b = dv.DeviceArray(12345)     # the cached block is reused
     assert b.ptr.value == ptr
     b._p = None                   # nothing to return to a cache the test is about to drop
+
+
+# ------------------------------------------------------------------ drop-in boundary: upstream property backend
+class _UpstreamLikeBackend:
+    """Only the public attributes of the upstream IdealGasBackend that a caller of TVSAModel can rely
+    on (properties/ideal_gas_backend.py:54-80): species, ng, M, cp and p_sat."""
+
+    def __init__(self, species, cp, M):
+        self.species, self.ng, self.cp, self.M = list(species), len(species), np.array(cp), np.array(M)
+
+    def p_sat(self, T):
+        from dac_vtsa_amd.properties import arden_buck
+        return arden_buck(T)
+
+
+def test_descriptor_builds_from_an_upstream_like_property_backend(monkeypatch):
+    from dac_vtsa_amd.core import residual
+    from dac_vtsa_amd.core.grid import Grid
+    from dac_vtsa_amd.core.state import initial_state
+    from dac_vtsa_amd.properties import get_backend
+    monkeypatch.setattr(residual.TVSAModel, "_create", lambda self: None)
+    cfg = load_config(override_dict={"grid.Nr": 3, "grid.Ntheta": 4, "grid.Nz": 6})
+    g = Grid.from_config(cfg)
+    ours = get_backend(cfg)
+    st = initial_state(cfg, g, ours)
+    up = _UpstreamLikeBackend(ours.species, ours.cp, ours.M)
+    for k in range(4):
+        a = residual.TVSAModel(cfg, g, ours, cfg.cycle[k], st).desc
+        b = residual.TVSAModel(cfg, g, up, cfg.cycle[k], st).desc
+        assert bytes(a) == bytes(b)
