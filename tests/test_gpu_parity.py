@@ -451,7 +451,7 @@ def test_upstream_signature_stage_solvers():
     zd, okd, itd, rated, _ = newton_krylov_dev(ops, t, yd, yd, dh, lu, 1e-4, 7, 1e-4, cons_tol=1e-3)
     assert ok and okd and it == itd
     assert np.array_equal(zk, dv.to_host(zd))
-    zm, okm, itm, ratem = newton_modified(m, t, y, y, dh, lu, scale, 7, 1e-4, fail_rate=0.9, pos=pos, cons=W,
+    zm, okm, itm, ratem = newton_modified(m, t, y, y, dh, lu, scale, 25, 1e-4, fail_rate=0.9, pos=pos, cons=W,
                                           cons_tol=1e-3)
     assert okm, (itm, ratem)
     oj = OracleFDJacobian(om)
