@@ -3,6 +3,7 @@
 // Reductions use one launch: per-block partial sums, then the last block to finish (ticket
 // counter) adds the partials in a fixed order, so results are deterministic.
 #include <chrono>
+#include <mutex>
 #include <thread>
 
 #include "vtsa_internal.h"
@@ -745,3 +746,63 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
 }
 
 }  // namespace vtsa
+
+// ------------------------------------------------------------------ no-allocation rms
+// vtsa_rms_noalloc: the reduction of vtsa_rms (same kernel, same grid, so the same bits) with its
+// per-block partials and ticket counter in a file-scope device array instead of a handle's
+// buffers, and the result written to a caller-owned device double.  Each device holds its own
+// copy of the array; its address is looked up once per device (hipGetSymbolAddress) with that
+// device current, and only that address is handed to kernels.  The counter must start at zero:
+// the first call on a device clears the array with a fill kernel on the caller's stream; every
+// later reduction leaves the counter at zero again (its last block resets it).
+namespace vtsa {
+namespace {
+constexpr int NOALLOC_PARTIALS = RMAXB;                     // one double per block
+constexpr int NOALLOC_DOUBLES = NOALLOC_PARTIALS + 32;      // + the ticket counter, padded
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+std::mutex noalloc_mu;
+double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+__global__ void k_noalloc_clear(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// Device address of this device's scratch array; cleared on first use.  nullptr on error.
+double* noalloc_scratch(hipStream_t st) {
+  int dev = -1;
+  if (check(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("vtsa_rms_noalloc: device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    double* p = nullptr;
+    if (check(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return nullptr;
+    hipLaunchKernelGGL(k_noalloc_clear, dim3((NOALLOC_DOUBLES + 255) / 256), dim3(256), 0, st, p, NOALLOC_DOUBLES);
+    if (check(hipGetLastError(), "clear")) return nullptr;
+    noalloc_ptr[dev] = p;
+  }
+  return noalloc_ptr[dev];
+}
+}  // namespace
+
+}  // namespace vtsa
+
+extern "C" int vtsa_rms_noalloc(const double* a, const double* s, int32_t len, double* out, vtsa_stream st) {
+  using namespace vtsa;
+  if (!a || !out) { set_error("vtsa_rms_noalloc: null argument"); return -1; }
+  if (len < 1 || len > VTSA_NOALLOC_MAX_LEN) {
+    set_error("vtsa_rms_noalloc: len must be in 1 .. VTSA_NOALLOC_MAX_LEN (2^30)");
+    return -2;
+  }
+  hipStream_t hs = (hipStream_t)st;
+  double* p = noalloc_scratch(hs);
+  if (!p) return -1;
+  static_assert(sizeof(unsigned) <= sizeof(double), "counter slot");
+  hipLaunchKernelGGL(k_rms, dim3(rgrid(len)), dim3(RBLK), 0, hs, (int)len, a, s, p,
+                     reinterpret_cast<unsigned*>(p + NOALLOC_PARTIALS), out);
+  return check(hipGetLastError(), "vtsa_rms_noalloc launch") ? -1 : 0;
+}

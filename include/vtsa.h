@@ -142,6 +142,16 @@ int vtsa_gmres(vtsa_handle* h, double t, const double* z, const double* f, const
 int vtsa_scale(vtsa_handle* h, const double* z, double rtol, double* s, vtsa_stream st);
 /* out[0] = sqrt(mean((a/s)^2)) (s NULL: plain rms) */
 int vtsa_rms(vtsa_handle* h, const double* a, const double* s, int32_t len, double* out, vtsa_stream st);
+/* No counterpart in the upstream code; for callers that cannot allocate device memory during
+ * the call (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * *out (a DEVICE double) = sqrt(mean((a/s)^2)) over len entries (s NULL: plain rms), the same
+ * bits as vtsa_rms.  No handle, no hipMalloc/hipFree: the reduction's per-block partials and
+ * ticket counter live in a fixed file-scope device array of 4.3 KB per GPU, held until the process
+ * ends.  len must be in 1 .. VTSA_NOALLOC_MAX_LEN (else -2).  Asynchronous on st; the first call
+ * on a device looks the array up and clears it (do that call before any capture).  Calls that
+ * overlap on one GPU (two streams, or two threads) race on the shared array. */
+#define VTSA_NOALLOC_MAX_LEN (1 << 30)
+int vtsa_rms_noalloc(const double* a, const double* s, int32_t len, double* out, vtsa_stream st);
 /* f = f(t,z); R = z - dh f - rhs_known; rn = rms(R/s); pn = rms(M^-1 R / s) */
 int vtsa_newton_eval(vtsa_handle* h, double t, const double* z, double dh, const double* rhs_known,
                      const double* s, double* f, double* R, double* rn, double* pn, vtsa_stream st);
