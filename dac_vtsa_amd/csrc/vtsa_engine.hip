@@ -98,9 +98,14 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   int rc = 0;
   {
     int dev = 0, v = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
-      h.lds_max = v;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+        h.lds_max = v;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && v > 0)
+        h.lds_per_cu = v;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+        h.n_cu = v;
+    }
     (void)hipGetLastError();
   }
   rc |= upload(&h.V, V, Nr);
@@ -473,7 +478,7 @@ int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {
   if (!H) { set_error("vtsa_set_option: null handle"); return -1; }
   switch (option) {
     case VTSA_OPT_GS_WHOLE_LINES:
-      if (value < 0 || value > 1) { set_error("vtsa_set_option: staging level must be 0 or 1"); return -2; }
+      if (value < 0 || value > 4) { set_error("vtsa_set_option: staging level must be 0 .. 4"); return -2; }
       H->h.gs_full = value;
       return 0;
     default: set_error("vtsa_set_option: unknown option"); return -2;

@@ -68,6 +68,8 @@ struct Handle {
   int gs_sweeps = 1;
   int gs_full = 1;                   // LDS staging of a half-sweep: 0 segments, 1 whole line
   int lds_max = 65536;               // LDS bytes one block may use on this device
+  int lds_per_cu = 65536;            // LDS bytes of one CU
+  int n_cu = 256;                    // compute units of the device
   double* gsb = nullptr;             // [N] scaled right-hand side
   double* gsu = nullptr;             // [N] unscaled iterate
   std::vector<int> gs_ncell, gs_nouter, gs_nwall;   // per colour: cell lines, outer-ring lines, wall lines

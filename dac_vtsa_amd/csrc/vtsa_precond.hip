@@ -10,6 +10,8 @@
 //    lines) so that the r and theta couplings are taken into account as well.
 #include <mutex>
 
+#include <algorithm>
+
 #include "vtsa_internal.h"
 #include "vtsa_reduce.h"
 
@@ -449,10 +451,13 @@ __device__ inline void zline_sweeps(const T* __restrict__ F, int X0, int Nz, dou
 // so each step is one 12-wide dot product per row (rows in lanes 0..11), half as many dependent
 // steps as cell by cell, and b', h and z' are data-parallel passes of the whole block.  A line
 // with odd Nz ends in a pair whose second cell is a zero phantom.  Per pair, PP values:
-// G (12 x 12) | L1_{2K+1} | H (12 x 12) | Dinv_{2K} | Dinv_{2K+1} | ^U1_{2K}, rows of 12 or 6.
+// G (12 x 12) | H (12 x 12) | L1_{2K+1} | Dinv_{2K} | Dinv_{2K+1} | ^U1_{2K}, rows of 12 or 6; the
+// two recurrence matrices come first, so that a line whose whole factor does not fit the LDS
+// budget can stage only them (PGH values per pair) and read the rest from global memory.
 constexpr int GSW = 512;   // threads of a whole-line half-sweep block
 constexpr int PP = 432;
-constexpr int PO_G = 0, PO_C = 144, PO_H = 180, PO_D = 324, PO_U = 396;
+constexpr int PGH = 288;
+constexpr int PO_G = 0, PO_H = 144, PO_C = 288, PO_D = 324, PO_U = 396;
 
 __host__ __device__ constexpr int npairs(int Nz) { return (Nz + 1) / 2; }
 // xs and zs, [2 npairs + 1][6] each (the last cell a spare slot)
@@ -464,6 +469,22 @@ __host__ __device__ constexpr size_t pair_buf_bytes(int Nz) {   // whole copies 
 template <typename T>
 __host__ __device__ constexpr size_t gs_full_lds_bytes(int Nz) {
   return sizeof(double) * 2 * pair_vec_doubles(Nz) + pair_buf_bytes<T>(Nz);
+}
+template <typename T>
+__host__ __device__ constexpr size_t pair_gh_buf_bytes(int Nz) {   // whole copies of GSW x 16 bytes
+  return ((size_t)npairs(Nz) * PGH * sizeof(T) + GSW * 16 - 1) / (GSW * 16) * (GSW * 16);
+}
+template <typename T>
+__host__ __device__ constexpr size_t gs_gh_lds_bytes(int Nz) {
+  return sizeof(double) * 2 * pair_vec_doubles(Nz) + pair_gh_buf_bytes<T>(Nz);
+}
+template <typename T>
+__host__ __device__ constexpr size_t pair_g_buf_bytes(int Nz) {   // one 12 x 12 matrix per pair
+  return ((size_t)npairs(Nz) * 144 * sizeof(T) + GSW * 16 - 1) / (GSW * 16) * (GSW * 16);
+}
+template <typename T>
+__host__ __device__ constexpr size_t gs_g_lds_bytes(int Nz) {
+  return sizeof(double) * 2 * pair_vec_doubles(Nz) + pair_g_buf_bytes<T>(Nz);
 }
 
 size_t gs_pairs_count(int ncol, int Nz) { return (size_t)ncol * npairs(Nz) * PP; }
@@ -539,6 +560,27 @@ __device__ inline void stage_pairs(const T* __restrict__ P, int L, int NP, T* ds
   }
 }
 
+// Copy values [OFF, OFF + CNT) of each of the NP pairs of line L into dst, packed at CNT values
+// per pair: 16-byte chunks, GSW per pass, straight into LDS; chunk c of the packed run comes from
+// pair c / cpp, chunk c % cpp of that pair.  Lanes past the end re-read the last chunk.
+template <typename T, int OFF, int CNT>
+__device__ inline void stage_pairs_part(const T* __restrict__ P, int L, int NP, T* dst) {
+  constexpr int cpp = CNT * (int)sizeof(T) / 16;
+  static_assert(CNT * sizeof(T) % 16 == 0 && OFF * sizeof(T) % 16 == 0, "16-byte chunks");
+  const int total = NP * cpp;
+  const int tid = threadIdx.x, w = tid >> 6;
+  const char* base = reinterpret_cast<const char*>(P + (size_t)L * NP * PP + OFF);
+  char* d = reinterpret_cast<char*>(dst);
+  for (int j = 0; j * GSW < total; ++j) {
+    const int c = j * GSW + tid;
+    const int cc = c < total ? c : total - 1;
+    const int K = cc / cpp, q = cc - K * cpp;
+    __builtin_amdgcn_global_load_lds(base + ((size_t)K * PP * sizeof(T) + (size_t)q * 16),
+                                     (__attribute__((address_space(3))) void*)(d + (size_t)j * GSW * 16 + w * 1024),
+                                     16, 0, 0);
+  }
+}
+
 // b - sum_{c<12} g_c v_c, v_c taken from lane c of the lane's 16-lane row (DPP row_newbcast), as
 // (b - g0 v0 - g1 v1 - .. - g5 v5) + (-g6 v6 - .. - g11 v11): two interleaved chains, so no FMA
 // waits for the one before it and a single add joins them.
@@ -566,12 +608,12 @@ __device__ inline double bsub_pair(double b, double v, const double (&g)[12]) {
 // rows, then this pair's store; the look-ahead index is clamped rather than tested, the
 // scheduler may not move the reads (sched_barrier), and every path into a pair has one store
 // between its rows and the next pair's, so a pair waits for its own rows only.
-template <typename T, bool FORWARD>
+template <typename T, bool FORWARD, int STRIDE>
 __device__ inline void pair_recurrence(const T* Pl, int NP, int off, const double* rhs, double* out) {
   const int rho = threadIdx.x;
   struct Row { double g[12]; double b; };
   auto ld = [&](int K, Row& w) {
-    const T* R = Pl + (size_t)K * PP + off + rho * 12;
+    const T* R = Pl + (size_t)K * STRIDE + off + rho * 12;
 #pragma unroll
     for (int c = 0; c < 12; ++c) w.g[c] = (double)R[c];
     w.b = rhs[K * 12 + rho];
@@ -807,6 +849,12 @@ __global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restr
 //         start (stage_pairs), spread over eight wavefronts so that no wavefront stalls on its
 //         count of outstanding copies, the right-hand side is gathered by all of them, and the
 //         first wavefront runs the pair recurrences between data-parallel passes.
+// MODE 2: as MODE 1 with only G and H staged (2/3 of the LDS); the data-parallel passes read
+//         L1, Dinv and ^U1 from global memory.  For lines whose whole factor does not fit, or
+//         when staging it would leave fewer blocks resident per CU than a launch has lines.
+// MODE 3: as MODE 2 with G staged at the start and H copied over it after the forward
+//         recurrence (1/3 of the LDS), the copies in flight during the two passes between the
+//         recurrences.
 // Both solve the same line systems; the roundings differ.
 // The apply's outputs, on a line's last visit: out = U / post for the line's unknowns, and the
 // rms of out / red_s over all unknowns (the sum of squares per line into partial[L]; the last
@@ -877,11 +925,13 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     const int i = L / Nth, j = L - (L / Nth) * Nth;
     const int X0 = L * Nz;
     const int NP = npairs(Nz);
-    double* zs = lds + pair_vec_doubles(Nz);                                  // MODE 1
-    T* pb = reinterpret_cast<T*>(lds + 2 * pair_vec_doubles(Nz));             // MODE 1
+    double* zs = lds + pair_vec_doubles(Nz);                                  // MODE 1, 2
+    T* pb = reinterpret_cast<T*>(lds + 2 * pair_vec_doubles(Nz));             // MODE 1, 2
     T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);                  // MODE 0
     T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
-    if (MODE >= 1) stage_pairs<T>(Fac, L, NP, pb);
+    if (MODE == 1) stage_pairs<T>(Fac, L, NP, pb);
+    else if (MODE == 2) stage_pairs_part<T, PO_G, PGH>(Fac, L, NP, pb);
+    else if (MODE == 3) stage_pairs_part<T, PO_G, 144>(Fac, L, NP, pb);
     else zline_stage_first<T>(Fac, X0, Nz, fb0);
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
@@ -922,25 +972,32 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       return gs_b(a, g.s0, g.bs, idx, first_visit) + (dh / g.s0[idx]) * acc;
     };
     for (int e = lane; e < 6 * Nz; e += NT) xs[(e % Nz) * 6 + e / Nz] = rhs(e);
-    if (MODE == 1) {
+    if (MODE >= 1) {
       if (lane < 6 * (2 * NP - Nz)) xs[Nz * 6 + lane] = 0.0;   // phantom cell of an odd line
+      // the data-parallel passes read L1, Dinv and ^U1 from the staged line (MODE 1) or from
+      // global memory (MODE 2: G and H staged; MODE 3: G, then H in its place); the recurrences
+      // read G and H from LDS
+      const T* pd = MODE == 1 ? pb : Fac + (size_t)L * NP * PP;
       stage_wait();
       // b'_{2K+1} = b_{2K+1} - L1_{2K+1} b_{2K}
       for (int e = lane; e < NP * 6; e += NT) {
         const int K = e / 6, r = e - K * 6;
-        const T* C = pb + (size_t)K * PP + PO_C + r * 6;
+        const T* C = pd + (size_t)K * PP + PO_C + r * 6;
         double v = xs[K * 12 + 6 + r];
 #pragma unroll
         for (int c = 0; c < 6; ++c) v = fma(-(double)C[c], xs[K * 12 + c], v);
         xs[K * 12 + 6 + r] = v;
       }
       lds_barrier();
-      if (lane < 12) pair_recurrence<T, true>(pb, NP, PO_G, xs, xs);
+      constexpr int SG = MODE == 1 ? PP : MODE == 2 ? PGH : 144;   // LDS stride of a pair's matrices
+      if (lane < 12) pair_recurrence<T, true, SG>(pb, NP, PO_G, xs, xs);
       lds_barrier();
+      // MODE 3: H replaces G in the buffer; the copies run under the two passes below
+      if (MODE == 3) stage_pairs_part<T, PO_H, 144>(Fac, L, NP, pb);
       // h_k = Dinv_k z_k
       for (int e = lane; e < NP * 12; e += NT) {
         const int K = e / 12, rr = e - K * 12, hh = rr / 6;
-        const T* D = pb + (size_t)K * PP + PO_D + rr * 6;
+        const T* D = pd + (size_t)K * PP + PO_D + rr * 6;
         double v = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) v = fma((double)D[c], xs[K * 12 + hh * 6 + c], v);
@@ -950,14 +1007,15 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       // z'_{2K} = h_{2K} - ^U1_{2K} h_{2K+1}
       for (int e = lane; e < NP * 6; e += NT) {
         const int K = e / 6, r = e - K * 6;
-        const T* U = pb + (size_t)K * PP + PO_U + r * 6;
+        const T* U = pd + (size_t)K * PP + PO_U + r * 6;
         double v = zs[K * 12 + r];
 #pragma unroll
         for (int c = 0; c < 6; ++c) v = fma(-(double)U[c], zs[K * 12 + 6 + c], v);
         zs[K * 12 + r] = v;
       }
+      if (MODE == 3) stage_wait();
       lds_barrier();
-      if (lane < 12) pair_recurrence<T, false>(pb, NP, PO_H, zs, xs);
+      if (lane < 12) pair_recurrence<T, false, SG>(pb, NP, MODE == 3 ? 0 : PO_H, zs, xs);
       lds_barrier();
     } else {
       zline_sweeps<T>(Fac, X0, Nz, xs, fb0, fb1);
@@ -1141,7 +1199,7 @@ static double gs_bytes(const Handle& h, int c, bool first, size_t fsz, int mode,
                        const ApplyArgs& a) {
   const double Nz = h.desc.Nz;
   const double cells = h.gs_ncell[c] * Nz, outer = h.gs_nouter[c] * Nz, walls = h.gs_nwall[c] * Nz;
-  const double per_line = (mode == 1 ? (double)npairs(h.desc.Nz) * PP : Nz * 180.0) * fsz;
+  const double per_line = (mode >= 1 ? (double)npairs(h.desc.Nz) * PP : Nz * 180.0) * fsz;
   const double per_unknown = (fv ? 8.0 * (1 + (a.pre ? 1 : 0) + (a.store_in ? 1 : 0)) : 0.0) +
                              (fin ? 8.0 * (1 + (a.post ? 1 : 0) + (a.red_s ? 1 : 0)) : 0.0);
   const double per_cell = 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24)) + 6.0 * per_unknown;
@@ -1183,7 +1241,8 @@ int prof_resolve(Handle& h) {
 
 template <typename T, int MODE>
 static size_t gs_lds_bytes(int Nz) {
-  return MODE == 1 ? gs_full_lds_bytes<T>(Nz) : zline_lds_bytes<T>(Nz);
+  return MODE == 1 ? gs_full_lds_bytes<T>(Nz) : MODE == 2 ? gs_gh_lds_bytes<T>(Nz)
+                   : MODE == 3 ? gs_g_lds_bytes<T>(Nz) : zline_lds_bytes<T>(Nz);
 }
 
 // Whether half-sweep variant MODE can run: its LDS fits the device's limit for one block and the
@@ -1209,9 +1268,29 @@ static bool gs_mode_ok(const Handle& h) {
 template <typename T>
 static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   const int N = h.N, Nz = h.desc.Nz;
-  // whole-line staging when the option allows it and the device grants the LDS
-  const int mode = (h.gs_full >= 1 && gs_mode_ok<T, 1>(h)) ? 1 : 0;
-  const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds0 = gs_lds_bytes<T, 0>(Nz);
+  // Whole-line staging when the option allows it and the device grants the LDS: the whole pair
+  // factor (MODE 1) while the blocks a CU can hold at that LDS size cover the widest colour in
+  // one round, else G and H only (MODE 2), else the segmented cell-by-cell form (MODE 0).
+  const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds2 = gs_lds_bytes<T, 2>(Nz), lds3 = gs_lds_bytes<T, 3>(Nz);
+  const size_t lds0 = gs_lds_bytes<T, 0>(Nz);
+  int widest = 0;
+  for (size_t c = 0; c + 1 < h.gs_cstart.size(); ++c) widest = std::max(widest, h.gs_cstart[c + 1] - h.gs_cstart[c]);
+  auto per_cu = [&](size_t lds) { return std::max<long long>(1, (long long)h.lds_per_cu / (long long)lds); };
+  // option VTSA_OPT_GS_WHOLE_LINES: 0 segments, 1 automatic, 2 G and H, 3 the whole factor,
+  // 4 G then H; automatic takes the first of the whole factor, G and H, G then H whose blocks
+  // all fit on the CUs at once, else the first that fits at all
+  const int o = h.gs_full;
+  const bool ok1 = (o == 1 || o == 3) && gs_mode_ok<T, 1>(h);
+  const bool ok2 = (o == 1 || o == 2) && gs_mode_ok<T, 2>(h);
+  const bool ok3 = (o == 1 || o == 4) && gs_mode_ok<T, 3>(h);
+  auto one_round = [&](size_t lds) { return (long long)widest <= (long long)h.n_cu * per_cu(lds); };
+  int mode = 0;
+  if (ok1 && one_round(lds1)) mode = 1;
+  else if (ok2 && one_round(lds2)) mode = 2;
+  else if (ok3 && one_round(lds3)) mode = 3;
+  else if (ok1) mode = 1;
+  else if (ok2) mode = 2;
+  else if (ok3) mode = 3;
   GsArgs g{h.Jo, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1, h.Tw};
   const int C = (int)h.gs_cstart.size() - 1;
   // With two colours no line's first visit reads a line not yet visited, so the right-hand side
@@ -1250,6 +1329,14 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
     }
     if (mode == 1)
       hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
+                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
+                         partial, counter);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_gs_half<T, 2>), dim3(cnt), dim3(GSW), lds2, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
+                         h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
+                         partial, counter);
+    else if (mode == 3)
+      hipLaunchKernelGGL((k_gs_half<T, 3>), dim3(cnt), dim3(GSW), lds3, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
                          partial, counter);
     else

@@ -228,9 +228,13 @@ int vtsa_stream_destroy(vtsa_stream st);
  * measured with HIP events recorded on that stream (kernel timing for the roofline). */
 int vtsa_time_rhs(vtsa_handle* h, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st);
 
-/* Engine options (no upstream counterpart).  VTSA_OPT_GS_WHOLE_LINES: 1 (default) a line
- * Gauss-Seidel half-sweep stages its line's whole factor in LDS when the device grants a block
- * that much; 0 stages factor segments during the sweep.  Same arithmetic, bit-identical results. */
+/* Engine options (no upstream counterpart).  VTSA_OPT_GS_WHOLE_LINES, how a line Gauss-Seidel
+ * half-sweep stages its line's factor in LDS: 1 (default) automatic -- the first of the forms
+ * below whose blocks the CUs hold all at once, else the first the device grants at all; 3 the
+ * whole pair factor, 2 the two recurrence matrices of each pair (the rest read from global
+ * memory), 4 the forward matrix, then the backward one in its place (each when granted); 0 cell
+ * factors in segments during the sweep.  Levels 1-4 give bit-identical results; level 0 solves
+ * the same line systems cell by cell (other roundings). */
 enum { VTSA_OPT_GS_WHOLE_LINES = 1 };
 int vtsa_set_option(vtsa_handle* h, int32_t option, int32_t value);
 

@@ -301,12 +301,15 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     ref = _line_gs_reference(np.eye(m.n_core) - dh * J, m, r, sweeps)
     sc = dv.to_host(s0)
     outs = []
-    for level, lt in ((1, tol_pair), (0, tol)):   # whole line (pair recurrences), segmented (cell by cell)
+    # whole pair factor staged, both recurrence matrices staged, the forward then the backward one
+    # (pair recurrences), segmented (cell by cell)
+    for level, lt in ((3, tol_pair), (2, tol_pair), (4, tol_pair), (0, tol)):
         _abi.call("vtsa_set_option", m.handle, 1, level)
         out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
         assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < lt, level
         outs.append(out)
-    assert np.max(np.abs(outs[0] - outs[1]) / sc) / np.max(np.abs(ref) / sc) < tol_pair
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert np.max(np.abs(outs[0] - outs[3]) / sc) / np.max(np.abs(ref) / sc) < tol_pair
 
 
 @pytest.mark.parametrize("dh,rtol", [(0.05, 1e-2), (0.05, 1e-4), (0.3, 1e-2)])
