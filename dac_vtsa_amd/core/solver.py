@@ -470,14 +470,21 @@ class TRBDF2:
     "project_strict": the same with no shifted entry below zero; with either projection a stage
     whose balance stays open fails its step, which is retried with a smaller h),
     ``guess`` (a learned.StageNet: each stage starts from the network's guess when its
-    preconditioned residual is below the predictor's) and ``record`` (a learned.StageDataset
-    that receives the features and converged corrections of every solved stage).  The
+    preconditioned residual is below the predictor's), ``record`` (a learned.StageDataset
+    that receives the features and converged corrections of every solved stage), and the parity
+    mode: ``h_sequence`` (the accepted step end times of another run, e.g. the upstream
+    integrator's: each step ends at the next of them, every step that converges is accepted, a
+    stage that fails with a fresh Jacobian raises SolverError) and ``error_filter`` ("precond":
+    the local error filtered by the preconditioner, standing in for the upstream LU solve;
+    "exact": by a GMRES(64) solve of (I - D h J) e = z2 - y3 to 1e-10 relative); ``kappa``
+    overrides solver.newton_kappa (whose configured minimum is 1e-4) for such comparisons.  The
     preconditioner stands in for the upstream SuperLU factor of the theta-block matrix wherever
     that factor is used: GMRES, the Newton residual norm and the error-estimate filter."""
 
     def __init__(self, model, cfg, h_max: float, h_init: Optional[float] = None, logger=None,
                  precond: str = "zline_gs", precision: str = "fp64", krylov_dim: int = 30,
-                 gs_sweeps: int = 2, conserve: str = "polish", guess=None, record=None):
+                 gs_sweeps: int = 2, conserve: str = "polish", guess=None, record=None, h_sequence=None,
+                 error_filter: str = "precond", kappa: Optional[float] = None):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -485,7 +492,7 @@ class TRBDF2:
         self.h_max = h_max
         self.h_init = h_init if h_init is not None else s.h_initial
         self.max_iter = s.max_newton_iter
-        self.kappa = s.newton_kappa
+        self.kappa = s.newton_kappa if kappa is None else float(kappa)
         self.jac_max_age = s.jacobian_max_age
         self.max_steps = s.max_steps_per_step
         self.jac = FDJacobian(model)
@@ -525,6 +532,11 @@ class TRBDF2:
         if guess is not None or record is not None:
             self.stats.update({"guess_used": 0, "guess_kept_predictor": 0})
             self._s_ref, self._gf, self._gR, self._gP, self._zg = (dv.empty(self.ops.N) for _ in range(5))
+        # parity mode (no upstream counterpart): replay the accepted step end times of another run
+        self.h_sequence = None if h_sequence is None else np.asarray(h_sequence, dtype=float)
+        if error_filter not in ("precond", "exact"):
+            raise ValueError(f"error_filter={error_filter!r}")
+        self.error_filter = error_filter
         self.last_err = None
         self._new_lu = None
         self._new_J = None
@@ -532,6 +544,26 @@ class TRBDF2:
 
     def scale(self, y):
         return self.ops.scale(y, self.rtol)
+
+    def _error_norm(self, t, y, z2, f2, y3, dh):
+        """RMS of the filtered local error (core/solver.py:453-463): M^-1 (z2 - y3) scaled by
+        atol + rtol max(|y|, |z2|), M the preconditioner ("precond") or I - dh J(z2) solved by
+        GMRES ("exact")."""
+        ops = self.ops
+        if self.error_filter == "precond":
+            return ops.error_norm(y, z2, y3, self.rtol)
+        N = ops.N
+        d, s, b, w, ws, e = (dv.empty(N) for _ in range(6))
+        ops.vec(7, d, x=y3, y=z2)                       # z2 - y3
+        ops.scale(z2, self.rtol, s)
+        ops.vec(8, b, x=d, y=s)                         # -(z2 - y3)/s
+        info = ops.gmres(t, z2, f2, s, dh, None, b, w, 1e-10, 0.0, 64)
+        self.stats["exact_filter_unconverged"] = self.stats.get("exact_filter_unconverged", 0) + int(info.info != 0)
+        ops.vec(9, ws, x=w, y=s)
+        ops.prec_apply(ws, e)                           # -M^-1 (z2 - y3); the sign drops out of the rms
+        eh, yh, zh = dv.to_host(e), dv.to_host(y), dv.to_host(z2)
+        sc = self.atol + self.rtol * np.maximum(np.abs(yh), np.abs(zh))
+        return float(np.sqrt(np.mean((eh / sc) ** 2)))
 
     def _factor(self, y_ref, h):
         self.ops.precond_setup(D * h, self.ops.scale(y_ref, self.rtol), self.precond_kind, self.precision,
@@ -640,14 +672,22 @@ class TRBDF2:
         self._gm = [0]
         sd = self.scheme == "sdirk2"
         c_mid = G_SD if sd else GAMMA
+        replay = self.h_sequence is not None
+        k_rep = 0
         while t < t_end * (1.0 - 1e-12):
             gm0 = self._gm[0]
             if res.n_steps + res.n_reject + res.n_newton_fail > self.max_steps:
                 raise SolverError(f"too many steps in '{m.step['name']}' at t={t:.4g}s")
             t_target = t_out[k_out] if k_out < len(t_out) else t_end
             h_desired = h
-            h = min(h, self.h_max, t_target - t)
-            truncated = h < h_desired * (1.0 - 1e-9)
+            if replay:
+                if k_rep >= len(self.h_sequence):
+                    raise SolverError(f"replay: the step sequence ends before t_end (t={t:.6g}s)")
+                h = float(self.h_sequence[k_rep]) - t
+                truncated = False
+            else:
+                h = min(h, self.h_max, t_target - t)
+                truncated = h < h_desired * (1.0 - 1e-9)
             if h < 1e-10 * max(1.0, t_end):
                 raise SolverError(f"step size underflow (h={h:.3e}) in '{m.step['name']}' at t={t:.6g}s")
             if not have_J or jac_age >= self.jac_max_age:
@@ -688,6 +728,8 @@ class TRBDF2:
                 res.n_newton_fail += 1
                 if not fresh:
                     have_J = False
+                elif replay:
+                    raise SolverError(f"replay: stage failed at t={t:.6g}s, h={h:.3e} with a fresh Jacobian")
                 else:
                     h_cap = min(h_cap, self.cap_after_fail * h)
                     h *= 0.3
@@ -701,11 +743,11 @@ class TRBDF2:
                 ops.vec(5, y3, x=y, y=f_n, z=f1, w=f2, a=h, b=SW1, c=SW2, e=SW3)
             else:
                 ops.vec(5, y3, x=y, y=f_n, z=f1, w=f2, a=h, b=W1, c=W2, e=W3)
-            en = ops.error_norm(y, z2, y3, self.rtol)
+            en = self._error_norm(t + h, y, z2, f2, y3, dh)
             self.last_err = en
             if not np.isfinite(en):
                 en = 1e6
-            if en > 1.0:
+            if en > 1.0 and not replay:
                 res.n_reject += 1
                 h *= max(0.2, 0.9 * en ** (-1.0 / 3.0))
                 continue
@@ -717,6 +759,7 @@ class TRBDF2:
             ops.vec(6, slope, x=y, y=z2, a=h)
             have_slope = True
             t_new = t + h
+            k_rep += 1
             res.steps_taken.append((t, h))
             if stop_fn is not None and stop_value is not None:
                 sv = stop_fn(t_new, z2 if dev_cb else dv.to_host(z2))

@@ -1,8 +1,7 @@
-"""Oracle restatement of the step runner and the cycle harness (test infrastructure only).
+"""Oracle restatement of the step runner (test infrastructure only).
 
 ``run_step`` follows steps/_runner.py:87-188 (with ``record``/``_end_summary`` :50-84,104-121);
-``run_cycle``/``run_to_css`` follow vtsa_cycle.py:72-138 without the CSV hand-off (the state
-is passed in memory, which is bit-identical to the 17-digit CSV round trip).
+pinned by tests/golden/runner_cfg1.npz (outlet rows, accumulators, inventories, snapshots).
 """
 from __future__ import annotations
 
@@ -169,18 +168,6 @@ def run_step(state_in, cfg, grid, gas, step, cycle_idx=1, t_cycle0=0.0, T_jacket
                     ads_end={s: inv1[f"ads_{s}"] for s in model.adsorbed})
     sr.steps_taken = res.steps_taken
     return state_out, sr
-
-
-def run_cycle(state, cfg, grid, gas, cycle_idx=1, T_jac_prev=None, **kw):
-    """All steps of one cycle, state handed on in memory (vtsa_cycle.py:72-90)."""
-    steps_out, t_cycle = [], 0.0
-    for step in cfg.cycle:
-        state, sr = run_step(state, cfg, grid, gas, step, cycle_idx=cycle_idx, t_cycle0=t_cycle,
-                             T_jacket_prev=T_jac_prev, **kw)
-        t_cycle += sr.duration
-        T_jac_prev = sr.T_jacket if sr.jacket_on else T_jac_prev
-        steps_out.append(sr)
-    return state, steps_out, T_jac_prev
 
 
 def make(cfg):

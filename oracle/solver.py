@@ -227,7 +227,7 @@ def newton_krylov(model, t, z0, rhs_known, dh, lu, scale_fn, max_iter, kappa, kr
 class OracleTRBDF2:
     """Two-stage implicit integrator with Newton-aware step control (core/solver.py:297-511)."""
 
-    def __init__(self, model, cfg, h_max, h_init=None, logger=None, h_sequence=None):
+    def __init__(self, model, cfg, h_max, h_init=None, logger=None):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -299,9 +299,12 @@ class OracleTRBDF2:
         return z, ok, (0.0 if ok else rate)
 
     def integrate(self, y0, t_end, t_out, on_step: Optional[Callable] = None,
-                  stop_fn: Optional[Callable] = None, stop_value=None, stop_direction=1, t0: float = 0.0):
+                  stop_fn: Optional[Callable] = None, stop_value=None, stop_direction=1, t0: float = 0.0,
+                  h_sequence=None):
         """``t0`` (default 0, as upstream) starts the integration inside the step, for a timed
-        window of the CPU baseline; the first step size is then self.h_init."""
+        window of the CPU baseline; the first step size is then self.h_init.  ``h_sequence``
+        (no upstream counterpart): accepted step end times to replay; each step ends at the next
+        one, every converged step is accepted, a stage failing with a fresh Jacobian raises."""
         m = self.m
         wall0 = time.time()
         t = float(t0)
@@ -327,8 +330,12 @@ class OracleTRBDF2:
                 raise SolverError(f"too many steps in '{m.step['name']}' at t={t:.4g}s")
             t_target = t_out[k_out] if k_out < len(t_out) else t_end
             h_desired = h
-            h = min(h, self.h_max, t_target - t)
-            truncated = h < h_desired * (1.0 - 1e-9)
+            if h_sequence is not None:
+                h = float(h_sequence[res.n_steps]) - t
+                truncated = False
+            else:
+                h = min(h, self.h_max, t_target - t)
+                truncated = h < h_desired * (1.0 - 1e-9)
             if h < 1e-10 * max(1.0, t_end):
                 raise SolverError(f"step size underflow (h={h:.3e}) in '{m.step['name']}' at t={t:.6g}s")
             if J is None or jac_age >= self.jac_max_age:
@@ -368,6 +375,8 @@ class OracleTRBDF2:
                 res.n_newton_fail += 1
                 if not fresh:
                     J = None
+                elif h_sequence is not None:
+                    raise SolverError(f"replay: stage failed at t={t:.6g}s with a fresh Jacobian")
                 else:
                     h_cap = min(h_cap, self.cap_after_fail * h)
                     h *= 0.3
@@ -385,7 +394,7 @@ class OracleTRBDF2:
             en = rms(self.last_err)
             if not np.isfinite(en):
                 en = 1e6
-            if en > 1.0:
+            if en > 1.0 and h_sequence is None:
                 res.n_reject += 1
                 h *= max(0.2, 0.9 * en ** (-1.0 / 3.0))
                 continue

@@ -15,9 +15,18 @@ for the functions that no closed form, invariant or known answer pins on its own
                         trbdf2: accepted (t, h), counters, end state and accumulators;
 * ``integ_cfg1_hybrid.npz`` : 10 s of the same step with solver.newton_mode = hybrid (modified
                         Newton on the stored factor, Krylov on failure);
-* ``closures.npz``    : isotherm / p_sat / property values over a grid of inputs.
+* ``closures.npz``    : isotherm / p_sat / property values over a grid of inputs;
+* ``runner_cfg1.npz`` : on the seeded 4x4x20 state, per step type: inventories, the four stop
+                        metrics and q_feed_eq; the step runner over 20 s of adsorption:
+                        accepted times, outlet rows, StepResult accumulators and inventories,
+                        snapshots and the end state; the clean and feed-equilibrium initial beds;
+* ``integ_cfg1_200s.npz`` : the whole 200 s adsorption step of the seeded 4x4x20 state with a
+                        tight Newton tolerance (newton_kappa 1e-4, the smallest the configuration allows), sdirk2 and trbdf2: accepted
+                        times, counters, snapshots, end state and accumulators;
+* ``state_ref.csv``   : a state file written by the upstream save_state_csv (the seeded 3x4x6
+                        state), read back by this package's loader.
 
-Usage (from the repository root):  python -m tests.golden.make_golden
+Usage (from the repository root):  python -m tests.golden.make_golden [fixture stems ...]
 """
 from __future__ import annotations
 
@@ -40,6 +49,8 @@ from core.solver import TRBDF2  # noqa: E402
 from core.state import initial_state, CycleState  # noqa: E402
 from physics.isotherms import EquilibriumModel  # noqa: E402
 from properties import get_backend, arden_buck  # noqa: E402
+from steps._runner import run_step  # noqa: E402
+from simio.checkpoint import save_state_csv  # noqa: E402
 
 from dac_vtsa_amd.seeds import seeded_state  # noqa: E402  (this repository)
 
@@ -218,7 +229,108 @@ def make_closures(path):
     np.savez_compressed(path, **out)
 
 
-if __name__ == "__main__":
+INV_KEYS = ("gas_CO2", "ads_CO2", "tot_CO2", "gas_H2O", "ads_H2O", "tot_H2O", "gas_N2", "ads_N2", "tot_N2",
+            "E_bed", "E_wall", "E_total")
+STOP_CRITERIA = (("min_temperature", 300.0), ("max_temperature", 350.0), ("end_saturation", 0.5),
+                 ("outlet_ratio", 0.5))
+
+
+def make_runner(path):
+    cfg = load_config(override_dict={"grid.Nr": 4, "grid.Ntheta": 4, "grid.Nz": 20})
+    g, st = _seeded(cfg, "small", 0)
+    props = get_backend(cfg)
+    out = {"inv_keys": np.array(INV_KEYS), "stop_criteria": np.array([c for c, _ in STOP_CRITERIA])}
+    _state_arrays("", st, out)
+    for si, step in enumerate(cfg.cycle):
+        stops = []
+        for crit, val in STOP_CRITERIA:
+            stp = dict(step)
+            stp["stop_on"] = {"criterion": crit, "value": val}
+            m = TVSAModel(cfg, g, props, stp, st)
+            stops.append(m.stop_metric(7.0, m.pack(st)))
+        out[f"s{si}/stop"] = np.array(stops)
+        inv = m.inventories(m.pack(st))
+        out[f"s{si}/inv"] = np.array([inv[k] for k in INV_KEYS])
+        out[f"s{si}/q_feed_eq"] = m.q_feed_eq()
+    step = dict(cfg.cycle[0])
+    step["duration"] = 20.0
+    times = []
+    st_out, sr = run_step(st, cfg, g, props, step)
+    rows = sr.outlet
+    num = [k for k, v in rows[0].items() if isinstance(v, (int, float, np.floating)) and not isinstance(v, bool)]
+    out["outlet_keys"] = np.array(num)
+    out["outlet_str_keys"] = np.array([k for k in rows[0] if k not in num])
+    out["outlet_str"] = np.array([[str(r[k]) for k in out["outlet_str_keys"]] for r in rows])
+    out["outlet"] = np.array([[float(r[k]) for k in num] for r in rows])
+    m = TVSAModel(cfg, g, props, step, st)
+    out["acc_names"] = np.array(m.acc_names)
+    out["acc"] = np.array([sr.acc[k] for k in m.acc_names])
+    out["inv_start"] = np.array([sr.inv_start[k] for k in INV_KEYS])
+    out["inv_end"] = np.array([sr.inv_end[k] for k in INV_KEYS])
+    out["duration"], out["T_jacket"] = sr.duration, sr.T_jacket
+    out["counters"] = np.array([sr.solver[k] for k in ("steps", "rejected", "newton_fail", "krylov", "jacobians",
+                                                       "lu", "nfev")])
+    out["snap_t"] = np.array([sn["t_step"] for sn in sr.snapshots])
+    for k in ("c", "q", "T", "Tw"):
+        out["snap_" + k] = np.array([getattr(sn["state"], k) for sn in sr.snapshots])
+        out["end_" + k] = getattr(st_out, k)
+    out["t_acc"] = np.array([r["t_step_s"] for r in rows[1:]])
+    for mode in ("clean", "feed_equilibrium"):
+        cm = load_config(override_dict={"grid.Nr": 4, "grid.Ntheta": 4, "grid.Nz": 20, "run.initial_state": mode})
+        si = initial_state(cm, Grid.from_config(cm), get_backend(cm))
+        _state_arrays(f"init_{mode}/", si, out)
+    np.savez_compressed(path, **out)
+
+
+def make_integ_200(path, kappa=1e-4):
+    out = {}
+    for scheme in ("sdirk2", "trbdf2"):
+        cfg = load_config(override_dict={"grid.Nr": 4, "grid.Ntheta": 4, "grid.Nz": 20, "solver.scheme": scheme,
+                                         "solver.newton_kappa": kappa})
+        g, st = _seeded(cfg, "small", 0)
+        if scheme == "sdirk2":
+            _state_arrays("", st, out)
+        props = get_backend(cfg)
+        m = TVSAModel(cfg, g, props, cfg.cycle[0], st)
+        y0 = m.pack(st)
+        integ = TRBDF2(m, cfg, h_max=0.02 * 200.0)
+        times = []
+        t0 = time.time()
+        res = integ.integrate(y0, 200.0, np.linspace(0, 200.0, 12), on_step=lambda t, y, a: times.append(t))
+        p = f"{scheme}/"
+        out[p + "wall"] = time.time() - t0
+        out[p + "y0"], out[p + "y"], out[p + "acc"], out[p + "t_acc"] = y0, res.y, res.acc, np.array(times)
+        out[p + "counters"] = np.array([res.n_steps, res.n_reject, res.n_newton_fail, res.n_krylov,
+                                        res.n_jac, res.n_lu, res.nfev])
+        out[p + "snap_t"] = np.array([s[0] for s in res.snapshots])
+        out[p + "snap_y"] = np.array([s[1] for s in res.snapshots])
+        out[p + "inv0"] = np.array([m.inventories(y0)[k] for k in INV_KEYS])
+        out[p + "inv1"] = np.array([m.inventories(res.y)[k] for k in INV_KEYS])
+        out[p + "stats"] = np.array([integ.stats.get(k, 0) for k in ("gmres_iters", "balance_unclosed")])
+        print(scheme, "200 s steps", res.n_steps, "wall", out[p + "wall"], flush=True)
+    out["kappa"] = kappa
+    np.savez_compressed(path, **out)
+
+
+def make_state_csv(path):
+    cfg = load_config(override_dict=SMALL)
+    g = Grid.from_config(cfg)
+    save_state_csv(small_state(cfg, g, get_backend(cfg)), g, path)
+
+
+MAKERS = {"closures": make_closures, "rhs_small": make_rhs_small, "rhs_cfg1": make_rhs_cfg1, "rhs_mid": make_rhs_mid,
+          "runner_cfg1": make_runner, "integ_cfg1_200s": make_integ_200}
+
+
+if __name__ == "__main__" and len(sys.argv) > 1:
+    for stem in sys.argv[1:]:
+        t0 = time.time()
+        if stem == "state_ref":
+            make_state_csv(os.path.join(HERE, "state_ref.csv"))
+        else:
+            MAKERS[stem](os.path.join(HERE, stem + ".npz"))
+        print(stem, "done in %.1f s" % (time.time() - t0), flush=True)
+elif __name__ == "__main__":
     t0 = time.time()
     make_closures(os.path.join(HERE, "closures.npz"))
     make_rhs_small(os.path.join(HERE, "rhs_small.npz"))

@@ -398,3 +398,113 @@ def test_mlp_feature_target_guess_maps():
     z = mlp.apply_guess(np.full((n, 16), -1e9, np.float32), norm, zp, s, n)
     assert np.array_equal(z[:5 * n], 0.1 * zp[:5 * n])
     assert np.all(z[5 * n:6 * n] < zp[5 * n:6 * n] - 1e5)
+
+
+# ------------------------------------------------------------------ runner, inventories, stop criteria
+def _runner_step(cfg, si, crit=None, val=None, duration=None):
+    step = dict(cfg.cycle[si])
+    if crit is not None:
+        step["stop_on"] = {"criterion": crit, "value": val}
+    if duration is not None:
+        step["duration"] = duration
+    return step
+
+
+def test_inventories_stop_metrics_and_feed_loading_match_golden():
+    """OracleModel.inventories, stop_metric (all four criteria) and q_feed_eq on the seeded
+    4x4x20 state for every step type, against the upstream values (core/residual.py:426-482)."""
+    from oracle.model import OracleModel
+    from oracle.runner import make
+    z = golden("runner_cfg1.npz")
+    st = state_from(z)
+    cfg = load_config(override_dict=CFG1)
+    g, gas = make(cfg)
+    keys = [str(k) for k in z["inv_keys"]]
+    crits = [str(c) for c in z["stop_criteria"]]
+    vals = dict((("min_temperature", 300.0), ("max_temperature", 350.0), ("end_saturation", 0.5),
+                 ("outlet_ratio", 0.5)))
+    for si in range(4):
+        for ci, crit in enumerate(crits):
+            om = OracleModel(cfg, g, gas, _runner_step(cfg, si, crit, vals[crit]), st)
+            v = om.stop_metric(7.0, om.pack(st))
+            ref = float(z[f"s{si}/stop"][ci])
+            assert abs(v - ref) <= 1e-12 * max(abs(ref), 1e-300), (si, crit, v, ref)
+        inv = om.inventories(om.pack(st))
+        np.testing.assert_allclose([inv[k] for k in keys], z[f"s{si}/inv"], rtol=1e-13, atol=0.0)
+        assert abs(om.q_feed_eq() - float(z[f"s{si}/q_feed_eq"])) <= 1e-13 * float(z[f"s{si}/q_feed_eq"])
+
+
+def test_initial_states_match_golden():
+    """Clean and feed-equilibrium beds (core/state.py:62-93)."""
+    from oracle.runner import initial_state, make
+    z = golden("runner_cfg1.npz")
+    for mode in ("clean", "feed_equilibrium"):
+        cfg = load_config(override_dict={**CFG1, "run.initial_state": mode})
+        g, gas = make(cfg)
+        st = initial_state(cfg, g, gas)
+        for k in ("c", "q", "T", "Tw"):
+            np.testing.assert_allclose(getattr(st, k), z[f"init_{mode}/{k}"], rtol=1e-14, atol=0.0)
+
+
+def test_run_step_matches_golden():
+    """The oracle step runner (steps/_runner.py:87-188) over 20 s of adsorption from the seeded
+    4x4x20 state: the same accepted steps and counters as upstream, every outlet row, the
+    StepResult accumulators and inventories, the snapshots and the end state."""
+    from oracle.runner import make, run_step
+    z = golden("runner_cfg1.npz")
+    st = state_from(z)
+    cfg = load_config(override_dict=CFG1)
+    g, gas = make(cfg)
+    st_out, sr = run_step(st, cfg, g, gas, _runner_step(cfg, 0, duration=20.0))
+    cnt = [sr.solver[k] for k in ("steps", "rejected", "newton_fail", "krylov", "jacobians", "lu", "nfev")]
+    assert cnt == z["counters"].tolist()
+    rows = sr.outlet
+    keys = [str(k) for k in z["outlet_keys"]]
+    got = np.array([[float(r[k]) for k in keys] for r in rows])
+    assert got.shape == z["outlet"].shape
+    scale = np.maximum(np.abs(z["outlet"]).max(axis=0), 1e-300)
+    assert np.max(np.abs(got - z["outlet"]) / scale) < 1e-9
+    assert [[str(r[k]) for k in z["outlet_str_keys"]] for r in rows] == z["outlet_str"].tolist()
+    names = [str(k) for k in z["acc_names"]]
+    acc = np.array([sr.acc[k] for k in names])
+    np.testing.assert_allclose(acc, z["acc"], rtol=1e-9, atol=1e-12 * np.abs(z["acc"]).max())
+    ik = [str(k) for k in z["inv_keys"]] if "inv_keys" in z else None
+    np.testing.assert_allclose([sr.inv_start[k] for k in ik], z["inv_start"], rtol=1e-14)
+    np.testing.assert_allclose([sr.inv_end[k] for k in ik], z["inv_end"], rtol=1e-10)
+    np.testing.assert_allclose([sn["t_step"] for sn in sr.snapshots], z["snap_t"], rtol=1e-14)
+    sc = np.abs(z["end_c"]) + 1e-6
+    assert np.max(np.abs(st_out.c - z["end_c"]) / sc) < 1e-8
+    np.testing.assert_allclose(st_out.T, z["end_T"], rtol=1e-11)
+    np.testing.assert_allclose(st_out.Tw, z["end_Tw"], rtol=1e-11)
+
+
+@pytest.mark.parametrize("scheme", ["sdirk2", "trbdf2"])
+def test_integrator_full_adsorption_step_matches_golden(scheme):
+    """The whole 200 s adsorption step of the seeded 4x4x20 state at newton_kappa 1e-4: the oracle
+    takes the upstream's accepted steps and ends on its state."""
+    from oracle.solver import OracleTRBDF2
+    z = golden("integ_cfg1_200s.npz")
+    cfg, m = oracle_model({**CFG1, "solver.scheme": scheme, "solver.newton_kappa": 1e-4}, 0, state_from(z))
+    y0 = m.pack(state_from(z))
+    np.testing.assert_allclose(y0, z[f"{scheme}/y0"], rtol=1e-15)
+    times = []
+    res = OracleTRBDF2(m, cfg, h_max=4.0).integrate(y0, 200.0, np.linspace(0, 200.0, 12),
+                                                   on_step=lambda t, y, a: times.append(t))
+    cnt = [res.n_steps, res.n_reject, res.n_newton_fail, res.n_krylov, res.n_jac, res.n_lu, res.nfev]
+    assert cnt == z[f"{scheme}/counters"].tolist()
+    np.testing.assert_allclose(times, z[f"{scheme}/t_acc"], rtol=1e-12)
+    sc = m.atol + 1e-4 * np.abs(z[f"{scheme}/y"])
+    assert np.max(np.abs(res.y - z[f"{scheme}/y"]) / sc) < 1e-6
+
+
+def test_replay_reproduces_the_upstream_step_sequence():
+    """Replaying the upstream run's accepted times (h_sequence) with the same Newton tolerance gives
+    the upstream trajectory (the replay follows the same steps without the error test)."""
+    from oracle.solver import OracleTRBDF2
+    z = golden("integ_cfg1_200s.npz")
+    cfg, m = oracle_model({**CFG1, "solver.newton_kappa": 1e-4}, 0, state_from(z))
+    res = OracleTRBDF2(m, cfg, h_max=4.0).integrate(m.pack(state_from(z)), 200.0, np.linspace(0, 200.0, 12),
+                                                   h_sequence=z["sdirk2/t_acc"])
+    assert res.n_steps == len(z["sdirk2/t_acc"]) and res.n_reject == 0
+    sc = m.atol + 1e-4 * np.abs(z["sdirk2/y"])
+    assert np.max(np.abs(res.y - z["sdirk2/y"]) / sc) < 1e-4
