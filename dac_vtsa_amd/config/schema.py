@@ -1,21 +1,24 @@
 """Configuration loading and validation for the TVSA step engine.
 
-Mirrors the public behaviour of the upstream loader (config/schema.py:529-594 and
-config/parameters.py:25-44): defaults are deep-merged with an optional user YAML, a dict of
-dotted overrides and ``key=value`` strings; string references such as ``operation.T_heat`` in
-cycle steps are resolved; every key is type/range checked; unknown keys, ``Ntheta == 2`` and
-inconsistent cycle definitions raise :class:`ConfigError`; derived quantities are added; the
-result is a read-only attribute view.
+Accepts and rejects the same inputs as the upstream loader (config/schema.py:529-594 and
+config/parameters.py:25-44): the defaults are merged with an optional user YAML (a user ``cycle``
+replaces the default one), a dict of dotted overrides and ``key=value`` strings; every leaf key
+must be known and is checked against its spec (type, range, choices); cycle steps are normalised
+(valve states, jacket, stop criterion, product port; string references such as
+``operation.T_heat`` are resolved against the configuration and remembered, so that a later
+override of the referenced key reaches the step); cross-field rules and derived quantities follow;
+the result is a read-only attribute view.
 
-The key table below gives, per dotted key, ``(type, lo, hi, choices, nullable)``.  Ranges are
-the upstream ones (config/schema.py:58-281) so that the same inputs are accepted and rejected.
+KEYS gives, per dotted key, ``(type, lo, hi, choices, nullable)``; the ranges are the upstream ones
+(config/schema.py:58-281).
 """
 from __future__ import annotations
 
 import copy
 import math
 import os
-from typing import Any, Dict, List, Optional, Sequence, Tuple
+from dataclasses import dataclass
+from typing import Any, Callable, Dict, Iterator, List, Optional, Sequence, Tuple
 
 import yaml
 
@@ -23,7 +26,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULTS_PATH = os.path.join(HERE, "defaults.yaml")
 
 F, I, S, B, L, A = "float", "int", "str", "bool", "list", "any"
-INF = None
 
 
 def _f(lo=None, hi=None):
@@ -142,6 +144,7 @@ STEP_TYPES = ("adsorption", "blowdown", "heating", "cooling", "pressurization")
 VALVES = ("V1", "V2", "V3")
 VALVE_STATES = ("CLOSED", "OPEN", "OPEN_ATMOSPHERIC", "OPEN_VACUUM")
 STOP_CRITERIA = ("outlet_ratio", "end_saturation", "min_temperature", "max_temperature")
+STEP_FIELDS = frozenset({"name", "type", "duration", "valves", "jacket", "stop_on", "product_port", "notes"})
 
 
 class ConfigError(ValueError):
@@ -170,12 +173,10 @@ class Config:
         return self.__getattr__(name)
 
     def get(self, dotted: str, default=None):
-        node: Any = self._data
-        for part in dotted.split("."):
-            if not (isinstance(node, dict) and part in node):
-                return default
-            node = node[part]
-        return node
+        try:
+            return get_dotted(self._data, dotted)
+        except (KeyError, TypeError):
+            return default
 
     def to_dict(self) -> Dict[str, Any]:
         return copy.deepcopy(self._data)
@@ -184,215 +185,283 @@ class Config:
         return f"Config({list(self._data)})"
 
 
-def deep_merge(base: Dict, upd: Optional[Dict]) -> Dict:
-    out = copy.deepcopy(base)
-    for k, v in (upd or {}).items():
-        out[k] = deep_merge(out[k], v) if isinstance(v, dict) and isinstance(out.get(k), dict) \
-            else copy.deepcopy(v)
-    return out
+# ------------------------------------------------------------------ nested-dict helpers
+def get_dotted(tree: Dict, dotted: str):
+    node = tree
+    for part in dotted.split("."):
+        node = node[part]
+    return node
 
 
-def set_dotted(d: Dict, dotted: str, value: Any) -> None:
-    *head, last = dotted.split(".")
-    for p in head:
-        if not isinstance(d.get(p), dict):
-            d[p] = {}
-        d = d[p]
-    d[last] = value
+def set_dotted(tree: Dict, dotted: str, value: Any) -> None:
+    """Set tree[a][b]...[z] = value for dotted = "a.b...z", creating (or replacing non-dict)
+    intermediate levels."""
+    parts = dotted.split(".")
+    node = tree
+    for part in parts[:-1]:
+        child = node.get(part)
+        if not isinstance(child, dict):
+            child = node[part] = {}
+        node = child
+    node[parts[-1]] = value
 
 
-def get_dotted(d: Dict, dotted: str):
-    for p in dotted.split("."):
-        d = d[p]
-    return d
+def deep_merge(base: Dict, update: Optional[Dict]) -> Dict:
+    """A copy of base with update merged in: sub-dicts merge recursively, other values replace."""
+    merged = copy.deepcopy(base)
+    stack = [(merged, update or {})]
+    while stack:
+        dst, src = stack.pop()
+        for key, value in src.items():
+            if isinstance(value, dict) and isinstance(dst.get(key), dict):
+                stack.append((dst[key], value))
+            else:
+                dst[key] = copy.deepcopy(value)
+    return merged
 
 
-def _leaves(d: Dict, prefix: str = "") -> Dict[str, Any]:
-    out = {}
-    for k, v in d.items():
-        key = f"{prefix}.{k}" if prefix else str(k)
-        if isinstance(v, dict) and key not in KEYS:
-            out.update(_leaves(v, key))
+def _leaf_keys(tree: Dict, prefix: str = "") -> Iterator[str]:
+    """Dotted paths of the leaves (a spec'd key whose value is a dict counts as a leaf)."""
+    for key, value in tree.items():
+        path = f"{prefix}.{key}" if prefix else str(key)
+        if isinstance(value, dict) and path not in SPECS:
+            yield from _leaf_keys(value, path)
         else:
-            out[key] = v
-    return out
+            yield path
 
 
 def parse_override(text: str) -> Tuple[str, Any]:
-    if "=" not in text:
-        raise ConfigError(f"override '{text}' must look like key=value")
-    k, v = text.split("=", 1)
-    return k.strip(), yaml.safe_load(v)
+    """``"a.b=value"`` -> ("a.b", YAML value)."""
+    key, sep, value = text.partition("=")
+    if not sep:
+        raise ConfigError(f"override {text!r} is not of the form key=value")
+    return key.strip(), yaml.safe_load(value)
 
 
-def _coerce(key: str, v: Any) -> Any:
-    typ, lo, hi, choices, nullable = KEYS[key]
-    if v is None:
-        if nullable:
-            return None
-        raise ConfigError(f"{key}: value is required")
-    if typ in (F, I) and isinstance(v, str):
-        try:                      # YAML 1.1 reads '2.0e4' as a string
-            v = float(v)
-        except ValueError:
-            raise ConfigError(f"{key}: expected a number, got {v!r}")
-    if typ == F:
-        if isinstance(v, bool) or not isinstance(v, (int, float)):
-            raise ConfigError(f"{key}: expected a number, got {v!r}")
-        v = float(v)
-        if not math.isfinite(v):
-            raise ConfigError(f"{key}: must be finite")
-    elif typ == I:
-        if isinstance(v, bool) or not isinstance(v, (int, float)) or int(v) != v:
-            raise ConfigError(f"{key}: expected an integer, got {v!r}")
-        v = int(v)
-    elif typ == B and not isinstance(v, bool):
-        raise ConfigError(f"{key}: expected true/false, got {v!r}")
-    elif typ == S and not isinstance(v, str):
-        raise ConfigError(f"{key}: expected a string, got {v!r}")
-    elif typ == L:
-        if not isinstance(v, (list, tuple)):
-            raise ConfigError(f"{key}: expected a list, got {v!r}")
-        v = list(v)
-    if choices is not None and v not in choices:
-        raise ConfigError(f"{key}: {v!r} not in {choices}")
-    if typ in (F, I):
-        if lo is not None and v < lo:
-            raise ConfigError(f"{key} = {v} below allowed minimum {lo}")
-        if hi is not None and v > hi:
-            raise ConfigError(f"{key} = {v} above allowed maximum {hi}")
-    return v
+# ------------------------------------------------------------------ per-key specs
+@dataclass(frozen=True)
+class Spec:
+    kind: str
+    lo: Optional[float] = None
+    hi: Optional[float] = None
+    choices: Optional[Tuple] = None
+    nullable: bool = False
+
+    def check(self, key: str, value: Any) -> Any:
+        """The value converted to the spec's type, or ConfigError."""
+        if value is None:
+            if self.nullable:
+                return None
+            raise ConfigError(f"{key} must be set")
+        value = self._typed(key, value)
+        if self.choices is not None and value not in self.choices:
+            raise ConfigError(f"{key} = {value!r}: expected one of {list(self.choices)}")
+        if self.kind in (F, I):
+            if self.lo is not None and value < self.lo:
+                raise ConfigError(f"{key} = {value} is below its minimum {self.lo}")
+            if self.hi is not None and value > self.hi:
+                raise ConfigError(f"{key} = {value} is above its maximum {self.hi}")
+        return value
+
+    def _typed(self, key, value):
+        kind = self.kind
+        if kind in (F, I):
+            if isinstance(value, str):
+                # YAML 1.1 leaves e.g. 2.0e4 (no sign in the exponent) a string
+                try:
+                    value = float(value)
+                except ValueError:
+                    raise ConfigError(f"{key} = {value!r} is not a number") from None
+            if isinstance(value, bool) or not isinstance(value, (int, float)):
+                raise ConfigError(f"{key} = {value!r} is not a number")
+            if kind == F:
+                if not math.isfinite(value):
+                    raise ConfigError(f"{key} = {value!r} is not finite")
+                return float(value)
+            if int(value) != value:
+                raise ConfigError(f"{key} = {value!r} is not an integer")
+            return int(value)
+        if kind == B and not isinstance(value, bool):
+            raise ConfigError(f"{key} = {value!r}: expected true or false")
+        if kind == S and not isinstance(value, str):
+            raise ConfigError(f"{key} = {value!r}: expected a string")
+        if kind == L:
+            if not isinstance(value, (list, tuple)):
+                raise ConfigError(f"{key} = {value!r}: expected a list")
+            return list(value)
+        return value
 
 
-def _ref(raw: Dict, value: Any) -> Any:
-    if isinstance(value, str) and "." in value:
-        try:
-            return float(get_dotted(raw, value))
-        except (KeyError, TypeError, ValueError):
-            raise ConfigError(f"cannot resolve reference '{value}'")
-    return value
+SPECS: Dict[str, Spec] = {key: Spec(*row) for key, row in KEYS.items()}
 
 
-def _is_number(x: Any) -> bool:
-    try:
-        float(x)
-        return True
-    except (TypeError, ValueError):
+# ------------------------------------------------------------------ cycle steps
+def _is_reference(value) -> bool:
+    """A string naming another configuration key (it holds a dot and is not a number)."""
+    if not isinstance(value, str) or "." not in value:
         return False
+    try:
+        float(value)
+    except ValueError:
+        return True
+    return False
 
 
-def _cycle(raw: Dict) -> List[Dict[str, Any]]:
+class _StepReader:
+    """Normalises one cycle step of the raw configuration."""
+
+    def __init__(self, raw: Dict, index: int):
+        self.raw, self.where = raw, f"cycle[{index}]"
+
+    def fail(self, what: str):
+        raise ConfigError(f"{self.where}: {what}")
+
+    def number(self, value, what: str) -> float:
+        """A number, or the value of the key a reference string names."""
+        if _is_reference(value):
+            try:
+                value = get_dotted(self.raw, value)
+            except (KeyError, TypeError):
+                self.fail(f"{what} refers to {value!r}, which is not a configuration key")
+        if isinstance(value, str):
+            try:
+                value = float(value)
+            except ValueError:
+                self.fail(f"{what} = {value!r} is not a number")
+        if isinstance(value, bool) or not isinstance(value, (int, float)):
+            self.fail(f"{what} = {value!r} is not a number")
+        return float(value)
+
+    def read(self, step: Any) -> Dict[str, Any]:
+        if not isinstance(step, dict):
+            self.fail("a step must be a mapping")
+        extra = sorted(set(step) - STEP_FIELDS)
+        if extra:
+            self.fail(f"unknown step fields {extra}")
+        out = copy.deepcopy(step)
+        out["name"] = step.get("name") or step.get("type")
+        if step.get("type") not in STEP_TYPES:
+            self.fail(f"type must be one of {list(STEP_TYPES)}")
+        if step.get("duration") is None:
+            self.fail("a duration is required")
+        out["duration"] = self.number(step.get("duration"), "duration")
+        if out["duration"] <= 0.0:
+            self.fail("duration must be positive")
+        out["valves"] = self.valves(step.get("valves") or {})
+        out["jacket"] = self.jacket(step.get("jacket") or {"mode": "adiabatic"})
+        out["stop_on"] = self.stop(step.get("stop_on"))
+        port = step.get("product_port") or "V3"
+        if port not in VALVES + ("none",):
+            self.fail("product_port must be V1, V2, V3 or none")
+        out["product_port"] = port
+        return out
+
+    def valves(self, given: Dict) -> Dict[str, str]:
+        extra = sorted(set(given) - set(VALVES))
+        if extra:
+            self.fail(f"unknown valves {extra}")
+        states = {}
+        for v in VALVES:
+            states[v] = given.get(v, "CLOSED")
+            if states[v] not in VALVE_STATES:
+                self.fail(f"valve {v} state {states[v]!r} is not one of {list(VALVE_STATES)}")
+        return states
+
+    def jacket(self, given: Dict) -> Dict[str, Any]:
+        mode = given.get("mode", "temperature")
+        if mode == "adiabatic":
+            return {"mode": mode}
+        if mode != "temperature":
+            self.fail("jacket mode must be temperature or adiabatic")
+        source = given.get("T_source", given.get("T"))
+        if source is None:
+            self.fail("a temperature jacket needs T")
+        T = self.number(source, "jacket T")
+        if not 150.0 < T < 900.0:
+            self.fail(f"jacket T = {T} K is out of range")
+        out = {"mode": mode, "T": T}
+        if _is_reference(source):
+            out["T_source"] = source
+        return out
+
+    def stop(self, given: Optional[Dict]) -> Optional[Dict[str, Any]]:
+        if given is None:
+            return None
+        if given.get("criterion") not in STOP_CRITERIA:
+            self.fail(f"stop_on criterion must be one of {list(STOP_CRITERIA)}")
+        source = given.get("value_source", given.get("value"))
+        out = {"criterion": given["criterion"], "value": self.number(source, "stop_on value")}
+        if _is_reference(source):
+            out["value_source"] = source
+        return out
+
+
+def _read_cycle(raw: Dict) -> List[Dict[str, Any]]:
     steps = raw.get("cycle")
     if not isinstance(steps, list) or not steps:
-        raise ConfigError("cycle: must be a non-empty list of steps")
-    allowed = {"name", "type", "duration", "valves", "jacket", "stop_on", "product_port", "notes"}
-    out, names = [], set()
-    for i, st in enumerate(steps):
-        if not isinstance(st, dict):
-            raise ConfigError(f"cycle[{i}] must be a mapping")
-        st = copy.deepcopy(st)
-        if set(st) - allowed:
-            raise ConfigError(f"cycle[{i}]: unknown keys {sorted(set(st) - allowed)}")
-        name = st.get("name") or st.get("type")
-        if name in names:
-            raise ConfigError(f"cycle: duplicate step name '{name}'")
-        names.add(name)
-        st["name"] = name
-        if st.get("type") not in STEP_TYPES:
-            raise ConfigError(f"cycle[{i}].type must be one of {STEP_TYPES}")
-        dur = _ref(raw, st.get("duration"))
-        if not isinstance(dur, (int, float)) or isinstance(dur, bool) or dur <= 0:
-            raise ConfigError(f"cycle[{i}].duration must be > 0 s")
-        st["duration"] = float(dur)
-        valves = st.get("valves") or {}
-        if set(valves) - set(VALVES):
-            raise ConfigError(f"cycle[{i}].valves: unknown valves {sorted(set(valves) - set(VALVES))}")
-        vv = {}
-        for v in VALVES:
-            s = valves.get(v, "CLOSED")
-            if s not in VALVE_STATES:
-                raise ConfigError(f"cycle[{i}].valves.{v} = {s!r} not in {VALVE_STATES}")
-            vv[v] = s
-        st["valves"] = vv
-        jk = st.get("jacket") or {"mode": "adiabatic"}
-        mode = jk.get("mode", "temperature")
-        if mode not in ("temperature", "adiabatic"):
-            raise ConfigError(f"cycle[{i}].jacket.mode must be temperature|adiabatic")
-        jout = {"mode": mode}
-        if mode == "temperature":
-            src = jk.get("T_source", jk.get("T"))
-            T = _ref(raw, src)
-            if not isinstance(T, (int, float)) or not (150 < T < 900):
-                raise ConfigError(f"cycle[{i}].jacket.T invalid ({T!r})")
-            jout["T"] = float(T)
-            if isinstance(src, str) and "." in src and not _is_number(src):
-                jout["T_source"] = src
-        st["jacket"] = jout
-        stop = st.get("stop_on")
-        if stop is not None:
-            crit = stop.get("criterion")
-            if crit not in STOP_CRITERIA:
-                raise ConfigError(f"cycle[{i}].stop_on.criterion invalid: {crit!r}")
-            vsrc = stop.get("value_source", stop.get("value"))
-            new = {"criterion": crit, "value": float(_ref(raw, vsrc))}
-            if isinstance(vsrc, str) and "." in vsrc and not _is_number(vsrc):
-                new["value_source"] = vsrc
-            stop = new
-        st["stop_on"] = stop
-        pp = st.get("product_port") or "V3"
-        if pp not in VALVES + ("none",):
-            raise ConfigError(f"cycle[{i}].product_port must be V1/V2/V3/none")
-        st["product_port"] = pp
-        out.append(st)
-    ports = raw["ports"]
-    for st in out:
-        per_end: Dict[str, List[str]] = {"top": [], "bottom": []}
-        for v, s in st["valves"].items():
-            if s != "CLOSED":
-                per_end[ports[v]].append(v)
-            if s == "OPEN" and v != "V1":
-                raise ConfigError(f"step '{st['name']}': state OPEN (feed blower) is only valid for V1")
-        for end, lst in per_end.items():
-            if len(lst) > 1:
-                raise ConfigError(f"step '{st['name']}': valves {lst} are open at the same end ({end})")
-    return out
+        raise ConfigError("cycle must be a non-empty list of steps")
+    cycle = [_StepReader(raw, i).read(st) for i, st in enumerate(steps)]
+    seen = set()
+    for st in cycle:
+        if st["name"] in seen:
+            raise ConfigError(f"cycle: two steps are named {st['name']!r}")
+        seen.add(st["name"])
+        open_at = {"top": [], "bottom": []}
+        for v, state in st["valves"].items():
+            if state == "OPEN" and v != "V1":
+                raise ConfigError(f"step {st['name']!r}: only V1 (the feed blower) can be OPEN")
+            if state != "CLOSED":
+                open_at[raw["ports"][v]].append(v)
+        for end, vs in open_at.items():
+            if len(vs) > 1:
+                raise ConfigError(f"step {st['name']!r}: valves {vs} are open at the same end ({end})")
+    return cycle
+
+
+# ------------------------------------------------------------------ whole configuration
+def _interval(x, lo, hi) -> bool:
+    return len(x) == 2 and lo <= x[0] < x[1] <= hi
+
+
+RULES: List[Tuple[Callable[[Dict], bool], str]] = [
+    (lambda r: r["sorbent"]["total_voidage"] >= r["sorbent"]["bed_voidage"],
+     "sorbent.total_voidage must not be below sorbent.bed_voidage"),
+    (lambda r: r["run"]["min_cycles"] <= r["run"]["max_cycles"], "run.min_cycles exceeds run.max_cycles"),
+    (lambda r: _interval(r["jacket"]["theta_coverage_deg"], 0, 360),
+     "jacket.theta_coverage_deg must be [start, end] with 0 <= start < end <= 360"),
+    (lambda r: _interval(r["jacket"]["z_coverage"], 0, 1),
+     "jacket.z_coverage must be [start, end] with 0 <= start < end <= 1"),
+    (lambda r: r["operation"]["P_vacuum"] < r["ambient"]["P"], "operation.P_vacuum must be below ambient.P"),
+    (lambda r: r["grid"]["Ntheta"] != 2, "grid.Ntheta = 2 is degenerate: use 1 (axisymmetric) or at least 3"),
+]
+
+
+def _derived(raw: Dict) -> Dict[str, float]:
+    sb, geo = raw["sorbent"], raw["geometry"]
+    area = math.pi * geo["bed_radius"] ** 2
+    volume = area * geo["bed_length"]
+    rho_b = sb["particle_density"] * (1.0 - sb["bed_voidage"])
+    return {"particle_porosity": (sb["total_voidage"] - sb["bed_voidage"]) / (1.0 - sb["bed_voidage"]),
+            "bulk_density": rho_b, "bed_volume": volume, "cross_section": area, "sorbent_mass": rho_b * volume}
 
 
 def validate(raw: Dict[str, Any]) -> Config:
     raw = copy.deepcopy(raw)
     raw.pop("derived", None)
-    unknown = [k for k in _leaves({k: v for k, v in raw.items() if k != "cycle"}) if k not in KEYS]
+    unknown = [k for k in _leaf_keys({k: v for k, v in raw.items() if k != "cycle"}) if k not in SPECS]
     if unknown:
-        raise ConfigError(f"unknown configuration keys (typo?): {unknown}")
-    for key in KEYS:
+        raise ConfigError(f"unknown configuration keys {unknown}")
+    for key, spec in SPECS.items():
         try:
-            v = get_dotted(raw, key)
+            value = get_dotted(raw, key)
         except (KeyError, TypeError):
-            raise ConfigError(f"missing configuration key: {key}")
-        set_dotted(raw, key, _coerce(key, v))
-    s = raw["sorbent"]
-    if s["total_voidage"] < s["bed_voidage"]:
-        raise ConfigError("sorbent.total_voidage must be >= sorbent.bed_voidage")
-    if raw["run"]["min_cycles"] > raw["run"]["max_cycles"]:
-        raise ConfigError("run.min_cycles > run.max_cycles")
-    th, zc = raw["jacket"]["theta_coverage_deg"], raw["jacket"]["z_coverage"]
-    if len(th) != 2 or not (0 <= th[0] < th[1] <= 360):
-        raise ConfigError("jacket.theta_coverage_deg must be [start, end] with 0 <= start < end <= 360")
-    if len(zc) != 2 or not (0 <= zc[0] < zc[1] <= 1):
-        raise ConfigError("jacket.z_coverage must be [start, end] with 0 <= start < end <= 1")
-    if raw["operation"]["P_vacuum"] >= raw["ambient"]["P"]:
-        raise ConfigError("operation.P_vacuum must be below ambient.P")
-    if raw["grid"]["Ntheta"] == 2:
-        raise ConfigError("grid.Ntheta = 2 is degenerate; use 1 (axisymmetric) or >= 3")
-    raw["cycle"] = _cycle(raw)
-    eb, et = s["bed_voidage"], s["total_voidage"]
-    R, Lb = raw["geometry"]["bed_radius"], raw["geometry"]["bed_length"]
-    d = {"particle_porosity": (et - eb) / (1.0 - eb),
-         "bulk_density": s["particle_density"] * (1.0 - eb),
-         "bed_volume": math.pi * R ** 2 * Lb,
-         "cross_section": math.pi * R ** 2}
-    d["sorbent_mass"] = d["bulk_density"] * d["bed_volume"]
-    raw["derived"] = d
+            raise ConfigError(f"configuration key {key} is missing") from None
+        set_dotted(raw, key, spec.check(key, value))
+    for ok, message in RULES:
+        if not ok(raw):
+            raise ConfigError(message)
+    raw["cycle"] = _read_cycle(raw)
+    raw["derived"] = _derived(raw)
     return Config(raw)
 
 
@@ -408,11 +477,12 @@ def load(user_yaml: Optional[str] = None, overrides: Optional[Sequence[str]] = N
         with open(user_yaml) as fh:
             user = yaml.safe_load(fh) or {}
         user.pop("derived", None)
-        if "cycle" in user:
-            raw["cycle"] = user.pop("cycle")
+        cycle = user.pop("cycle", None)
         raw = deep_merge(raw, user)
-    for k, v in (override_dict or {}).items():
-        set_dotted(raw, k, v)
-    for ov in overrides or []:
-        set_dotted(raw, *parse_override(ov))
+        if cycle is not None:
+            raw["cycle"] = cycle
+    for key, value in (override_dict or {}).items():
+        set_dotted(raw, key, value)
+    for text in overrides or []:
+        set_dotted(raw, *parse_override(text))
     return validate(raw)

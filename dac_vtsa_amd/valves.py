@@ -1,7 +1,17 @@
-"""Valve states -> end-of-column boundary conditions (upstream valves/valve_logic.py:40-148).
+"""End-of-column boundary conditions of a cycle step (upstream valves/valve_logic.py:40-148).
 
-Evaluated once per step on the host; the resulting kinds and parameters go into the device
-model descriptor.
+A step opens or closes the valves V1..V3; each valve sits at one end of the column
+(``cfg.ports``).  Per end, the open valve there decides the boundary-condition kind handed to
+the device model:
+
+* no open valve -> ``closed`` (no flow);
+* ``OPEN`` (the feed blower) -> ``feed``: fixed molar inflow of feed gas;
+* ``OPEN_ATMOSPHERIC`` -> ``valve`` (linear conductance to a header) or ``pressure`` (ramped
+  Dirichlet), towards the feed header for V1 and the ambient for the others;
+* ``OPEN_VACUUM`` -> ``pump`` (finite pump speed, optional check valve) or ``pressure``.
+
+Evaluated once per step on the host; the kinds and their parameters go into the device
+descriptor.  The feed profile gives the inlet distribution over the end face.
 """
 from __future__ import annotations
 
@@ -13,10 +23,12 @@ import numpy as np
 from .properties import R_GAS
 
 KIND_CODE = {"closed": 0, "feed": 1, "pump": 2, "valve": 3, "pressure": 4}
+VALVES = ("V1", "V2", "V3")
 
 
 @dataclass
 class EndCondition:
+    """Boundary condition of one end: kind, the external state it connects to, pump/valve data."""
     end: str
     valve: Optional[str]
     state: str
@@ -32,76 +44,100 @@ class EndCondition:
 
 
 def humid_air_composition(species, P, T, RH, y_co2_dry, p_sat):
+    """Mole fractions of humid air in the order of ``species``: water from the relative humidity,
+    CO2 from its dry-basis fraction, the inert gas the rest."""
     y_w = RH * p_sat / P if "H2O" in species else 0.0
-    y = {"H2O": y_w, "CO2": y_co2_dry * (1.0 - y_w)}
-    y["N2"] = 1.0 - y["CO2"] - y_w
-    return np.array([y[s] for s in species])
+    y_c = y_co2_dry * (1.0 - y_w)
+    table = {"CO2": y_c, "H2O": y_w, "N2": 1.0 - y_c - y_w}
+    return np.array([table[s] for s in species])
+
+
+def _open_valve(cfg, step, end):
+    """The valve that is open at ``end`` during the step (the last of V1..V3 when several are)."""
+    found = None
+    for v in VALVES:
+        if cfg.ports[v] == end and step["valves"][v] != "CLOSED":
+            found = v
+    return found
 
 
 def end_conditions(cfg, step: Dict, species, props) -> Dict[str, EndCondition]:
-    fd, amb, op = cfg.feed, cfg.ambient, cfg.operation
+    """{"top": EndCondition, "bottom": EndCondition} of one step."""
     water = "H2O" in species
-    y_feed = humid_air_composition(species, fd.P, fd.T, fd.RH if water else 0.0, fd.y_co2_dry, float(props.p_sat(fd.T)))
-    y_amb = humid_air_composition(species, amb.P, amb.T, amb.RH if water else 0.0, amb.y_co2_dry,
-                                  float(props.p_sat(amb.T)))
-    out = {}
+    feed, amb, op = cfg.feed, cfg.ambient, cfg.operation
+
+    def air(src):
+        return humid_air_composition(species, src.P, src.T, src.RH if water else 0.0, src.y_co2_dry,
+                                     float(props.p_sat(src.T)))
+
+    feed_gas = (feed.P, air(feed), feed.T)
+    ambient_gas = (amb.P, air(amb), amb.T)
+    product = step.get("product_port", "V3")
+    conds = {}
     for end in ("top", "bottom"):
-        valve = None
-        for v in ("V1", "V2", "V3"):
-            if cfg.ports[v] == end and step["valves"][v] != "CLOSED":
-                valve = v
-        if valve is None:
-            out[end] = EndCondition(end, None, "CLOSED", "closed")
+        v = _open_valve(cfg, step, end)
+        if v is None:
+            conds[end] = EndCondition(end, None, "CLOSED", "closed")
             continue
-        st = step["valves"][valve]
-        is_prod = step.get("product_port", "V3") == valve
-        if st == "OPEN":
-            out[end] = EndCondition(end, valve, st, "feed", fd.P, y_feed, fd.T, is_feed=True)
-        elif st == "OPEN_ATMOSPHERIC":
-            kind = "valve" if op.vent_model == "valve" else "pressure"
-            if valve == "V1":
-                out[end] = EndCondition(end, valve, st, kind, fd.P, y_feed, fd.T, is_product=is_prod, is_feed=True,
-                                        speed=op.vent_conductance)
-            else:
-                out[end] = EndCondition(end, valve, st, kind, amb.P, y_amb, amb.T, is_product=is_prod,
-                                        speed=op.vent_conductance)
+        state = step["valves"][v]
+        if state == "OPEN":
+            P, y, T = feed_gas
+            conds[end] = EndCondition(end, v, state, "feed", P, y, T, is_feed=True)
+        elif state == "OPEN_ATMOSPHERIC":
+            P, y, T = feed_gas if v == "V1" else ambient_gas
+            conds[end] = EndCondition(end, v, state, "valve" if op.vent_model == "valve" else "pressure", P, y, T,
+                                      is_product=product == v, is_feed=v == "V1", speed=op.vent_conductance)
         else:
-            kind = "pump" if op.vacuum_model == "pump" else "pressure"
-            out[end] = EndCondition(end, valve, st, kind, op.P_vacuum, y_amb, amb.T,
-                                    check_valve=bool(op.vacuum_check_valve), is_product=is_prod, is_vacuum=True,
-                                    speed=op.pump_speed_per_area)
-    return out
+            _, y, T = ambient_gas
+            conds[end] = EndCondition(end, v, state, "pump" if op.vacuum_model == "pump" else "pressure",
+                                      op.P_vacuum, y, T, check_valve=bool(op.vacuum_check_valve),
+                                      is_product=product == v, is_vacuum=True, speed=op.pump_speed_per_area)
+    return conds
 
 
 def feed_molar_flux(cfg):
+    """Molar flux of feed gas per unit cross-section, u_feed P_feed / (R T_feed)."""
     return cfg.feed.velocity * cfg.feed.P / (R_GAS * cfg.feed.T)
 
 
+def _sector_fraction(grid, centre_deg, width_deg):
+    """Fraction of each theta sector's angle inside the sector [c - w/2, c + w/2] (periodic)."""
+    half = 0.5 * np.radians(width_deg)
+    c = np.radians(centre_deg)
+    frac = np.zeros(grid.Nth)
+    for j in range(grid.Nth):
+        a, b = grid.th_f[j] - c, grid.th_f[j + 1] - c
+        inside = 0.0
+        for wrap in (-2.0 * np.pi, 0.0, 2.0 * np.pi):
+            inside += max(0.0, min(b + wrap, half) - max(a + wrap, -half))
+        frac[j] = inside / grid.dth
+    return frac
+
+
 def feed_profile(cfg, grid) -> np.ndarray:
-    """Inlet distribution (Nr, Nth) with area-weighted mean 1."""
+    """Inlet distribution over the end face, shape (Nr, Nth), normalised to an area-weighted mean
+    of 1 (``feed.profile``: uniform, cosine_theta, linear_r, central_disc, sector)."""
     fd = cfg.feed
-    A = fd.profile_amplitude
-    rr, th = np.meshgrid(grid.r_c, grid.th_c, indexing="ij")
-    if fd.profile == "uniform":
-        f = np.ones_like(rr)
-    elif fd.profile == "cosine_theta":
-        f = 1.0 + A * np.cos(th - np.radians(fd.profile_theta0_deg))
-    elif fd.profile == "linear_r":
-        f = 1.0 + A * (2.0 * rr / grid.R - 1.0)
-    elif fd.profile == "central_disc":
+    shape = (grid.Nr, grid.Nth)
+    r = np.broadcast_to(np.asarray(grid.r_c, dtype=float)[:, None], shape)
+    th = np.broadcast_to(np.asarray(grid.th_c, dtype=float)[None, :], shape)
+    kind = fd.profile
+    if kind == "uniform":
+        f = np.ones(shape)
+    elif kind == "cosine_theta":
+        f = 1.0 + fd.profile_amplitude * np.cos(th - np.radians(fd.profile_theta0_deg))
+    elif kind == "linear_r":
+        f = 1.0 + fd.profile_amplitude * (2.0 * r / grid.R - 1.0)
+    elif kind == "central_disc":
+        # share of each ring's area inside the disc r <= r_in
         r_in = fd.profile_radius_fraction * grid.R
-        lo, hi = grid.r_f[:-1], grid.r_f[1:]
-        frac = np.clip((np.minimum(hi, r_in) ** 2 - lo ** 2) / (hi ** 2 - lo ** 2), 0.0, 1.0)
-        f = np.repeat(frac[:, None], grid.Nth, axis=1)
-    elif fd.profile == "sector":
-        half = 0.5 * np.radians(fd.profile_sector_deg)
-        t0 = np.radians(fd.profile_theta0_deg)
-        lo, hi = grid.th_f[:-1] - t0, grid.th_f[1:] - t0
-        ov = np.zeros(grid.Nth)
-        for sh in (-2 * np.pi, 0.0, 2 * np.pi):
-            ov += np.clip(np.minimum(hi + sh, half) - np.maximum(lo + sh, -half), 0.0, None)
-        f = np.repeat((ov / grid.dth)[None, :], grid.Nr, axis=0)
+        inner, outer = np.asarray(grid.r_f[:-1]), np.asarray(grid.r_f[1:])
+        share = (np.minimum(outer, r_in) ** 2 - inner ** 2) / (outer ** 2 - inner ** 2)
+        f = np.broadcast_to(np.clip(share, 0.0, 1.0)[:, None], shape).copy()
+    elif kind == "sector":
+        f = np.broadcast_to(_sector_fraction(grid, fd.profile_theta0_deg, fd.profile_sector_deg)[None, :],
+                            shape).copy()
     else:
-        raise ValueError(fd.profile)
-    w = grid.Az * np.ones_like(f)
-    return f * w.sum() / (f * w).sum()
+        raise ValueError(kind)
+    area = grid.Az * np.ones(shape)
+    return f * area.sum() / (f * area).sum()

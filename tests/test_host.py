@@ -112,6 +112,31 @@ def test_state_csv_roundtrip_is_exact():
         assert np.array_equal(getattr(st, a), getattr(st2, a)), a
 
 
+def test_reads_and_rewrites_an_upstream_written_state_file():
+    """tests/golden/state_ref.csv (+ _wall) was written by the upstream save_state_csv from the
+    3x4x6 test state: it loads bit-exactly into that state, and writing the loaded state back
+    reproduces both files byte for byte."""
+    from dac_vtsa_amd.core.grid import Grid
+    from dac_vtsa_amd.simio.checkpoint import load_state_csv, save_state_csv
+    from tests._helpers import GOLDEN, SMALL, golden
+    cfg = load_config(override_dict=SMALL)
+    g = Grid.from_config(cfg)
+    ref = os.path.join(GOLDEN, "state_ref.csv")
+    st = load_state_csv(ref, g, ["CO2", "H2O", "N2"], ["CO2", "H2O"])
+    z = golden("rhs_small.npz")
+    for a in ("c", "q", "T", "Tw"):
+        assert np.array_equal(getattr(st, a), z["base/" + a]), a
+    with tempfile.TemporaryDirectory() as d:
+        p = save_state_csv(st, g, os.path.join(d, "state_ref.csv"))
+        for suffix in ("", "_wall"):
+            mine = open(p[:-4] + suffix + ".csv", "rb").read()
+            theirs = open(ref[:-4] + suffix + ".csv", "rb").read()
+            assert mine == theirs, suffix
+    with pytest.raises(ValueError):
+        load_state_csv(ref, Grid.from_config(load_config(override_dict={**SMALL, "grid.Nz": 7})),
+                       ["CO2", "H2O", "N2"], ["CO2", "H2O"])
+
+
 # ------------------------------------------------------------------ Jacobian pattern and block gather
 @pytest.mark.parametrize("dims,wall", [((3, 4, 6), True), ((4, 4, 20), True), ((2, 1, 4), True), ((1, 3, 5), False),
                                        ((3, 5, 2), True), ((2, 3, 3), False)])
