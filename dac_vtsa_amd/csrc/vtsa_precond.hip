@@ -612,31 +612,76 @@ template <typename T, bool FORWARD, int STRIDE>
 __device__ inline void pair_recurrence(const T* Pl, int NP, int off, const double* rhs, double* out) {
   const int rho = threadIdx.x;
   struct Row { double g[12]; double b; };
-  auto ld = [&](int K, Row& w) {
-    const T* R = Pl + (size_t)K * STRIDE + off + rho * 12;
-#pragma unroll
-    for (int c = 0; c < 12; ++c) w.g[c] = (double)R[c];
-    w.b = rhs[K * 12 + rho];
-  };
   const int first = FORWARD ? 0 : NP - 1, step = FORWARD ? 1 : -1;
   double s = 0.0;   // lane rho: component rho of the previous pair's solution
-  Row wa, wb;
-  ld(first, wa);
-  out[NP * 12 + (rho < 6 ? rho : rho - 6)] = 0.0;   // spare slot
-  int K = first, left = NP;
-  for (; left >= 2; K += 2 * step, left -= 2) {
-    ld(K + step, wb);
-    __builtin_amdgcn_sched_barrier(0);
-    s = bsub_pair(wa.b, s, wa.g);
-    out[K * 12 + rho] = s;
-    __builtin_amdgcn_sched_barrier(0);
-    ld(left > 2 ? K + 2 * step : K + step, wa);
-    __builtin_amdgcn_sched_barrier(0);
-    s = bsub_pair(wb.b, s, wb.g);
-    out[(K + step) * 12 + rho] = s;
-    __builtin_amdgcn_sched_barrier(0);
+  if constexpr (sizeof(T) == sizeof(double)) {
+    auto ld = [&](int K, Row& w) {
+      const T* R = Pl + (size_t)K * STRIDE + off + rho * 12;
+#pragma unroll
+      for (int c = 0; c < 12; ++c) w.g[c] = (double)R[c];
+      w.b = rhs[K * 12 + rho];
+    };
+    Row wa, wb;
+    ld(first, wa);
+    out[NP * 12 + (rho < 6 ? rho : rho - 6)] = 0.0;   // spare slot
+    int K = first, left = NP;
+    for (; left >= 2; K += 2 * step, left -= 2) {
+      ld(K + step, wb);
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(wa.b, s, wa.g);
+      out[K * 12 + rho] = s;
+      __builtin_amdgcn_sched_barrier(0);
+      ld(left > 2 ? K + 2 * step : K + step, wa);
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(wb.b, s, wb.g);
+      out[(K + step) * 12 + rho] = s;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (left == 1) out[K * 12 + rho] = bsub_pair(wa.b, s, wa.g);
+  } else {
+    // Narrow rows: loaded two pairs ahead and widened one pair ahead, so that the conversions
+    // read registers whose loads were issued a whole pair step earlier instead of waiting for
+    // the load just issued.  Look-ahead indices past the end are clamped to the last pair.
+    struct Raw { T g[12]; double b; };
+    auto issue = [&](int K, Raw& w) {
+      const T* R = Pl + (size_t)K * STRIDE + off + rho * 12;
+#pragma unroll
+      for (int c = 0; c < 12; ++c) w.g[c] = R[c];
+      w.b = rhs[K * 12 + rho];
+    };
+    auto widen = [&](const Raw& r, Row& w) {
+#pragma unroll
+      for (int c = 0; c < 12; ++c) w.g[c] = (double)r.g[c];
+      w.b = r.b;
+    };
+    const int last = FORWARD ? NP - 1 : 0;
+    auto ahead = [&](int K, int d) {   // K + d steps, clamped to the last pair
+      const int k2 = K + d * step;
+      return FORWARD ? (k2 > last ? last : k2) : (k2 < last ? last : k2);
+    };
+    Raw ra, rb;
+    Row ca, cb;
+    issue(first, ra);
+    widen(ra, ca);
+    issue(ahead(first, 1), rb);
+    out[NP * 12 + (rho < 6 ? rho : rho - 6)] = 0.0;   // spare slot
+    int K = first, left = NP;
+    for (; left >= 2; K += 2 * step, left -= 2) {
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(ca.b, s, ca.g);
+      out[K * 12 + rho] = s;
+      __builtin_amdgcn_sched_barrier(0);
+      widen(rb, cb);
+      issue(ahead(K, 2), ra);
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(cb.b, s, cb.g);
+      out[(K + step) * 12 + rho] = s;
+      __builtin_amdgcn_sched_barrier(0);
+      widen(ra, ca);
+      issue(ahead(K, 3), rb);
+    }
+    if (left == 1) out[K * 12 + rho] = bsub_pair(ca.b, s, ca.g);
   }
-  if (left == 1) out[K * 12 + rho] = bsub_pair(wa.b, s, wa.g);
 }
 
 // ------------------------------------------------------------------ z-line / block-Jacobi apply
@@ -791,7 +836,8 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
 // unscaled, U = S0 x~, so the couplings to other lines need only J and U:
 //     (M~ x~)_R = x~_R - (dh / s0_R) sum_C J_RC U_C      (R, C in different lines)
 struct GsArgs {
-  const double* Jo; const double* Jcw; const double* Jw; const double* Jwc; const double* s0;
+  const void* Jo;   // off-line blocks in the factor type of the half-sweep (double or float)
+  const double* Jcw; const double* Jw; const double* Jwc; const double* s0;
   double* bs; double* U; const int* order; double dh; int first; const double* Tw;
 };
 
@@ -828,7 +874,8 @@ __global__ void k_gs_gather(int N, ApplyArgs a, const double* __restrict__ s0, d
 // them with consecutive cells in consecutive lanes.  OT cells per block through LDS; once per
 // preconditioner set-up.
 constexpr int OT = 32;
-__global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restrict__ Jc, double* __restrict__ Jo) {
+template <typename TO>
+__global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restrict__ Jc, TO* __restrict__ Jo) {
   __shared__ double t[144][OT + 1];
   const int X0 = blockIdx.x * OT;
   const int nc = min(OT, n - X0);
@@ -839,7 +886,7 @@ __global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restr
   __syncthreads();
   for (int e = threadIdx.x; e < 144 * OT; e += blockDim.x) {
     const int v = e / OT, x = e - v * OT;
-    if (x < nc) Jo[(size_t)v * n + X0 + x] = t[v][x];
+    if (x < nc) Jo[(size_t)v * n + X0 + x] = (TO)t[v][x];
   }
 }
 
@@ -949,10 +996,10 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
         double jv[4][6], uv[4][6];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double* Jr = g.Jo + (size_t)(q * 36 + r * 6) * n + X;   // coalesced over k
+          const T* Jr = reinterpret_cast<const T*>(g.Jo) + (size_t)(q * 36 + r * 6) * n + X;   // coalesced over k
           const int Y = (nb[q] >= 0 ? nb[q] : X0) + k;
 #pragma unroll
-          for (int c = 0; c < 6; ++c) { jv[q][c] = Jr[(size_t)c * n]; uv[q][c] = g.U[c * n + Y]; }
+          for (int c = 0; c < 6; ++c) { jv[q][c] = (double)Jr[(size_t)c * n]; uv[q][c] = g.U[c * n + Y]; }
         }
         double jw = 0.0, uw = 0.0;
         if (nw > 0) {
@@ -1157,7 +1204,12 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
                        dh, (double*)h.F);
   if (h.nw > 0) hipLaunchKernelGGL(k_prec_wall, dim3((h.nw + 255) / 256), dim3(256), 0, s, h.nw, h.Jw, dh, h.Fw);
   if (kind == VTSA_PREC_ZLINE_GS) {
-    hipLaunchKernelGGL(k_gs_offline, dim3((h.n + OT - 1) / OT), dim3(256), 0, s, h.n, h.Jc, h.Jo);
+    // the off-line blocks are stored in the precision of the factors (the FP32 form halves the
+    // half-sweep's right-hand-side traffic)
+    if (prec == VTSA_FP32)
+      hipLaunchKernelGGL(k_gs_offline<float>, dim3((h.n + OT - 1) / OT), dim3(256), 0, s, h.n, h.Jc, (float*)h.Jo);
+    else
+      hipLaunchKernelGGL(k_gs_offline<double>, dim3((h.n + OT - 1) / OT), dim3(256), 0, s, h.n, h.Jc, h.Jo);
     if (h.nw > 0)
       hipLaunchKernelGGL(k_gs_wall_factor, dim3((h.desc.Nth + 63) / 64), dim3(64), 0, s, h.desc.Nth, h.desc.Nz, h.n,
                          h.Jw, h.s0, dh, h.Tw);
@@ -1202,7 +1254,7 @@ static double gs_bytes(const Handle& h, int c, bool first, size_t fsz, int mode,
   const double per_line = (mode >= 1 ? (double)npairs(h.desc.Nz) * PP : Nz * 180.0) * fsz;
   const double per_unknown = (fv ? 8.0 * (1 + (a.pre ? 1 : 0) + (a.store_in ? 1 : 0)) : 0.0) +
                              (fin ? 8.0 * (1 + (a.post ? 1 : 0) + (a.red_s ? 1 : 0)) : 0.0);
-  const double per_cell = 8.0 * 18 + (first ? 0.0 : 8.0 * (144 + 24)) + 6.0 * per_unknown;
+  const double per_cell = 8.0 * 18 + (first ? 0.0 : 144.0 * fsz + 8.0 * 24) + 6.0 * per_unknown;
   const double per_outer = first ? 0.0 : 8.0 * 5;
   const double per_wall = 8.0 * 6 + (first ? 0.0 : 8.0 * 16) + per_unknown;
   return h.gs_ncell[c] * per_line + cells * per_cell + outer * per_outer + walls * per_wall;
