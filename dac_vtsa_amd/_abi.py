@@ -81,6 +81,9 @@ _SIGS = {
     "vtsa_scale": [P, P, D, P, P],
     "vtsa_rms": [P, P, P, I32, C.POINTER(D), P],
     "vtsa_rms_noalloc": [P, P, I32, P, P],
+    "vtsa_gmres_hessenberg": [P, P, C.POINTER(I32)],
+    "vtsa_defl_set": [P, I32, I32, P, P, P],
+    "vtsa_defl_apply": [P, P, P, P],
     "vtsa_newton_eval": [P, D, P, D, P, P, P, P, C.POINTER(D), C.POINTER(D), P],
     "vtsa_newton_residual": [P, D, P, D, P, P, P, P, P, P],
     "vtsa_newton_try": [P, D, P, P, D, D, P, P, P, P, P, C.POINTER(D), C.POINTER(D), C.POINTER(D), P],

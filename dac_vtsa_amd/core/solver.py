@@ -38,6 +38,64 @@ SW3 = (2.0 - 3.0 * G_SD) / (6.0 * (1.0 - G_SD))
 
 PRECONDITIONERS = {"block_jacobi": 0, "zline": 1, "zline_gs": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1}
+MAX_RESTART = 64      # columns of the device Hessenberg (vtsa_gmres_hessenberg)
+LOWRANK_MAX = 16      # largest rank of the device low-rank correction
+
+
+def lowrank_correction(H, m, k, theta_min=0.0, theta_max=np.inf):
+    """Low-rank right-preconditioner correction from one Arnoldi cycle (no upstream counterpart).
+
+    H: the (m+1) x m Arnoldi Hessenberg of the right-preconditioned operator B on the basis V_m.
+    The harmonic Ritz pairs of B on span V_m solve (H_m + h^2 H_m^-T e_m e_m^T) g = theta g with
+    H_m the square part and h = H[m, m-1] (Morgan, SIAM J. Matrix Anal. Appl. 16 (1995) 1154).
+    The k values of smallest modulus within [theta_min, theta_max] (a complex pair is taken
+    whole) give a real orthonormal G (m x kk, kk <= k + 1) spanning their vectors; with
+    U = V_m G the correction is Q = I + U D U^T, D = T^-1 - I, T = G^T H_m G = U^T B U, so that B Q has eigenvalues near 1
+    on span U.  Q stretches span U by up to 1/theta_min.  Returns (G, D), or None when no value
+    qualifies or H_m or T is singular to working precision."""
+    if m < 2 or k < 1:
+        return None
+    Hm = np.asarray(H[:m, :m], dtype=float)
+    hl = float(H[m, m - 1])
+    em = np.zeros(m)
+    em[-1] = 1.0
+    try:
+        F = Hm + hl * hl * np.outer(np.linalg.solve(Hm.T, em), em)
+        theta, vecs = np.linalg.eig(F)
+    except np.linalg.LinAlgError:
+        return None
+    if not np.all(np.isfinite(theta)):
+        return None
+    mod = np.abs(theta)
+    order = [i for i in np.argsort(mod, kind="stable") if theta_min <= mod[i] <= theta_max]
+    if not order:
+        return None
+    cols, taken = [], set()
+    for i in order:
+        if len(cols) >= k:
+            break
+        if i in taken:
+            continue
+        taken.add(i)
+        v = vecs[:, i]
+        if abs(theta[i].imag) > 1e-12 * abs(theta[i]):
+            # its conjugate partner spans the same real plane
+            j = min((j for j in order if j not in taken), key=lambda j: abs(theta[j] - np.conj(theta[i])),
+                    default=None)
+            if j is not None:
+                taken.add(j)
+            cols += [v.real, v.imag]
+        else:
+            cols.append(v.real)
+    if len(cols) > LOWRANK_MAX:
+        cols = cols[:LOWRANK_MAX]
+    G, R = np.linalg.qr(np.stack(cols, axis=1))
+    sv = np.abs(np.diag(R))
+    G = G[:, sv > 1e-10 * sv.max()]
+    T = G.T @ Hm @ G
+    if np.linalg.cond(T) > 1e12:
+        return None
+    return G, np.linalg.inv(T) - np.eye(G.shape[1])
 
 
 class SolverError(RuntimeError):
@@ -70,6 +128,13 @@ class DeviceOps:
         self.m = model
         self.h = model.handle
         self.N = model.n_core
+        # low-rank correction: rank asked for (0 off), the smallest cycle it is built from, and
+        # whether the device holds one for the current factorisation
+        self.lowrank = 0
+        self.lowrank_window = (0.0, np.inf)
+        self.lowrank_min_iters = 8
+        self.lowrank_on = False
+        self.lowrank_built = 0
 
     def st(self):
         return dv.stream()
@@ -119,7 +184,44 @@ class DeviceOps:
         _abi.call("vtsa_gmres", self.h, float(t), dv.ptr(z), dv.ptr(f), dv.ptr(s), float(dh), float(zs_norm),
                   dv.ptr(b), dv.ptr(w), float(rtol), float(atol), int(restart), C.byref(info), self.st())
         self.m.nfev += info.iters + 1
+        if self.lowrank and not self.lowrank_on and info.iters >= self.lowrank_min_iters:
+            self.lowrank_from_last_cycle(self.lowrank)
         return info
+
+    def hessenberg(self):
+        """(H, m): the Arnoldi Hessenberg ((m+1) x m) of the last GMRES cycle."""
+        buf = (C.c_double * ((MAX_RESTART + 1) * MAX_RESTART))()
+        m = C.c_int32()
+        _abi.call("vtsa_gmres_hessenberg", self.h, buf, C.byref(m))
+        H = np.frombuffer(buf, dtype=float).reshape(MAX_RESTART + 1, MAX_RESTART)
+        return H[:m.value + 1, :m.value].copy(), m.value
+
+    def lowrank_set(self, G, Dm):
+        """Install Q = I + (V_m G) Dm (V_m G)^T, V_m the basis of the last GMRES cycle."""
+        m, k = G.shape
+        Gc = (C.c_double * (m * k))(*np.ascontiguousarray(G, dtype=float).ravel())
+        Dc = (C.c_double * (k * k))(*np.ascontiguousarray(Dm, dtype=float).ravel())
+        _abi.call("vtsa_defl_set", self.h, int(k), int(m), Gc, Dc, self.st())
+        self.lowrank_on = True
+        self.lowrank_built += 1
+
+    def lowrank_clear(self):
+        _abi.call("vtsa_defl_set", self.h, 0, 0, None, None, self.st())
+        self.lowrank_on = False
+
+    def lowrank_from_last_cycle(self, k):
+        """Build the rank-k correction from the last GMRES cycle's Hessenberg and basis; returns
+        whether one was installed."""
+        H, m = self.hessenberg()
+        gd = lowrank_correction(H, m, k, *self.lowrank_window)
+        if gd is None:
+            return False
+        self.lowrank_set(*gd)
+        return True
+
+    def lowrank_apply(self, v, out):
+        _abi.call("vtsa_defl_apply", self.h, dv.ptr(v), dv.ptr(out), self.st())
+        return out
 
     def prec_apply(self, r, out):
         _abi.call("vtsa_precond_apply", self.h, dv.ptr(r), dv.ptr(out), self.st())
@@ -162,8 +264,10 @@ class DeviceOps:
         return en.value
 
     def precond_setup(self, dh, scale, kind, precision, sweeps=1):
+        """New preconditioner factors; removes the low-rank correction of the previous ones."""
         _abi.call("vtsa_precond_setup", self.h, float(dh), dv.ptr(scale), int(kind), int(precision), int(sweeps),
                   self.st())
+        self.lowrank_on = False
 
 
 def _finite(x):
@@ -477,14 +581,21 @@ class TRBDF2:
     stage that fails with a fresh Jacobian raises SolverError) and ``error_filter`` ("precond":
     the local error filtered by the preconditioner, standing in for the upstream LU solve;
     "exact": by a GMRES(64) solve of (I - D h J) e = z2 - y3 to 1e-10 relative); ``kappa``
-    overrides solver.newton_kappa (whose configured minimum is 1e-4) for such comparisons.  The
+    overrides solver.newton_kappa (whose configured minimum is 1e-4) for such comparisons.
+    ``lowrank`` (k > 0): after each new factorisation, the first GMRES cycle of at least 8
+    iterations yields a rank-k correction of the preconditioner (lowrank_correction) from its
+    harmonic Ritz values of modulus within ``lowrank_window``, used by every later cycle until the
+    next factorisation.  The lower bound keeps the correction's stretch of the Krylov vectors
+    (up to 1/theta) moderate: the finite-difference matvec of a vector stretched by 1e4 or more
+    loses the true-residual test and forces Jacobian refreshes.  The
     preconditioner stands in for the upstream SuperLU factor of the theta-block matrix wherever
     that factor is used: GMRES, the Newton residual norm and the error-estimate filter."""
 
     def __init__(self, model, cfg, h_max: float, h_init: Optional[float] = None, logger=None,
                  precond: str = "zline_gs", precision: str = "fp64", krylov_dim: int = 30,
                  gs_sweeps: int = 2, conserve: str = "polish", guess=None, record=None, h_sequence=None,
-                 error_filter: str = "precond", kappa: Optional[float] = None):
+                 error_filter: str = "precond", kappa: Optional[float] = None, lowrank: int = 0,
+                 lowrank_window=(1e-3, 0.3)):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -522,6 +633,10 @@ class TRBDF2:
         self.cap_after_fail = 0.5
         self.cap_relax = 1.15
         self.ops = DeviceOps(model)
+        if not 0 <= int(lowrank) <= LOWRANK_MAX:
+            raise ValueError(f"lowrank={lowrank!r}: 0..{LOWRANK_MAX}")
+        self.ops.lowrank = int(lowrank)
+        self.ops.lowrank_window = (float(lowrank_window[0]), float(lowrank_window[1]))
         self.record = record
         self.guess = None
         if guess is not None:
@@ -795,6 +910,8 @@ class TRBDF2:
         res.y = y if device_result else dv.to_host(y)
         res.acc, res.t = acc, t
         self.stats["gmres_iters"] = self._gm[0]
+        if ops.lowrank:
+            self.stats["lowrank_built"] = ops.lowrank_built
         res.nfev = m.nfev
         res.wall_time = time.time() - wall0
         return res

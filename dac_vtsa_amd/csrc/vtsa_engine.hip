@@ -35,13 +35,14 @@ static void free_handle(Handle* h) {
   if (!h) return;
   double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2, h->FF,
                     h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
-                    h->Vb, h->red.partial, h->red.result};
+                    h->Vb, h->red.partial, h->red.result, h->Ud, h->defl_small, h->defl_partial, h->defl_tmp};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (h->F) (void)hipFree(h->F);
   if (h->P) (void)hipFree(h->P);
   if (h->gs) (void)hipFree(h->gs);
   if (h->red.counter) (void)hipFree(h->red.counter);
+  if (h->defl_counter) (void)hipFree(h->defl_counter);
   if (h->gs_host) (void)hipHostFree(h->gs_host);
   if (h->host_scalars) (void)hipHostFree(h->host_scalars);
   if (h->progress) (void)hipHostFree(h->progress);
@@ -272,6 +273,38 @@ int vtsa_precond_setup(vtsa_handle* H, double dh, const double* scale, int32_t k
   if (precision != VTSA_FP64 && precision != VTSA_FP32) { set_error("unknown precision"); return -2; }
   if (sweeps < 1 || sweeps > 16) { set_error("vtsa_precond_setup: sweeps must be in 1..16"); return -2; }
   launch_prec_setup(H->h, dh, scale, kind, precision, sweeps, S(s));
+  H->h.defl_k = 0;   // a new factorisation invalidates the low-rank correction built for the old one
+  return finish(S(s));
+}
+
+int vtsa_gmres_hessenberg(vtsa_handle* H, double* Hr, int32_t* m) {
+  if (!H || !Hr || !m) { set_error("vtsa_gmres_hessenberg: null argument"); return -1; }
+  const GmresScalars& G = *H->h.gs_host;
+  *m = G.iters;
+  for (int r = 0; r <= MAX_RESTART; ++r)
+    for (int c = 0; c < MAX_RESTART; ++c) Hr[r * MAX_RESTART + c] = G.Hr[r][c];
+  return 0;
+}
+
+int vtsa_defl_set(vtsa_handle* H, int32_t k, int32_t m, const double* G, const double* D, vtsa_stream s) {
+  if (!H) { set_error("vtsa_defl_set: null handle"); return -1; }
+  if (k < 0 || k > DEFL_MAX || (k > 0 && (!G || !D || m < 1 || m > H->h.vb_cols - 1))) {
+    set_error("vtsa_defl_set: k must be in 0..16 with G (m x k) and D (k x k), m within the last basis");
+    return -2;
+  }
+  const int rc = defl_set_impl(H->h, k, m, G, D, S(s));
+  if (rc) return rc;
+  return finish(S(s));
+}
+
+int vtsa_defl_apply(vtsa_handle* H, const double* in, double* out, vtsa_stream s) {
+  if (!H || !in || !out) { set_error("vtsa_defl_apply: null argument"); return -1; }
+  if (H->h.defl_k == 0) {
+    if (in != out && check(hipMemcpyAsync(out, in, sizeof(double) * H->h.N, hipMemcpyDeviceToDevice, S(s)), "copy"))
+      return -1;
+    return finish(S(s));
+  }
+  defl_apply_launch(H->h, in, nullptr, nullptr, out, 0, S(s));
   return finish(S(s));
 }
 

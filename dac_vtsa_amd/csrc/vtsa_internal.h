@@ -12,10 +12,12 @@ namespace vtsa {
 constexpr int NACC = 36;
 constexpr int BLK = 256;          // default threads per block for cell-parallel kernels
 constexpr int MAX_RESTART = 64;
+constexpr int DEFL_MAX = 16;      // columns of the low-rank correction of the GMRES preconditioner
 
 // Device-resident scalars of one GMRES cycle and of the reductions around it.
 struct GmresScalars {
   double H[MAX_RESTART + 1][MAX_RESTART];   // H[row][col] (Hessenberg after Givens)
+  double Hr[MAX_RESTART + 1][MAX_RESTART];  // the Arnoldi Hessenberg as computed (before Givens)
   double cs[MAX_RESTART], sn[MAX_RESTART], S[MAX_RESTART + 1], yv[MAX_RESTART];
   double h_tmp[MAX_RESTART + 1];            // Gram-Schmidt coefficients of the current column
   double h0sq;                              // |w|^2 before orthogonalisation
@@ -88,6 +90,14 @@ struct Handle {
   double* host_scalars = nullptr;   // pinned [64]
   int* progress = nullptr;          // coherent pinned host word: GMRES columns completed (< 0: done)
   int* progress_dev = nullptr;      // its device address
+  // low-rank correction Q = I + Ud Dd Ud^T of the GMRES right preconditioner (scaled variables):
+  // Ud [defl_k][N] orthonormal columns, Dd [defl_k][defl_k]; defl_k = 0: none
+  int defl_k = 0;
+  double* Ud = nullptr;             // [DEFL_MAX][N] (allocated on first use)
+  double* defl_small = nullptr;     // device: Dd [DEFL_MAX^2] | d [DEFL_MAX] | G [MAX_RESTART * DEFL_MAX]
+  double* defl_partial = nullptr;   // [DEFL_MAX][RMAXB] reduction partials
+  unsigned* defl_counter = nullptr; // [2]
+  double* defl_tmp = nullptr;       // [N] the corrected Arnoldi vector
 };
 
 void set_error(const std::string& msg);
@@ -121,6 +131,9 @@ int cons_shift_impl(Handle& h, const double* z, const double* s, const double la
 // input to `store_in`; optionally accumulates sum(out^2) and sets sigma in gs (matvec use).
 void launch_prec_apply(Handle& h, const double* in, const double* in_scale, double* store_in, const double* pre,
                        const double* post, double* out, bool sigma_for_matvec, hipStream_t s);
+int defl_set_impl(Handle& h, int k, int m, const double* G, const double* D, hipStream_t st);
+void defl_apply_launch(Handle& h, const double* in, const double* in_scale, double* store, double* out,
+                       int respect_done, hipStream_t st);
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
                 const double* b, double* w, double rtol, double atol, int restart, vtsa_gmres_info* info,
                 hipStream_t st);

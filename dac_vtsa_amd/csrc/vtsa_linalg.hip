@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
   const int t = threadIdx.x;
   double tot = 0.0;
   if (t < 64) tot = wave_sum_strided(partial, gridDim.x);
-  for (int k = t; k <= col; k += blockDim.x) hc[k] = gs->H[k][col];
+  for (int k = t; k <= col; k += blockDim.x) hc[k] = gs->Hr[k][col] = gs->H[k][col];
   for (int k = t; k < col; k += blockDim.x) { csl[k] = gs->cs[k]; snl[k] = gs->sn[k]; }
   __syncthreads();
   if (t == 0) {
@@ -433,6 +433,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
     double h0 = sqrt(gs->h0sq);
     bool brk = h1 <= EPS64 * h0;
     hc[col + 1] = brk ? 0.0 : h1;
+    gs->Hr[col + 1][col] = hc[col + 1];
     gs->inv_h1 = brk ? 0.0 : 1.0 / h1;
     for (int k = 0; k < col; ++k) {
       double c = csl[k], s = snl[k];
@@ -663,6 +664,123 @@ void vec_op_impl(Handle& h, int op, int npos, double a, double b, double c, doub
   hipLaunchKernelGGL(k_vec_op, dim3(eblk(h.N)), dim3(256), 0, st, op, h.N, npos, a, b, c, e, x, y, z, w, out);
 }
 
+// ------------------------------------------------------------------ low-rank correction
+// Q v = v + Ud (Dd (Ud^T v)) in the scaled variables of GMRES.  Ud holds orthonormal approximations
+// of the invariant subspace of the right-preconditioned operator B = S^-1 A M^-1 S that belongs to
+// its eigenvalues nearest zero (harmonic Ritz vectors of an earlier cycle); with
+// Dd = T^-1 - I, T = Ud^T B Ud, the operator B Q maps that subspace onto itself with eigenvalues
+// near 1, so GMRES no longer spends iterations on those modes.
+//
+// k_defl_dots: v = in * scale (stored to `store` when given), c_j = Ud_j . v; the last block forms
+// d = Dd c.  With respect_done, a queued call after the cycle converged returns at once.
+__global__ void __launch_bounds__(RBLK) k_defl_dots(int N, int k, const double* in,
+                                                    const double* __restrict__ in_scale, double* store,
+                                                    const double* __restrict__ Ud, const double* __restrict__ Dd,
+                                                    double* __restrict__ d, const GmresScalars* __restrict__ gs,
+                                                    int respect_done, double* __restrict__ partial,
+                                                    unsigned* counter) {
+  __shared__ double sh[RBLK / 64];
+  __shared__ double cs[DEFL_MAX];
+  __shared__ bool slast;
+  if (respect_done && gs->done) return;
+  const double sc = in_scale ? *in_scale : 1.0;
+  double loc[DEFL_MAX];
+#pragma unroll
+  for (int j = 0; j < DEFL_MAX; ++j) loc[j] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const double v = in[i] * sc;
+    if (store) store[i] = v;
+#pragma unroll
+    for (int j = 0; j < DEFL_MAX; ++j)
+      if (j < k) loc[j] = fma(Ud[(size_t)j * N + i], v, loc[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < DEFL_MAX; ++j) {
+    if (j < k) {   // k is uniform over the grid: every thread takes the same branches
+      const double bs = wave_block_sum(loc[j], sh);
+      if (threadIdx.x == 0) store_partial(partial + (size_t)j * RMAXB + blockIdx.x, bs);
+    }
+  }
+  if (!last_block_totals(counter, &slast, partial, k, cs)) return;
+  if ((int)threadIdx.x < k) {
+    const int r = threadIdx.x;
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc = fma(Dd[r * DEFL_MAX + j], cs[j], acc);
+    d[r] = acc;
+  }
+}
+
+// out = v + Ud d with v = in * scale (out may alias in).
+__global__ void k_defl_axpy(int N, int k, const double* in, const double* __restrict__ in_scale,
+                            const double* __restrict__ Ud, const double* __restrict__ d, double* out,
+                            const GmresScalars* __restrict__ gs, int respect_done) {
+  if (respect_done && gs->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double v = in[i] * (in_scale ? *in_scale : 1.0);
+  for (int j = 0; j < k; ++j) v = fma(Ud[(size_t)j * N + i], d[j], v);
+  out[i] = v;
+}
+
+// Ud_j = sum_i V_i G[i][j] over the first m basis vectors (G row-major m x k, ld DEFL_MAX).
+__global__ void k_defl_build(int N, int m, int k, const double* __restrict__ Vb, size_t ld,
+                             const double* __restrict__ G, double* __restrict__ Ud) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double acc[DEFL_MAX];
+#pragma unroll
+  for (int j = 0; j < DEFL_MAX; ++j) acc[j] = 0.0;
+  for (int r = 0; r < m; ++r) {
+    const double v = Vb[(size_t)r * ld + i];
+#pragma unroll
+    for (int j = 0; j < DEFL_MAX; ++j)
+      if (j < k) acc[j] = fma(v, G[r * DEFL_MAX + j], acc[j]);
+  }
+  for (int j = 0; j < k; ++j) Ud[(size_t)j * N + i] = acc[j];
+}
+
+int defl_set_impl(Handle& h, int k, int m, const double* G, const double* D, hipStream_t st) {
+  if (k == 0) {
+    h.defl_k = 0;
+    return 0;
+  }
+  if (!h.Ud) {
+    if (check(hipMalloc((void**)&h.Ud, sizeof(double) * (size_t)DEFL_MAX * h.N), "hipMalloc")) return -1;
+    if (check(hipMalloc((void**)&h.defl_small, sizeof(double) * (DEFL_MAX * DEFL_MAX + DEFL_MAX +
+                                                                  MAX_RESTART * DEFL_MAX)), "hipMalloc")) return -1;
+    if (check(hipMalloc((void**)&h.defl_partial, sizeof(double) * DEFL_MAX * RMAXB), "hipMalloc")) return -1;
+    if (check(hipMalloc((void**)&h.defl_counter, sizeof(unsigned) * 2), "hipMalloc")) return -1;
+    if (check(hipMemset(h.defl_counter, 0, sizeof(unsigned) * 2), "hipMemset")) return -1;
+    if (check(hipMalloc((void**)&h.defl_tmp, sizeof(double) * h.N), "hipMalloc")) return -1;
+  }
+  std::vector<double> small(DEFL_MAX * DEFL_MAX + DEFL_MAX + MAX_RESTART * DEFL_MAX, 0.0);
+  for (int r = 0; r < k; ++r)
+    for (int c = 0; c < k; ++c) small[r * DEFL_MAX + c] = D[r * k + c];
+  double* Gp = small.data() + DEFL_MAX * DEFL_MAX + DEFL_MAX;
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < k; ++c) Gp[r * DEFL_MAX + c] = G[r * k + c];
+  if (check(hipMemcpyAsync(h.defl_small, small.data(), sizeof(double) * small.size(), hipMemcpyHostToDevice, st),
+            "memcpy"))
+    return -1;
+  // the copy reads host memory that goes out of scope: wait for it before returning
+  if (check(hipStreamSynchronize(st), "sync")) return -1;
+  hipLaunchKernelGGL(k_defl_build, dim3(eblk(h.N)), dim3(256), 0, st, h.N, m, k, h.Vb, (size_t)h.N,
+                     h.defl_small + DEFL_MAX * DEFL_MAX + DEFL_MAX, h.Ud);
+  h.defl_k = k;
+  return check(hipGetLastError(), "k_defl_build");
+}
+
+void defl_apply_launch(Handle& h, const double* in, const double* in_scale, double* store, double* out,
+                       int respect_done, hipStream_t st) {
+  double* d = h.defl_small + DEFL_MAX * DEFL_MAX;
+  hipLaunchKernelGGL(k_defl_dots, dim3(rgrid(h.N)), dim3(RBLK), 0, st, h.N, h.defl_k, in, in_scale, store, h.Ud,
+                     h.defl_small, d, h.gs, respect_done, h.defl_partial, h.defl_counter);
+  const double* v = store ? store : in;
+  const double* vs = store ? nullptr : in_scale;
+  hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(h.N)), dim3(256), 0, st, h.N, h.defl_k, v, vs, h.Ud, d, out, h.gs,
+                     respect_done);
+}
+
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
                 const double* b, double* w, double rtol, double atol, int restart, vtsa_gmres_info* info,
                 hipStream_t st) {
@@ -703,10 +821,17 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
       }
       if (v < 0) break;
     }
-    // u = P^-1 v_col with v_col = w_prev / |w_prev| normalised in place
+    // u = P^-1 v_col with v_col = w_prev / |w_prev| normalised in place (u = P^-1 Q v_col with the
+    // low-rank correction)
     const double* vin = col == 0 ? b : Vb + (size_t)col * ld;
-    ApplyArgs a{vin, &h.gs->inv_h1, Vb + (size_t)col * ld, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 1};
-    apply_launch(h, a, st);
+    if (h.defl_k > 0) {
+      defl_apply_launch(h, vin, &h.gs->inv_h1, Vb + (size_t)col * ld, h.defl_tmp, 1, st);
+      ApplyArgs a{h.defl_tmp, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 1};
+      apply_launch(h, a, st);
+    } else {
+      ApplyArgs a{vin, &h.gs->inv_h1, Vb + (size_t)col * ld, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 1};
+      apply_launch(h, a, st);
+    }
     double* wcol = Vb + (size_t)(col + 1) * ld;
     launch_matvec_residual(h, t, z, U, s, f, dh, wcol, true, st);
     hipLaunchKernelGGL(k_gm_dots, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
@@ -718,6 +843,8 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   }
   hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, h.gs);
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);
+  // with the low-rank correction the solution is x = P^-1 Q w: w is returned as Q w
+  if (h.defl_k > 0) defl_apply_launch(h, w, nullptr, nullptr, w, 0, st);
   // true residual r = b - A P^-1 w (one more matvec, as scipy does after the cycle)
   {
     ApplyArgs a{w, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 0};

@@ -142,6 +142,15 @@ int vtsa_gmres(vtsa_handle* h, double t, const double* z, const double* f, const
 int vtsa_scale(vtsa_handle* h, const double* z, double rtol, double* s, vtsa_stream st);
 /* out[0] = sqrt(mean((a/s)^2)) (s NULL: plain rms) */
 int vtsa_rms(vtsa_handle* h, const double* a, const double* s, int32_t len, double* out, vtsa_stream st);
+/* Low-rank correction of the GMRES right preconditioner (no upstream counterpart): GMRES then
+ * solves with P^-1 Q, Q = I + U D U^T in its scaled variables, U = V_m G (m x k, row-major, k <= 16)
+ * built on the device from the basis of the last GMRES cycle, D (k x k row-major).  k = 0 removes
+ * it; every vtsa_precond_setup removes it too.  vtsa_gmres_hessenberg copies the last cycle's
+ * Arnoldi Hessenberg as computed ((MAX 65) x 64 doubles, row-major, before Givens rotations) and
+ * its column count m (host memory, no device access).  vtsa_defl_apply: out = Q in. */
+int vtsa_gmres_hessenberg(vtsa_handle* h, double* H, int32_t* m);
+int vtsa_defl_set(vtsa_handle* h, int32_t k, int32_t m, const double* G, const double* D, vtsa_stream st);
+int vtsa_defl_apply(vtsa_handle* h, const double* in, double* out, vtsa_stream st);
 /* No counterpart in the upstream code; for callers that cannot allocate device memory during
  * the call (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * *out (a DEVICE double) = sqrt(mean((a/s)^2)) over len entries (s NULL: plain rms), the same
