@@ -13,8 +13,10 @@ schedule run continuously from the clean bed: ``--steps 20`` times exactly one f
 runs that are timed, each over the same K windows from the same bed:
 
 * value: the learned + FP32 form -- the learned stage guess (network trained on the device from a
-  plain FP64 run of another initial bed, the feed-equilibrium one) and FP32 preconditioner
-  factors; Newton, residual, Jacobian and GMRES stay FP64 and decide every step;
+  plain FP64 run of another initial bed, the feed-equilibrium one), the low-rank correction of
+  the preconditioner (rank ``--lowrank``, rebuilt after every factorisation from the harmonic Ritz
+  vectors of the first GMRES cycle) and FP32 preconditioner factors; Newton, residual, Jacobian
+  and GMRES stay FP64 and decide every step;
 * ``plain_fp64``: the plain method with FP64 factors, and ``vs_plain_fp64`` = value / its rate;
 * ``fp32``: FP32 factors without the guess.
 
@@ -121,7 +123,8 @@ def balance_errors(results, species):
 
 def solver_totals(results):
     keys = ("steps", "rejected", "newton_fail", "jacobians", "lu", "gmres_iters", "nfev", "cons_projected",
-            "balance_unclosed", "unclosed_rejected", "guess_used", "guess_kept_predictor")
+            "balance_unclosed", "unclosed_rejected", "guess_used", "guess_kept_predictor", "lowrank_built",
+            "lowrank_dropped")
     return {k: sum(int(sr.solver.get(k) or 0) for sr in results) for k in keys}
 
 
@@ -322,6 +325,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=2, help="line Gauss-Seidel sweeps of the preconditioner")
     ap.add_argument("--krylov", type=int, default=60, help="GMRES restart length")
+    ap.add_argument("--lowrank", type=int, default=4, help="rank of the value leg's preconditioner correction (0 off)")
     ap.add_argument("--cpu-budget-s", type=float, default=40.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
@@ -352,12 +356,12 @@ def main():
         _log(f"warm-up: {args.warmup} windows in {time.time() - t0:.2f} s")
     else:
         net, train = train_network(args, base_kw)
-    main_kw = {**base_kw, "precision": "fp32"}
+    main_kw = {**base_kw, "precision": "fp32", "lowrank": args.lowrank}
     if net is not None:
         main_kw["guess"] = net
     residual.reset_kernel_profile(8)
-    main, ends_main = timed_leg("learned+fp32" if net is not None else "fp32", cfg, grid, props, bed, args.steps,
-                                main_kw)
+    main, ends_main = timed_leg("learned+fp32" if net is not None or args.lowrank else "fp32", cfg, grid, props,
+                                bed, args.steps, main_kw)
     prof = dict(residual.KERNEL_PROFILE)
     residual.reset_kernel_profile(0)
     avg_s = prof["ms"] / 1e3 / max(prof["launches"], 1)
@@ -371,7 +375,9 @@ def main():
         "config": {"workload": f"full TVSA cycle (adsorption 8000 s, blowdown 60 s, heating 22400 s, cooling 400 s) "
                                f"on grid {GRID[0]}x{GRID[1]}x{GRID[2]} (387200 unknowns), clean bed",
                    "step": f"{WINDOW_S:.0f} simulated s of the cycle schedule (20 steps = one cycle)",
-                   "form": "learned stage guess + FP32 preconditioner factors" if net is not None else "FP32 factors",
+                   "form": ("learned stage guess + " if net is not None else "") +
+                           (f"rank-{args.lowrank} preconditioner correction + " if args.lowrank else "") +
+                           "FP32 preconditioner factors",
                    "solver": f"sdirk2 Newton-GMRES({args.krylov}), line Gauss-Seidel ({args.sweeps} sweeps), projection",
                    "rtol": cfg.solver.rtol, "conservation_tol": cfg.solver.conservation_tol},
         "balance": main["balance"], "solver": main["solver"], "steps_detail": main["steps"],
@@ -387,7 +393,8 @@ def main():
         plain, ends["plain"] = timed_leg("plain fp64", cfg, grid, props, bed, args.steps, {**base_kw, "precision": "fp64"})
         line["plain_fp64"] = plain
         line["vs_plain_fp64"] = main["value"] / plain["value"]
-    if not args.no_fp32 and net is not None:
+    learned = net is not None or args.lowrank > 0
+    if not args.no_fp32 and learned:
         fp32, ends["fp32"] = timed_leg("fp32", cfg, grid, props, bed, args.steps, {**base_kw, "precision": "fp32"})
         line["fp32"] = fp32
     if "plain" in ends and ends["plain"]:
@@ -397,7 +404,7 @@ def main():
             line["field_err_vs_plain_fp64"] = {"at": f"cycle {ref[0]} end of {ref[1]}"}
             for nm in ("main", "fp32"):
                 if nm in ends:
-                    line["field_err_vs_plain_fp64"]["learned_fp32" if nm == "main" and net is not None else
+                    line["field_err_vs_plain_fp64"]["learned_fp32" if nm == "main" and learned else
                                                      ("fp32" if nm == "fp32" else "value_run")] = \
                         field_errors(ends[nm][key - 1][2], ref[2], cfg)
     line["residual"] = residual_roofline(cfg, grid, props, bed)

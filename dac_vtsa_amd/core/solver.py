@@ -134,7 +134,9 @@ class DeviceOps:
         self.lowrank_window = (0.0, np.inf)
         self.lowrank_min_iters = 8
         self.lowrank_on = False
+        self.lowrank_held = False     # dropped until the next factorisation
         self.lowrank_built = 0
+        self.lowrank_dropped = 0
 
     def st(self):
         return dv.stream()
@@ -184,7 +186,7 @@ class DeviceOps:
         _abi.call("vtsa_gmres", self.h, float(t), dv.ptr(z), dv.ptr(f), dv.ptr(s), float(dh), float(zs_norm),
                   dv.ptr(b), dv.ptr(w), float(rtol), float(atol), int(restart), C.byref(info), self.st())
         self.m.nfev += info.iters + 1
-        if self.lowrank and not self.lowrank_on and info.iters >= self.lowrank_min_iters:
+        if self.lowrank and not (self.lowrank_on or self.lowrank_held) and info.iters >= self.lowrank_min_iters:
             self.lowrank_from_last_cycle(self.lowrank)
         return info
 
@@ -208,6 +210,12 @@ class DeviceOps:
     def lowrank_clear(self):
         _abi.call("vtsa_defl_set", self.h, 0, 0, None, None, self.st())
         self.lowrank_on = False
+
+    def lowrank_drop(self):
+        """Remove the correction and build none until the next factorisation."""
+        self.lowrank_clear()
+        self.lowrank_held = True
+        self.lowrank_dropped += 1
 
     def lowrank_from_last_cycle(self, k):
         """Build the rank-k correction from the last GMRES cycle's Hessenberg and basis; returns
@@ -267,7 +275,7 @@ class DeviceOps:
         """New preconditioner factors; removes the low-rank correction of the previous ones."""
         _abi.call("vtsa_precond_setup", self.h, float(dh), dv.ptr(scale), int(kind), int(precision), int(sweeps),
                   self.st())
-        self.lowrank_on = False
+        self.lowrank_on = self.lowrank_held = False
 
 
 def _finite(x):
@@ -404,7 +412,8 @@ def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, kryl
         ops.vec(9, ws, x=w, y=s)
         ops.prec_apply(ws, dz)
         lam = 1.0
-        for ls in range(4):
+        ls = 0
+        while ls < 4:
             rn_try, pn_try, step = ops.newton_try(t, z, dz, lam, dh, rhs_known, s, zt, ft, Rt)
             if math.isnan(step):
                 # dz holds a non-finite entry (core/solver.py:241-242); that test evaluates no residual
@@ -414,7 +423,19 @@ def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, kryl
                 pn_try = np.inf
             if _finite(pn_try) and (pn_try <= (1.0 - 1e-4 * lam) * pn or ls == 3):
                 break
+            if ls == 0 and ops.lowrank_on and not polish:
+                # (no upstream counterpart) the full step of a direction found with the low-rank
+                # correction fails the sufficient-decrease test: the correction is dropped until
+                # the next factorisation and the direction is found again without it
+                ops.lowrank_drop()
+                info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, eta, floor, krylov_dim)
+                if counter is not None:
+                    counter[0] += info.iters
+                ops.vec(9, ws, x=w, y=s)
+                ops.prec_apply(ws, dz)
+                continue
             lam *= 0.5
+            ls += 1
         if not _finite(pn_try):
             return z0, False, it + 1, np.inf, new_lu
         if trace is not None:
@@ -912,6 +933,7 @@ class TRBDF2:
         self.stats["gmres_iters"] = self._gm[0]
         if ops.lowrank:
             self.stats["lowrank_built"] = ops.lowrank_built
+            self.stats["lowrank_dropped"] = ops.lowrank_dropped
         res.nfev = m.nfev
         res.wall_time = time.time() - wall0
         return res
