@@ -49,10 +49,14 @@ FP64_PEAK_TFLOPS = 78.6
 GRID = (20, 32, 100)
 CYCLE_S = 30860.0
 WINDOW_S = CYCLE_S / 20.0
-# HBM bytes per launch of the dominant kernel on this workload from two rocprofv3 counter passes
-# (FETCH_SIZE and WRITE_SIZE, scaled by a 64 MiB device copy); None until measured for the
-# current kernel.
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per launch of the dominant kernel (k_gs_half<float, 2>, the value form's half-sweep) on
+# this workload from two rocprofv3 counter passes (FETCH_SIZE x 1.9993 + WRITE_SIZE x 1.0, the
+# factors of a 64 MiB device copy), over the first 1,543 s of the cycle.
+TRAFFIC_BYTES_PER_LAUNCH = 57753507
+# FP64 operations per launch on the kernel-probe workload (the heating step's model on the clean
+# bed at t = 0) from the rocprofv3 counters SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave
+# instructions x 64 lanes, an FMA counted as 2).
+FP64_FLOP_PER_LAUNCH = {"fd_jacobian": 20264750976, "residual": 280426880}
 METRIC = "cycle sim-s/wall-s, 1 GPU: learned+FP32 vs plain FP64 at MB/EB err <=1e-6"
 
 
@@ -195,9 +199,11 @@ def train_network(args, base_kw, log=_log):
     return net, info
 
 
-def residual_roofline(cfg, grid, props, state, n_launch=200):
-    """Average duration of one residual evaluation f(t, y) (k_props + k_flux), timed with HIP events
-    recorded on the stream it is launched on, and its algorithmic traffic (read y, write f)."""
+def kernel_probes(cfg, grid, props, state, n_rhs=200, n_jac=20):
+    """Average duration of one residual evaluation f(t, y) (k_props + k_flux) and of one coloured
+    FD Jacobian (k_fd_cells + k_fd_wall) on the heating step's model at the bed ``state``, each
+    timed with HIP events recorded on the stream it is launched on; the residual's algorithmic
+    traffic (read y, write f) and, where measured, both kernels' FP64 throughput."""
     import ctypes as C
     from dac_vtsa_amd import _abi
     from dac_vtsa_amd import device as dv
@@ -208,12 +214,24 @@ def residual_roofline(cfg, grid, props, state, n_launch=200):
     m.rhs_dev(0.0, y, f)
     dv.synchronize()
     ms = C.c_double()
-    _abi.call("vtsa_time_rhs", m.handle, 0.0, dv.ptr(y), dv.ptr(f), n_launch, C.byref(ms), dv.stream())
-    avg_s = ms.value / 1e3 / n_launch
+    _abi.call("vtsa_time_rhs", m.handle, 0.0, dv.ptr(y), dv.ptr(f), n_rhs, C.byref(ms), dv.stream())
+    rhs_s = ms.value / 1e3 / n_rhs
+    m.rhs_dev(0.0, y, f)
+    _abi.call("vtsa_time_fdjac", m.handle, 0.0, dv.ptr(y), dv.ptr(f), n_jac, C.byref(ms), dv.stream())
+    jac_s = ms.value / 1e3 / n_jac
     algo = 96 * grid.ncell + 16 * m.n_w
     m.close()
-    return {"kernels": "k_props + k_flux", "avg_launch_us": avg_s * 1e6, "algo_bytes_per_launch": algo,
-            "achieved_GBs": algo / avg_s / 1e9}
+    out = {"residual": {"kernels": "k_props + k_flux", "avg_launch_us": rhs_s * 1e6, "algo_bytes_per_launch": algo,
+                        "achieved_GBs": algo / rhs_s / 1e9},
+           "fd_jacobian": {"kernels": "k_fd_cells + k_fd_wall (all colours in one launch)",
+                           "avg_launch_us": jac_s * 1e6}}
+    for key, sec in (("residual", rhs_s), ("fd_jacobian", jac_s)):
+        flop = FP64_FLOP_PER_LAUNCH[key]
+        if flop:
+            tf = flop / sec / 1e12
+            out[key].update({"fp64_flop_per_launch": flop, "achieved_tflops": tf, "peak_tflops": FP64_PEAK_TFLOPS,
+                             "fp64_frac": tf / FP64_PEAK_TFLOPS})
+    return out
 
 
 def copy_probe(mib=64, repeat=5):
@@ -336,6 +354,10 @@ def main():
     ap.add_argument("--record-stride", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--copy-probe", action="store_true", help="only run the traffic-counter calibration copy")
+    ap.add_argument("--kernel-probe", action="store_true", help="only time the residual and FD Jacobian kernels")
+    ap.add_argument("--profile", choices=("plain", "fp32", "value"),
+                    help="only run --steps windows of one form (plain FP64; FP32 factors; FP32 factors with the "
+                         "low-rank correction, without the learned guess), for profiling")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -347,6 +369,16 @@ def main():
     from dac_vtsa_amd.core import residual
     base_kw = {"gs_sweeps": args.sweeps, "conserve": "project", "krylov_dim": args.krylov}
     cfg, grid, props, bed = setup()
+    if args.kernel_probe:
+        print(json.dumps(kernel_probes(cfg, grid, props, bed)), flush=True)
+        return 0
+    if args.profile:
+        form = {"plain": {"precision": "fp64"}, "fp32": {"precision": "fp32"},
+                "value": {"precision": "fp32", "lowrank": args.lowrank}}[args.profile]
+        res, _ = timed_leg(args.profile, cfg, grid, props, bed, args.steps, {**base_kw, **form})
+        print(json.dumps({"profile": args.profile, "windows": args.steps,
+                          **{k: res[k] for k in ("value", "sim_s", "wall_s", "solver")}}), flush=True)
+        return 0
     _log(f"workload {GRID[0]}x{GRID[1]}x{GRID[2]} cycle from the clean bed, {args.steps} windows of {WINDOW_S:.0f} s")
     net, train = (None, None)
     if args.no_learned:
@@ -407,7 +439,7 @@ def main():
                     line["field_err_vs_plain_fp64"]["learned_fp32" if nm == "main" and learned else
                                                      ("fp32" if nm == "fp32" else "value_run")] = \
                         field_errors(ends[nm][key - 1][2], ref[2], cfg)
-    line["residual"] = residual_roofline(cfg, grid, props, bed)
+    line.update(kernel_probes(cfg, grid, props, bed))
     if not args.no_cpu and "plain_fp64" in line:
         cb = cpu_baseline(args.cpu_budget_s, line["plain_fp64"]["solver"], line["plain_fp64"]["sim_s"])
         if cb is not None:

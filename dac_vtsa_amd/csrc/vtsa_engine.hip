@@ -487,24 +487,44 @@ int vtsa_stream_destroy(vtsa_stream st) {
   return check(hipStreamDestroy(S(st)), "hipStreamDestroy");
 }
 
-int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st) {
-  if (!H || !y || !f || !ms || repeat < 1) return -1;
-  Handle& h = H->h;
+extern "C++" {
+namespace {
+template <typename F>
+int time_launches(F&& launch, int32_t repeat, double* ms, hipStream_t st) {
   hipEvent_t e0, e1;
-  if (check(hipEventCreate(&e0), "hipEventCreate") || check(hipEventCreate(&e1), "hipEventCreate")) return -1;
-  (void)hipEventRecord(e0, S(st));
-  for (int r = 0; r < repeat; ++r) {
-    launch_props(h, y, h.pb, f, nullptr, S(st));
-    launch_flux(h, t, y, h.pb, f, S(st));
+  if (check(hipEventCreate(&e0), "hipEventCreate")) return -1;
+  if (check(hipEventCreate(&e1), "hipEventCreate")) {
+    (void)hipEventDestroy(e0);
+    return -1;
   }
-  (void)hipEventRecord(e1, S(st));
+  (void)hipEventRecord(e0, st);
+  for (int r = 0; r < repeat; ++r) launch();
+  (void)hipEventRecord(e1, st);
   int rc = check(hipEventSynchronize(e1), "hipEventSynchronize");
   float m = 0.f;
   if (!rc) rc = check(hipEventElapsedTime(&m, e0, e1), "hipEventElapsedTime");
   *ms = m;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return rc ? rc : finish(S(st));
+  return rc ? rc : finish(st);
+}
+}  // namespace
+}
+
+int vtsa_time_rhs(vtsa_handle* H, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st) {
+  if (!H || !y || !f || !ms || repeat < 1) { set_error("vtsa_time_rhs: null argument or repeat < 1"); return -1; }
+  Handle& h = H->h;
+  return time_launches([&] {
+    launch_props(h, y, h.pb, f, nullptr, S(st));
+    launch_flux(h, t, y, h.pb, f, S(st));
+  }, repeat, ms, S(st));
+}
+
+int vtsa_time_fdjac(vtsa_handle* H, double t, const double* y, const double* f0, int32_t repeat, double* ms,
+                    vtsa_stream st) {
+  if (!H || !y || !f0 || !ms || repeat < 1) { set_error("vtsa_time_fdjac: null argument or repeat < 1"); return -1; }
+  Handle& h = H->h;
+  return time_launches([&] { launch_fdjac(h, t, y, f0, S(st)); }, repeat, ms, S(st));
 }
 
 int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {

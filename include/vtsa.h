@@ -236,6 +236,10 @@ int vtsa_stream_destroy(vtsa_stream st);
 /* Elapsed milliseconds of `repeat` back-to-back residual evaluations f = f(t, y) on stream st,
  * measured with HIP events recorded on that stream (kernel timing for the roofline). */
 int vtsa_time_rhs(vtsa_handle* h, double t, const double* y, double* f, int32_t repeat, double* ms, vtsa_stream st);
+/* The same for `repeat` finite-difference Jacobian evaluations at (t, y) with f0 = f(t, y) (all
+ * colours' perturbed residuals in one launch, then the wall columns). */
+int vtsa_time_fdjac(vtsa_handle* h, double t, const double* y, const double* f0, int32_t repeat, double* ms,
+                    vtsa_stream st);
 
 /* Engine options (no upstream counterpart).  VTSA_OPT_GS_WHOLE_LINES, how a line Gauss-Seidel
  * half-sweep stages its line's factor in LDS: 1 (default) automatic -- the first of the forms
