@@ -98,10 +98,44 @@ def run_case(case: Dict, mode: str = "cycle", state_kind: str = "small", base_ov
     return out
 
 
-def run_batch(cases: List[Dict], workers: int = 16, log=None, **kw) -> Dict:
+def run_case_to_css(case: Dict, base_overrides: Optional[Dict] = None, integrator_kw: Optional[Dict] = None,
+                    state=None, max_cycles: Optional[int] = None) -> Dict:
+    """One case run cycle after cycle to cyclic steady state (vtsa_cycle.py:93-138, the CSS test of
+    validation/cyclic_steady_state.py:22-44) on the calling thread's current stream, from
+    ``state`` (default: the configuration's initial bed).  Returns per-cycle balances, the last
+    cycle's KPIs, whether CSS was reached, and the end state."""
+    from .balances import is_css
+    from .config import load_config
+    from .core.grid import Grid
+    from .core.state import initial_state
+    from .cycle import run_to_css
+    from .properties import get_backend
+    cfg = load_config(override_dict={**(base_overrides or {}), **case["overrides"]})
+    grid = Grid.from_config(cfg)
+    out = {"id": case["id"], "cells": grid.ncell, "grid": [grid.Nr, grid.Nth, grid.Nz], "ok": True}
+    t0 = time.time()
+    try:
+        st0 = state if state is not None else initial_state(cfg, grid, get_backend(cfg))
+        records, st = run_to_css(cfg, st0, max_cycles=max_cycles, integrator_kw=dict(integrator_kw or {}))
+        dv.synchronize()
+    except Exception as e:                     # a failed case is reported, the batch goes on
+        out.update(ok=False, error=f"{type(e).__name__}: {e}", wall_s=time.time() - t0, sim_s=0.0)
+        return out
+    last = records[-1]
+    out.update(wall_s=time.time() - t0, sim_s=sum(r.kpis["cycle_time_s"] for r in records), cycles=len(records),
+               css=bool(is_css(last.change, last.kpis, cfg, len(records))),
+               kpis=dict(last.kpis), state=st,
+               max_mass_rel_err=max(abs(r["relative_error"]) for rec in records for r in rec.mb_rows),
+               max_energy_rel_err=max(abs(r["relative_error"]) for rec in records for r in rec.eb_rows))
+    return out
+
+
+def run_batch(cases: List[Dict], workers: int = 16, log=None, runner=None, **kw) -> Dict:
     """Run the cases on ``workers`` threads, one HIP stream each; larger cases are started first.
-    ``log`` (optional) receives one line per finished case.  Returns {"cases": [per-case dicts in
-    input order], "wall_s", "sim_s", "value" (aggregate simulated seconds per wall second)}."""
+    ``runner`` is run_case (default) or run_case_to_css, called with the case and ``kw``; ``log``
+    (optional) receives one line per finished case.  Returns {"cases": [per-case dicts in input
+    order], "wall_s", "sim_s", "value" (aggregate simulated seconds per wall second)}."""
+    runner = runner or run_case
     order = sorted(range(len(cases)), key=lambda i: -np.prod(list(cases[i]["overrides"].get(g, 1) for g in
                                                                  ("grid.Nr", "grid.Ntheta", "grid.Nz"))))
     local = threading.local()
@@ -114,7 +148,7 @@ def run_batch(cases: List[Dict], workers: int = 16, log=None, **kw) -> Dict:
             with lock:
                 streams.append(s)
         with dv.use_stream(s):
-            r = run_case(cases[i], **kw)
+            r = runner(cases[i], **kw)
         if log:
             log(f"case {r['id']} {r['grid']}: {'ok' if r['ok'] else r['error']}, {r['sim_s']:.0f} s simulated "
                 f"in {r['wall_s']:.2f} s")

@@ -671,41 +671,57 @@ void vec_op_impl(Handle& h, int op, int npos, double a, double b, double c, doub
 // Dd = T^-1 - I, T = Ud^T B Ud, the operator B Q maps that subspace onto itself with eigenvalues
 // near 1, so GMRES no longer spends iterations on those modes.
 //
-// k_defl_dots: v = in * scale (stored to `store` when given), c_j = Ud_j . v; the last block forms
-// d = Dd c.  With respect_done, a queued call after the cycle converged returns at once.
-__global__ void __launch_bounds__(RBLK) k_defl_dots(int N, int k, const double* in,
-                                                    const double* __restrict__ in_scale, double* store,
-                                                    const double* __restrict__ Ud, const double* __restrict__ Dd,
-                                                    double* __restrict__ d, const GmresScalars* __restrict__ gs,
-                                                    int respect_done, double* __restrict__ partial,
-                                                    unsigned* counter) {
-  __shared__ double sh[RBLK / 64];
-  __shared__ double cs[DEFL_MAX];
+// k_defl_dots: v = in * scale (stored to `store` when given), c_j = Ud_j . v for j < K; the last
+// block forms d = Dd c.  The K block sums share one exchange through LDS.  With respect_done, a
+// queued call after the cycle converged returns at once.
+template <int K>
+__global__ void __launch_bounds__(RBLK) k_defl_dots(int N, const double* in, const double* __restrict__ in_scale,
+                                                    double* store, const double* __restrict__ Ud,
+                                                    const double* __restrict__ Dd, double* __restrict__ d,
+                                                    const GmresScalars* __restrict__ gs, int respect_done,
+                                                    double* __restrict__ partial, unsigned* counter) {
+  __shared__ double sh[(RBLK / 64) * K];
+  __shared__ double cs[K];
   __shared__ bool slast;
   if (respect_done && gs->done) return;
   const double sc = in_scale ? *in_scale : 1.0;
-  double loc[DEFL_MAX];
+  double loc[K];
 #pragma unroll
-  for (int j = 0; j < DEFL_MAX; ++j) loc[j] = 0.0;
+  for (int j = 0; j < K; ++j) loc[j] = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const double v = in[i] * sc;
     if (store) store[i] = v;
+    double u[K];
 #pragma unroll
-    for (int j = 0; j < DEFL_MAX; ++j)
-      if (j < k) loc[j] = fma(Ud[(size_t)j * N + i], v, loc[j]);
-  }
+    for (int j = 0; j < K; ++j) u[j] = Ud[(size_t)j * N + i];
 #pragma unroll
-  for (int j = 0; j < DEFL_MAX; ++j) {
-    if (j < k) {   // k is uniform over the grid: every thread takes the same branches
-      const double bs = wave_block_sum(loc[j], sh);
-      if (threadIdx.x == 0) store_partial(partial + (size_t)j * RMAXB + blockIdx.x, bs);
-    }
+    for (int j = 0; j < K; ++j) loc[j] = fma(u[j], v, loc[j]);
   }
-  if (!last_block_totals(counter, &slast, partial, k, cs)) return;
-  if ((int)threadIdx.x < k) {
+  // K butterflies side by side, then one LDS exchange of the per-wave sums
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) o[j] = __shfl_xor(loc[j], off, 64);
+#pragma unroll
+    for (int j = 0; j < K; ++j) loc[j] += o[j];
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int j = 0; j < K; ++j) sh[wv * K + j] = loc[j];
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    double bs = 0.0;
+    for (int w = 0; w < RBLK / 64; ++w) bs += sh[w * K + threadIdx.x];
+    store_partial(partial + (size_t)threadIdx.x * RMAXB + blockIdx.x, bs);
+  }
+  if (!last_block_totals(counter, &slast, partial, K, cs)) return;
+  if ((int)threadIdx.x < K) {
     const int r = threadIdx.x;
     double acc = 0.0;
-    for (int j = 0; j < k; ++j) acc = fma(Dd[r * DEFL_MAX + j], cs[j], acc);
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc = fma(Dd[r * DEFL_MAX + j], cs[j], acc);
     d[r] = acc;
   }
 }
@@ -773,8 +789,19 @@ int defl_set_impl(Handle& h, int k, int m, const double* G, const double* D, hip
 void defl_apply_launch(Handle& h, const double* in, const double* in_scale, double* store, double* out,
                        int respect_done, hipStream_t st) {
   double* d = h.defl_small + DEFL_MAX * DEFL_MAX;
-  hipLaunchKernelGGL(k_defl_dots, dim3(rgrid(h.N)), dim3(RBLK), 0, st, h.N, h.defl_k, in, in_scale, store, h.Ud,
-                     h.defl_small, d, h.gs, respect_done, h.defl_partial, h.defl_counter);
+  const dim3 grid(rgrid(h.N)), block(RBLK);
+  switch (h.defl_k) {
+#define VTSA_DEFL_DOTS(K)                                                                                   \
+  case K:                                                                                                   \
+    hipLaunchKernelGGL(k_defl_dots<K>, grid, block, 0, st, h.N, in, in_scale, store, h.Ud, h.defl_small, d, \
+                       h.gs, respect_done, h.defl_partial, h.defl_counter);                                 \
+    break;
+    VTSA_DEFL_DOTS(1) VTSA_DEFL_DOTS(2) VTSA_DEFL_DOTS(3) VTSA_DEFL_DOTS(4) VTSA_DEFL_DOTS(5) VTSA_DEFL_DOTS(6)
+    VTSA_DEFL_DOTS(7) VTSA_DEFL_DOTS(8) VTSA_DEFL_DOTS(9) VTSA_DEFL_DOTS(10) VTSA_DEFL_DOTS(11)
+    VTSA_DEFL_DOTS(12) VTSA_DEFL_DOTS(13) VTSA_DEFL_DOTS(14) VTSA_DEFL_DOTS(15) VTSA_DEFL_DOTS(16)
+#undef VTSA_DEFL_DOTS
+    default: return;   // defl_set_impl admits 1 .. DEFL_MAX only
+  }
   const double* v = store ? store : in;
   const double* vs = store ? nullptr : in_scale;
   hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(h.N)), dim3(256), 0, st, h.N, h.defl_k, v, vs, h.Ud, d, out, h.gs,
