@@ -531,7 +531,7 @@ int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {
   if (!H) { set_error("vtsa_set_option: null handle"); return -1; }
   switch (option) {
     case VTSA_OPT_GS_WHOLE_LINES:
-      if (value < 0 || value > 4) { set_error("vtsa_set_option: staging level must be 0 .. 4"); return -2; }
+      if (value < 0 || value > 5) { set_error("vtsa_set_option: staging level must be 0 .. 5"); return -2; }
       H->h.gs_full = value;
       return 0;
     default: set_error("vtsa_set_option: unknown option"); return -2;

@@ -684,6 +684,53 @@ __device__ inline void pair_recurrence(const T* Pl, int NP, int off, const doubl
   }
 }
 
+// FP32 recurrence matrices widened to FP64 in LDS, so that the pair recurrences run on FP64
+// operands (no conversions on the dependent chain) while HBM moves FP32.  A matrix set (G or H,
+// 144 values per pair) is copied as FP32 straight into the top of its FP64 buffer (no registers
+// held during the copy), then widened in place: every thread reads its float4 chunks, a barrier,
+// then every thread writes them as doubles; packed chunk c (pair c / 36) lands at buf + 4 c.
+// One pass of WCH chunks per thread, so lines of at most WCH * GSW / 36 pairs.
+constexpr int WCH = 4;
+template <typename T>
+__host__ __device__ constexpr bool widen_fits(int Nz) { return npairs(Nz) * 36 <= WCH * GSW; }
+__host__ __device__ constexpr size_t narrow_off(int Nz) {
+  return pair_g_buf_bytes<double>(Nz) - pair_g_buf_bytes<float>(Nz);
+}
+// the gather's partial sums, 4 x 6 x Nz doubles: below the FP32 copy of G when they fit there,
+// else after the FP64 buffer
+__host__ __device__ constexpr size_t widen_part_off(int Nz) {
+  return (size_t)192 * Nz <= narrow_off(Nz) ? 0 : pair_g_buf_bytes<double>(Nz);
+}
+__host__ __device__ constexpr size_t gs_w_lds_bytes(int Nz) {
+  return sizeof(double) * 2 * pair_vec_doubles(Nz) + pair_g_buf_bytes<double>(Nz) +
+         ((size_t)192 * Nz <= narrow_off(Nz) ? 0 : (size_t)192 * Nz);
+}
+template <int OFF>
+__device__ inline void stage_narrow(const float* __restrict__ P, int L, int NP, int Nz, double* buf) {
+  stage_pairs_part<float, OFF, 144>(P, L, NP, reinterpret_cast<float*>(reinterpret_cast<char*>(buf) + narrow_off(Nz)));
+}
+// after the copies completed (stage_wait); the caller's barrier orders the writes before use
+__device__ inline void widen_in_place(int NP, int Nz, double* buf) {
+  const float4* src = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(buf) + narrow_off(Nz));
+  const int total = NP * 36;
+  float4 v[WCH];
+#pragma unroll
+  for (int u = 0; u < WCH; ++u) {
+    const int c = u * GSW + (int)threadIdx.x;
+    if (c < total) v[u] = src[c];
+  }
+  lds_barrier();
+#pragma unroll
+  for (int u = 0; u < WCH; ++u) {
+    const int c = u * GSW + (int)threadIdx.x;
+    if (c < total) {
+      double2* d = reinterpret_cast<double2*>(buf + 4 * (size_t)c);
+      d[0] = make_double2((double)v[u].x, (double)v[u].y);
+      d[1] = make_double2((double)v[u].z, (double)v[u].w);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ z-line / block-Jacobi apply
 // One z-column per 64-lane block; wall rows (point Jacobi) in blocks of their own.
 template <typename T>
@@ -972,14 +1019,16 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     const int i = L / Nth, j = L - (L / Nth) * Nth;
     const int X0 = L * Nz;
     const int NP = npairs(Nz);
-    double* zs = lds + pair_vec_doubles(Nz);                                  // MODE 1, 2
-    T* pb = reinterpret_cast<T*>(lds + 2 * pair_vec_doubles(Nz));             // MODE 1, 2
+    double* zs = lds + pair_vec_doubles(Nz);                                  // MODE 1 .. 4
+    T* pb = reinterpret_cast<T*>(lds + 2 * pair_vec_doubles(Nz));             // MODE 1 .. 3
+    double* pbw = lds + 2 * pair_vec_doubles(Nz);                              // MODE 4
     T* fb0 = reinterpret_cast<T*>(lds + (size_t)Nz * 6 + 8);                  // MODE 0
     T* fb1 = reinterpret_cast<T*>(reinterpret_cast<char*>(fb0) + zline_buf_bytes<T>());
     if (MODE == 1) stage_pairs<T>(Fac, L, NP, pb);
     else if (MODE == 2) stage_pairs_part<T, PO_G, PGH>(Fac, L, NP, pb);
     else if (MODE == 3) stage_pairs_part<T, PO_G, 144>(Fac, L, NP, pb);
-    else zline_stage_first<T>(Fac, X0, Nz, fb0);
+    else if (MODE == 0) zline_stage_first<T>(Fac, X0, Nz, fb0);
+    if constexpr (MODE == 4) stage_narrow<PO_G>(Fac, L, NP, Nz, pbw);
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
     const int nb[4] = {i > 0 ? ((i - 1) * Nth + j) * Nz : -1, i + 1 < Nr ? ((i + 1) * Nth + j) * Nz : -1,
@@ -1018,14 +1067,62 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       const int idx = r * n + X;
       return gs_b(a, g.s0, g.bs, idx, first_visit) + (dh / g.s0[idx]) * acc;
     };
-    for (int e = lane; e < 6 * Nz; e += NT) xs[(e % Nz) * 6 + e / Nz] = rhs(e);
+    if constexpr (MODE == 4) {
+      // the off-line products by (slot q, cell k), each neighbour's 6 unknowns loaded once for
+      // its 6 rows: 4 Nz tasks of 42 loads instead of 6 Nz of 48, one round of loads per thread.
+      // The partial sums, [4][6][Nz], sit where the widened G goes later; an element adds its
+      // four in slot order.
+      double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(pbw) + widen_part_off(Nz));
+      if (!g.first) {
+        for (int t = lane; t < 4 * Nz; t += NT) {
+          const int q = t / Nz, k = t - q * Nz;
+          const int X = X0 + k;
+          const bool on = nb[q] >= 0;
+          const int Y = (on ? nb[q] : X0) + k;
+          const T* Jq = reinterpret_cast<const T*>(g.Jo) + (size_t)(q * 36) * n + X;   // coalesced over k
+          double uv[6], jv[36];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) uv[c] = g.U[c * n + Y];
+#pragma unroll
+          for (int rc = 0; rc < 36; ++rc) jv[rc] = (double)Jq[(size_t)rc * n];
+#pragma unroll
+          for (int r = 0; r < 6; ++r) {
+            double p = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) p += (on ? jv[r * 6 + c] : 0.0) * uv[c];
+            part[(q * 6 + r) * Nz + k] = p;
+          }
+        }
+        lds_barrier();
+      }
+      for (int e = lane; e < 6 * Nz; e += NT) {
+        const int r = e / Nz, k = e - r * Nz;
+        const int X = X0 + k;
+        double acc = 0.0;
+        if (!g.first) {
+          double jw = 0.0, uw = 0.0;
+          if (nw > 0) {
+            const int W = j * Nz + k;
+            jw = g.Jcw[W * 4 + (r <= 2 ? r : 3)];
+            uw = g.U[6 * n + W];
+          }
+          acc = ((part[r * Nz + k] + part[(6 + r) * Nz + k]) + part[(12 + r) * Nz + k]) + part[(18 + r) * Nz + k];
+          acc += (wall && (r <= 2 || r == 5) ? jw : 0.0) * uw;
+        }
+        const int idx = r * n + X;
+        xs[k * 6 + r] = gs_b(a, g.s0, g.bs, idx, first_visit) + (dh / g.s0[idx]) * acc;
+      }
+    } else {
+      for (int e = lane; e < 6 * Nz; e += NT) xs[(e % Nz) * 6 + e / Nz] = rhs(e);
+    }
     if (MODE >= 1) {
       if (lane < 6 * (2 * NP - Nz)) xs[Nz * 6 + lane] = 0.0;   // phantom cell of an odd line
       // the data-parallel passes read L1, Dinv and ^U1 from the staged line (MODE 1) or from
-      // global memory (MODE 2: G and H staged; MODE 3: G, then H in its place); the recurrences
-      // read G and H from LDS
+      // global memory (MODE 2: G and H staged; MODE 3: G, then H in its place; MODE 4: the same
+      // widened to FP64); the recurrences read G and H from LDS
       const T* pd = MODE == 1 ? pb : Fac + (size_t)L * NP * PP;
       stage_wait();
+      if constexpr (MODE == 4) widen_in_place(NP, Nz, pbw);
       // b'_{2K+1} = b_{2K+1} - L1_{2K+1} b_{2K}
       for (int e = lane; e < NP * 6; e += NT) {
         const int K = e / 6, r = e - K * 6;
@@ -1037,10 +1134,14 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       }
       lds_barrier();
       constexpr int SG = MODE == 1 ? PP : MODE == 2 ? PGH : 144;   // LDS stride of a pair's matrices
-      if (lane < 12) pair_recurrence<T, true, SG>(pb, NP, PO_G, xs, xs);
+      if (lane < 12) {
+        if constexpr (MODE == 4) pair_recurrence<double, true, 144>(pbw, NP, 0, xs, xs);
+        else pair_recurrence<T, true, SG>(pb, NP, PO_G, xs, xs);
+      }
       lds_barrier();
-      // MODE 3: H replaces G in the buffer; the copies run under the two passes below
+      // MODE 3, 4: H replaces G in the buffer; the copies run under the two passes below
       if (MODE == 3) stage_pairs_part<T, PO_H, 144>(Fac, L, NP, pb);
+      if constexpr (MODE == 4) stage_narrow<PO_H>(Fac, L, NP, Nz, pbw);
       // h_k = Dinv_k z_k
       for (int e = lane; e < NP * 12; e += NT) {
         const int K = e / 12, rr = e - K * 12, hh = rr / 6;
@@ -1060,9 +1161,13 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
         for (int c = 0; c < 6; ++c) v = fma(-(double)U[c], zs[K * 12 + 6 + c], v);
         zs[K * 12 + r] = v;
       }
-      if (MODE == 3) stage_wait();
+      if (MODE == 3 || MODE == 4) stage_wait();
+      if constexpr (MODE == 4) widen_in_place(NP, Nz, pbw);
       lds_barrier();
-      if (lane < 12) pair_recurrence<T, false, SG>(pb, NP, MODE == 3 ? 0 : PO_H, zs, xs);
+      if (lane < 12) {
+        if constexpr (MODE == 4) pair_recurrence<double, false, 144>(pbw, NP, 0, zs, xs);
+        else pair_recurrence<T, false, SG>(pb, NP, MODE == 3 ? 0 : PO_H, zs, xs);
+      }
       lds_barrier();
     } else {
       zline_sweeps<T>(Fac, X0, Nz, xs, fb0, fb1);
@@ -1294,13 +1399,15 @@ int prof_resolve(Handle& h) {
 template <typename T, int MODE>
 static size_t gs_lds_bytes(int Nz) {
   return MODE == 1 ? gs_full_lds_bytes<T>(Nz) : MODE == 2 ? gs_gh_lds_bytes<T>(Nz)
-                   : MODE == 3 ? gs_g_lds_bytes<T>(Nz) : zline_lds_bytes<T>(Nz);
+                   : MODE == 3 ? gs_g_lds_bytes<T>(Nz) : MODE == 4 ? gs_w_lds_bytes(Nz)
+                   : zline_lds_bytes<T>(Nz);
 }
 
 // Whether half-sweep variant MODE can run: its LDS fits the device's limit for one block and the
 // kernel accepts that much dynamic LDS (asked for once per variant and size).
 template <typename T, int MODE>
 static bool gs_mode_ok(const Handle& h) {
+  if (MODE == 4 && !widen_fits<T>(h.desc.Nz)) return false;
   static std::mutex mu;
   static int granted = -1;   // -1 not asked, 0 refused, 1 granted (for the largest size asked)
   static size_t granted_bytes = 0;
@@ -1324,22 +1431,28 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   // factor (MODE 1) while the blocks a CU can hold at that LDS size cover the widest colour in
   // one round, else G and H only (MODE 2), else the segmented cell-by-cell form (MODE 0).
   const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds2 = gs_lds_bytes<T, 2>(Nz), lds3 = gs_lds_bytes<T, 3>(Nz);
-  const size_t lds0 = gs_lds_bytes<T, 0>(Nz);
+  const size_t lds0 = gs_lds_bytes<T, 0>(Nz), lds4 = gs_lds_bytes<T, 4>(Nz);
+  constexpr bool narrow = sizeof(T) == sizeof(float);
   int widest = 0;
   for (size_t c = 0; c + 1 < h.gs_cstart.size(); ++c) widest = std::max(widest, h.gs_cstart[c + 1] - h.gs_cstart[c]);
   auto per_cu = [&](size_t lds) { return std::max<long long>(1, (long long)h.lds_per_cu / (long long)lds); };
   // option VTSA_OPT_GS_WHOLE_LINES: 0 segments, 1 automatic, 2 G and H, 3 the whole factor,
-  // 4 G then H; automatic takes the first of the whole factor, G and H, G then H whose blocks
-  // all fit on the CUs at once, else the first that fits at all
+  // 4 G then H, 5 G then H widened to FP64 (FP32 factors; FP64 ones: as 4); automatic takes the
+  // first of G then H widened (FP32 factors: FP64 operands on the recurrences), the whole
+  // factor, G and H, G then H whose blocks all fit on the CUs at once, else the first that fits
   const int o = h.gs_full;
+  bool ok4 = false;
+  if constexpr (narrow) ok4 = (o == 1 || o == 5) && gs_mode_ok<T, 4>(h);
   const bool ok1 = (o == 1 || o == 3) && gs_mode_ok<T, 1>(h);
   const bool ok2 = (o == 1 || o == 2) && gs_mode_ok<T, 2>(h);
-  const bool ok3 = (o == 1 || o == 4) && gs_mode_ok<T, 3>(h);
+  const bool ok3 = (o == 1 || o == 4 || (o == 5 && !narrow)) && gs_mode_ok<T, 3>(h);
   auto one_round = [&](size_t lds) { return (long long)widest <= (long long)h.n_cu * per_cu(lds); };
   int mode = 0;
-  if (ok1 && one_round(lds1)) mode = 1;
+  if (ok4 && one_round(lds4)) mode = 4;
+  else if (ok1 && one_round(lds1)) mode = 1;
   else if (ok2 && one_round(lds2)) mode = 2;
   else if (ok3 && one_round(lds3)) mode = 3;
+  else if (ok4) mode = 4;
   else if (ok1) mode = 1;
   else if (ok2) mode = 2;
   else if (ok3) mode = 3;
@@ -1379,7 +1492,12 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       e1 = prof_event(h);
       if (e0 && e1) (void)hipEventRecord(e0, s);
     }
-    if (mode == 1)
+    if (mode == 4) {
+      if constexpr (narrow)
+        hipLaunchKernelGGL((k_gs_half<T, 4>), dim3(cnt), dim3(GSW), lds4, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
+                           h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
+                           partial, counter);
+    } else if (mode == 1)
       hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
                          partial, counter);

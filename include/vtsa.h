@@ -245,9 +245,12 @@ int vtsa_time_fdjac(vtsa_handle* h, double t, const double* y, const double* f0,
  * half-sweep stages its line's factor in LDS: 1 (default) automatic -- the first of the forms
  * below whose blocks the CUs hold all at once, else the first the device grants at all; 3 the
  * whole pair factor, 2 the two recurrence matrices of each pair (the rest read from global
- * memory), 4 the forward matrix, then the backward one in its place (each when granted); 0 cell
- * factors in segments during the sweep.  Levels 1-4 give bit-identical results; level 0 solves
- * the same line systems cell by cell (other roundings). */
+ * memory), 4 the forward matrix, then the backward one in its place, 5 with FP32 factors the
+ * same with both matrices widened to FP64 in LDS and the off-line products summed per neighbour
+ * first, with FP64 factors as 4 (each when granted); 0 cell factors in segments during the sweep.
+ * Automatic tries 5 first for FP32 factors.  Levels 1-4 give bit-identical results, 5 with FP32
+ * factors the same recurrences on a right-hand side summed in another order; level 0 solves the
+ * same line systems cell by cell (other roundings). */
 enum { VTSA_OPT_GS_WHOLE_LINES = 1 };
 int vtsa_set_option(vtsa_handle* h, int32_t option, int32_t value);
 
