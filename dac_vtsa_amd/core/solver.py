@@ -39,6 +39,8 @@ SW3 = (2.0 - 3.0 * G_SD) / (6.0 * (1.0 - G_SD))
 PRECONDITIONERS = {"block_jacobi": 0, "zline": 1, "zline_gs": 2}
 PRECISIONS = {"fp64": 0, "fp32": 1}
 MAX_RESTART = 64      # columns of the device Hessenberg (vtsa_gmres_hessenberg)
+GUESS_MISS_LIMIT = 32  # learned guess: stages in a row lost to the predictor before it is tried less often
+GUESS_RETRY_EVERY = 8  # ... and then tried on every 8th stage
 LOWRANK_MAX = 16      # largest rank of the device low-rank correction
 
 
@@ -665,8 +667,9 @@ class TRBDF2:
             self.guess = StageGuess(guess, model.n_c // model.ng, self.ops.N)
         if (guess is not None or record is not None) and (model.ng != 3 or model.na != 2):
             raise ValueError("the learned stage guess needs the three-gas, two-adsorbate state layout")
+        self._guess_miss = self._guess_wait = 0
         if guess is not None or record is not None:
-            self.stats.update({"guess_used": 0, "guess_kept_predictor": 0})
+            self.stats.update({"guess_used": 0, "guess_kept_predictor": 0, "guess_skipped": 0})
             self._s_ref, self._gf, self._gR, self._gP, self._zg = (dv.empty(self.ops.N) for _ in range(5))
         # parity mode (no upstream counterpart): replay the accepted step end times of another run
         self.h_sequence = None if h_sequence is None else np.asarray(h_sequence, dtype=float)
@@ -708,9 +711,17 @@ class TRBDF2:
 
     def _start(self, t, z0, rhs_known, dh, y_ref, stage):
         """Starting value of a stage: the predictor z0, or the learned guess when both its
-        preconditioned and its plain stage residual are smaller; records the stage's features."""
+        preconditioned and its plain stage residual are smaller; records the stage's features.
+        After GUESS_MISS_LIMIT stages in a row whose guess lost to the predictor, the guess is
+        tried on every GUESS_RETRY_EVERY-th stage only, until it wins again (no upstream
+        counterpart: the trial costs two residuals and a preconditioner application)."""
         if self.guess is None and self.record is None:
             return z0
+        if self.record is None and self._guess_miss >= GUESS_MISS_LIMIT:
+            self._guess_wait += 1
+            if self._guess_wait % GUESS_RETRY_EVERY:
+                self.stats["guess_skipped"] += 1
+                return z0
         h = dh / D
         ops = self.ops
         s = ops.scale(y_ref, self.rtol, self._s_ref)
@@ -730,8 +741,10 @@ class TRBDF2:
         # preconditioned norm favours can leave conservation errors Newton does not remove
         if np.isfinite(pn1) and np.isfinite(rn1) and pn1 < pn0 and rn1 <= rn0:
             self.stats["guess_used"] += 1
+            self._guess_miss = 0
             return zg
         self.stats["guess_kept_predictor"] += 1
+        self._guess_miss += 1
         return z0
 
     def _solve_stage(self, t, z0, rhs_known, dh, lu, res, y_ref, stage=0):

@@ -173,13 +173,21 @@ def test_learned_guess_on_a_heating_step():
     print("held-out rms error (tolerance units, c_CO2 c_H2O c_N2 q_CO2 q_H2O e):\n"
           f"  predictor            {np.round(ep.mean(0), 3)}\n  preconditioned step  {np.round(eb.mean(0), 3)}\n"
           f"  learned guess        {np.round(eg.mean(0), 3)}   gain {np.round(hist['gain'], 3)}")
-    # the calibrated guess is never worse than its base on the held-out stages (sum of squares)
+    # the calibrated guess is never worse than its base on the held-out stages (sum of squares),
+    # and it is closer to the converged stages than the predictor every solve starts from
     assert np.sum(eg ** 2) <= np.sum(eb ** 2) * (1.0 + 1e-6)
+    assert np.sum(eg ** 2) < np.sum(ep ** 2)
     st0 = seeded_state(grid, "midcycle", 0)
-    st_plain, _ = run_step(st0, cfg, grid, props, step, integrator_kw={**base, "precision": "fp64"})
-    st, sr = run_step(st0, cfg, grid, props, step, integrator_kw={**base, "precision": "fp32", "guess": net})
+    st_plain, sr_plain = run_step(st0, cfg, grid, props, step, integrator_kw={**base, "precision": "fp64"})
+    st, sr = run_step(st0, cfg, grid, props, step, integrator_kw={**base, "precision": "fp32", "guess": net,
+                                                                  "lowrank": 4})
     s = sr.solver
     assert s["guess_used"] + s["guess_kept_predictor"] > 0
+    # iterations of the learned form (guess, rank-4 preconditioner correction, FP32 factors) and of
+    # the plain FP64 method on this step, for the record: they take different step sequences
+    print(f"GMRES iterations: learned form {s['gmres_iters']}, plain FP64 {sr_plain.solver['gmres_iters']}; "
+          f"guess used {s['guess_used']}, kept the predictor {s['guess_kept_predictor']}, "
+          f"skipped {s['guess_skipped']}; corrections built {s['lowrank_built']}")
     mb, eb_, _ = balances.check_cycle([sr], props.species)
     assert max(abs(r["relative_error"]) for r in mb if r["step"] != "CYCLE") <= 1e-6
     assert max(abs(r["relative_error"]) for r in eb_ if r["step"] != "CYCLE") <= 1e-6
@@ -190,7 +198,8 @@ def test_learned_guess_on_a_heating_step():
                        (st.c[2], st_plain.c[2], s_.atol_c_inert), (st.q, st_plain.q, s_.atol_q),
                        (st.T, st_plain.T, s_.atol_T), (st.Tw, st_plain.Tw, s_.atol_T)):
         err = np.max(np.abs(a - b) / (atol + s_.rtol * np.abs(b)))
-        assert err < 100.0, err
+        print(f"end-field difference {err:.3g} tolerance units")
+        assert err < 3.0, err
     net.close()
 
 
