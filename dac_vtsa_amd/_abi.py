@@ -107,6 +107,7 @@ _SIGS = {
     "vtsa_stream_destroy": [P],
     "vtsa_time_rhs": [P, D, P, P, I32, C.POINTER(D), P],
     "vtsa_time_fdjac": [P, D, P, P, I32, C.POINTER(D), P],
+    "vtsa_outlet_row": [P, D, P, I32, P, P],
     "vtsa_set_option": [P, I32, I32],
     "vtsa_profile": [P, I32],
     "vtsa_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(D), C.POINTER(D)],

@@ -35,7 +35,7 @@ static void free_handle(Handle* h) {
   if (!h) return;
   double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2, h->FF,
                     h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
-                    h->Vb, h->red.partial, h->red.result, h->Ud, h->defl_small, h->defl_partial, h->defl_tmp};
+                    h->Vb, h->red.partial, h->red.result, h->Ud, h->defl_small, h->defl_partial, h->defl_tmp, h->dg};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (h->F) (void)hipFree(h->F);
@@ -403,6 +403,21 @@ int vtsa_cons_fetch(vtsa_handle* H, int32_t n, double* wr, double* ws, vtsa_stre
   for (int q = 0; q < n; ++q)
     for (int k = 0; k < 4; ++k) { wr[4 * q + k] = out[8 * q + k]; ws[4 * q + k] = out[8 * q + 4 + k]; }
   return 0;
+}
+
+int vtsa_outlet_row(vtsa_handle* H, double t, const double* y, int32_t valve_mask, double* row, vtsa_stream st) {
+  if (!H || !y || !row) { set_error("vtsa_outlet_row: null argument"); return -1; }
+  Handle& h = H->h;
+  const int Nr = h.desc.Nr, Nth = h.desc.Nth, Nz = h.desc.Nz;
+  const size_t n = (size_t)h.n, nf = (size_t)Nr * Nth, nw2 = (size_t)Nth * Nz;
+  const size_t o_uz = 8 * n, o_ur = o_uz + std::max<size_t>(1, nf * (Nz - 1)),
+               o_uth = o_ur + std::max<size_t>(1, (size_t)(Nr - 1) * Nth * Nz), o_bnd = o_uth + n,
+               o_wall = o_bnd + 2 * 14 * nf, total = o_wall + 4 * nw2;
+  if (!h.dg && dalloc(&h.dg, total)) return -1;
+  launch_diag(h, t, y, h.pb, h.dg, h.dg + o_uz, h.dg + o_ur, h.dg + o_uth, h.dg + o_bnd, h.dg + o_wall, S(st));
+  launch_outlet_sums(h, h.dg, h.dg + o_bnd, valve_mask, row, S(st));
+  if (reduce_weighted_dev(h, y, 1, row + 24, S(st))) return -1;
+  return finish(S(st));
 }
 
 int vtsa_inventories(vtsa_handle* H, const double* y, double* out, vtsa_stream st) {

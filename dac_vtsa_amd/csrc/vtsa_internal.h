@@ -88,6 +88,7 @@ struct Handle {
   GmresScalars* gs_host = nullptr;
   Reduce red;
   double* host_scalars = nullptr;   // pinned [64]
+  double* dg = nullptr;              // diagnostics scratch of the outlet rows (allocated on first use)
   int* progress = nullptr;          // coherent pinned host word: GMRES columns completed (< 0: done)
   int* progress_dev = nullptr;      // its device address
   // low-rank correction Q = I + Ud Dd Ud^T of the GMRES right preconditioner (scaled variables):
@@ -107,6 +108,9 @@ int check(hipError_t e, const char* what);
 void launch_props(Handle& h, const double* y, double* pb, double* f, double* qstar, hipStream_t s);
 void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s);
 void launch_acc(Handle& h, double t, const double* y, const double* pb, double* acc, hipStream_t s);
+constexpr int OUTLET_ROW = 32;      // doubles per device outlet row
+void launch_outlet_sums(Handle& h, const double* cell, const double* bnd, int valve_mask, double* row, hipStream_t s);
+int reduce_weighted_dev(Handle& h, const double* a, int mode, double* result, hipStream_t st);
 void launch_diag(Handle& h, double t, const double* y, const double* pb, double* cell, double* uz, double* ur,
                  double* uth, double* bnd, double* wall, hipStream_t s);
 void launch_fdjac(Handle& h, double t, const double* y, const double* f0, hipStream_t s);

@@ -571,4 +571,80 @@ void launch_matvec_residual(Handle& h, double t, const double* z, const double* 
                      s, f, h.gs, rd, dh, w);
 }
 
+// ------------------------------------------------------------------ outlet rows
+// The sums one outlet row of the step runner is formed from (steps/_runner.py:104-121 forms the
+// row from the full diagnostic fields on the host; here one block reduces them on the device and
+// the host divides).  Per end e (0 top, 1 bottom; 11 values at 11 e): the molar flow leaving
+// through all faces (Fout = -F_in), sum P_face A, then over the faces with net outflow (Fout =
+// F_outpart for a valve end, bit e of valve_mask): their flow, their flow per gas (3), sum T tot;
+// then over all faces sum y_k A (3) and sum T A, at the layer next to that end.  At 22, 23: sum
+// V T and sum V P over the bed.  One block of 1024 threads, fixed order: deterministic.
+__global__ void __launch_bounds__(1024) k_outlet_sums(Model m, const double* __restrict__ cell,
+                                                      const double* __restrict__ bnd, int valve_mask,
+                                                      double* __restrict__ row) {
+  constexpr int NS = 24;
+  __shared__ double sh[16][NS];
+  const int Nr = m.d.Nr, Nth = m.d.Nth, Nz = m.d.Nz, n = m.n, nf = Nr * Nth;
+  double acc[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) acc[q] = 0.0;
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    const int i = f / Nth, j = f - i * Nth;
+    const double A = m.Az[i];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const double* B = bnd + (size_t)e * 14 * nf;
+      const int idx = (i * Nth + j) * Nz + (e == 0 ? 0 : Nz - 1);
+      const double T = cell[idx];
+      double F[NG];
+#pragma unroll
+      for (int k = 0; k < NG; ++k) F[k] = -B[(size_t)k * nf + f];
+      double tot = (F[0] + F[1]) + F[2];
+      acc[11 * e] += tot;
+      if ((valve_mask >> e) & 1) {
+#pragma unroll
+        for (int k = 0; k < NG; ++k) F[k] = B[(size_t)(9 + k) * nf + f];
+        tot = (F[0] + F[1]) + F[2];
+      }
+      acc[11 * e + 1] += B[(size_t)5 * nf + f] * A;
+      if (tot > 0.0) {
+        acc[11 * e + 2] += tot;
+#pragma unroll
+        for (int k = 0; k < NG; ++k) acc[11 * e + 3 + k] += F[k];
+        acc[11 * e + 6] += T * tot;
+      }
+#pragma unroll
+      for (int k = 0; k < NG; ++k) acc[11 * e + 7 + k] += cell[(size_t)(2 + k) * n + idx] * A;
+      acc[11 * e + 10] += T * A;
+    }
+  }
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    const double V = m.V[c / (Nth * Nz)];
+    acc[22] += V * cell[c];
+    acc[23] += V * cell[n + c];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) o[q] = __shfl_xor(acc[q], off, 64);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc[q] += o[q];
+  }
+  const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < NS; ++q) sh[wv][q] = acc[q];
+  __syncthreads();
+  if ((int)threadIdx.x < NS) {
+    double v = 0.0;
+    for (int w = 0; w < nwv; ++w) v += sh[w][threadIdx.x];
+    row[threadIdx.x] = v;
+  }
+}
+
+void launch_outlet_sums(Handle& h, const double* cell, const double* bnd, int valve_mask, double* row, hipStream_t s) {
+  hipLaunchKernelGGL(k_outlet_sums, dim3(1), dim3(1024), 0, s, h.m, cell, bnd, valve_mask, row);
+}
+
 }  // namespace vtsa

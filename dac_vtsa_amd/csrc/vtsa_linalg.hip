@@ -543,6 +543,13 @@ void launch_residual_combine(Handle& h, const double* z, const double* f, double
                      (const double*)nullptr, R, h.red.partial, h.red.counter + 2, h.red.result);
 }
 
+// The same reduction with its results left in device memory (inventories: mode 1, 7 values).
+int reduce_weighted_dev(Handle& h, const double* a, int mode, double* result, hipStream_t st) {
+  hipLaunchKernelGGL(k_weighted, dim3(rgrid(h.n + h.nw)), dim3(RBLK), 0, st, h.m, mode, a, nullptr, h.red.partial,
+                     h.red.counter + 3, result);
+  return check(hipGetLastError(), "k_weighted");
+}
+
 int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mode, double* out, hipStream_t st) {
   (void)len;
   hipLaunchKernelGGL(k_weighted, dim3(rgrid(h.n + h.nw)), dim3(RBLK), 0, st, h.m, mode, a, w, h.red.partial,

@@ -11,6 +11,8 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
+from .. import _abi
+from .. import device as dv
 from ..core.residual import TVSAModel
 from ..core.solver import TRBDF2, SolverError
 from ..properties import R_GAS
@@ -51,6 +53,84 @@ def _end_summary(model, d, end, c_feed_co2):
     return out
 
 
+OUTLET_ROW = 32      # doubles per device outlet row (vtsa_outlet_row)
+OUTLET_RING = 2048   # rows held on the device before they are copied to the host
+
+
+def _end_from_sums(model, r, e, end, c_feed_co2, A_tot):
+    """The outlet summary of one end (the keys and formulas of _end_summary) from the device
+    sums of vtsa_outlet_row, r = that row, e = 0 (top) or 1 (bottom)."""
+    b = 11 * e
+    Ftot, PA, F_leaving = float(r[b]), float(r[b + 1]), float(r[b + 2])
+    P_face = PA / A_tot
+    if F_leaving > 1e-14:
+        ys = [float(r[b + 3 + k]) / F_leaving for k in range(model.ng)]
+        T_g = float(r[b + 6]) / F_leaving
+    else:
+        ys = [float(r[b + 7 + k]) / A_tot for k in range(model.ng)]
+        T_g = float(r[b + 10]) / A_tot
+    out = {f"{end}_valve": model.conds[end].valve or "closed", f"{end}_valve_state": model.conds[end].state,
+           f"{end}_flow_out_mol_s": Ftot, f"{end}_P_Pa": P_face, f"{end}_T_K": T_g}
+    for k, s in enumerate(model.species):
+        out[f"{end}_y_{s}"] = ys[k]
+    c_co2 = ys[model.i_co2] * P_face / (R_GAS * T_g)
+    out[f"{end}_c_CO2_mol_m3"] = c_co2
+    out[f"{end}_C_over_Cfeed"] = c_co2 / c_feed_co2 if c_feed_co2 > 0 else np.nan
+    return out
+
+
+class _OutletRing:
+    """Outlet rows recorded on the device (no upstream counterpart: steps/_runner.py:104-121 copies
+    the diagnostic fields to the host at every recorded step).  Each record queues one
+    vtsa_outlet_row into a device ring; the host keeps the row's times and accumulator values and
+    forms the rows when the ring is full and at the end of the step, one copy for many rows."""
+
+    def __init__(self, model, head, c_feed_co2, out_rows):
+        self.m = model
+        self.head = head
+        self.c_feed = c_feed_co2
+        self.rows = out_rows
+        self.buf = dv.empty(OUTLET_RING * OUTLET_ROW)
+        self.pending = []
+        self.mask = sum(1 << e for e, end in enumerate(("top", "bottom")) if model.conds[end].kind == "valve")
+        g = model.grid
+        self.A_tot = float(np.sum(g.Az * np.ones((g.Nr, g.Nth))))
+        self.Vt = float(g.volume_full().sum())
+
+    def record(self, t, y_dev, acc):
+        if len(self.pending) == OUTLET_RING:
+            self.flush()
+        slot = len(self.pending)
+        _abi.call("vtsa_outlet_row", self.m.handle, float(t), dv.ptr(y_dev), self.mask,
+                  dv.ptr(self.buf.view(slot * OUTLET_ROW, OUTLET_ROW)), dv.stream())
+        ai = self.m.acc_index
+        extra = [float(acc[ai[f"n_out_{v}_CO2"]]) for v in ("V1", "V2", "V3")]
+        extra.append(float(acc[ai["Q_jac_pos"]] + acc[ai["Q_jac_neg"]]))
+        self.pending.append((float(t), extra))
+
+    def flush(self):
+        if not self.pending:
+            return
+        sums = dv.to_host(self.buf.view(0, len(self.pending) * OUTLET_ROW)).reshape(-1, OUTLET_ROW)
+        m = self.m
+        for (t, extra), r in zip(self.pending, sums):
+            row = self.head(t)
+            for e, end in enumerate(("top", "bottom")):
+                row.update(_end_from_sums(m, r, e, end, self.c_feed, self.A_tot))
+            row["bed_T_mean_K"] = float(r[22]) / self.Vt
+            row["bed_P_mean_Pa"] = float(r[23]) / self.Vt
+            for k, s in enumerate(m.adsorbed):
+                row[f"bed_q_{s}_mean"] = float(r[24 + 3 + k]) / m.rho_b / self.Vt
+            for v, val in zip(("V1", "V2", "V3"), extra[:3]):
+                row[f"cum_CO2_out_{v}_mol"] = val
+            row["cum_Q_jacket_J"] = extra[3]
+            self.rows.append(row)
+        self.pending.clear()
+
+    def last_t(self):
+        return self.pending[-1][0] if self.pending else (self.rows[-1]["t_step_s"] if self.rows else -np.inf)
+
+
 def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle0: float = 0.0,
              T_jacket_prev: Optional[float] = None, logger=None, snapshots: Optional[int] = None,
              record_outlet: bool = True, integrator_kw: Optional[Dict] = None, progress_s: float = 30.0,
@@ -60,7 +140,8 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
     ``record_outlet``, ``integrator_kw`` (preconditioner kind/precision) and ``stop_at`` (end the
     step at the first accepted time step reaching this step time, as a stop criterion does; only
     for steps without one of their own) have no upstream counterpart; the defaults reproduce the
-    upstream behaviour."""
+    upstream behaviour.  The outlet rows are reduced on the device (record_outlet=True) or, with
+    record_outlet="host", from diagnostic fields copied to the host as upstream does."""
     log = logger or (lambda *a, **k: None)
     model = TVSAModel(cfg, grid, props, step, state_in, T_jacket_prev)
     y0 = model.pack(state_in)
@@ -77,10 +158,18 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
     Vt = float(V.sum())
     Vr = np.ascontiguousarray(V)
 
+    def head(t):
+        return {"cycle": cycle_idx, "step": step["name"], "t_step_s": t, "t_cycle_s": t_cycle0 + t,
+                "t_total_s": state_in.t + t}
+
+    ring = _OutletRing(model, head, c_feed_co2, outlet) if record_outlet and record_outlet != "host" else None
+
     def record(t, y, acc):
+        if ring is not None:
+            ring.record(t, y if not isinstance(y, np.ndarray) else dv.to_dev(y), acc)
+            return
         d = model.diag(t, y)
-        row = {"cycle": cycle_idx, "step": step["name"], "t_step_s": t, "t_cycle_s": t_cycle0 + t,
-               "t_total_s": state_in.t + t}
+        row = head(t)
         for end in ("top", "bottom"):
             row.update(_end_summary(model, d, end, c_feed_co2))
         row["bed_T_mean_K"] = float(np.sum(d["T"] * Vr) / Vt)
@@ -133,6 +222,8 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
         for level, mode in ((1, "project_strict"), (2, "polish")):
             log(f"    {step['name']}: {e}; running the step again with conserve={mode}")
             outlet.clear()
+            if ring is not None:
+                ring.pending.clear()
             counter[0] = 0
             if record_outlet:
                 record(0.0, y0, np.zeros(model.n_acc))
@@ -152,8 +243,12 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
     solver_stats = {"method": "trbdf2", "steps": res.n_steps, "rejected": res.n_reject,
                     "newton_fail": res.n_newton_fail, "krylov": res.n_krylov, "jacobians": res.n_jac,
                     "lu": res.n_lu, "nfev": res.nfev, "groups": "all-columns", **integ.stats}
-    if record_outlet and outlet[-1]["t_step_s"] < res.t:
-        record(res.t, res.y, res.acc)
+    if record_outlet:
+        last = ring.last_t() if ring is not None else outlet[-1]["t_step_s"]
+        if last < res.t:
+            record(res.t, res.y, res.acc)
+        if ring is not None:
+            ring.flush()
     wall = time.time() - wall0
     state_out = model.to_state(res.y, state_in, state_in.t + res.t, step["name"])
     state_out.cycle = cycle_idx

@@ -195,6 +195,13 @@ int vtsa_cons_shift(vtsa_handle* h, const double* z, const double* s, const doub
  * vtsa_cons_fetch returns slots 0..n-1 on the host, wr[4 n] and ws[4 n] (synchronises). */
 int vtsa_cons_stash(vtsa_handle* h, const double* R, const double* s, int32_t slot, vtsa_stream st);
 int vtsa_cons_fetch(vtsa_handle* h, int32_t n, double* wr, double* ws, vtsa_stream st);
+/* One outlet row of the step record, reduced on the device into row[32] (device memory, no
+ * host wait), replacing the host reductions of steps/_runner.py:104-121 over the full
+ * diagnostic fields: per end (top at 0, bottom at 11) the flow leaving through all faces, sum
+ * P_face A, over the faces with net outflow (the valve's outflow part for an end whose bit is set
+ * in valve_mask) their flow, their flow per gas (3) and sum T flow, over all faces sum y_k A (3)
+ * and sum T A; at 22, 23 sum V T and sum V P; at 24 .. 30 the inventories of vtsa_inventories. */
+int vtsa_outlet_row(vtsa_handle* h, double t, const double* y, int32_t valve_mask, double* row, vtsa_stream st);
 /* inventories: out[0..2] gas moles, out[3..4] adsorbed moles, out[5] E_bed, out[6] E_wall
  * (core/residual.py:426-443) */
 int vtsa_inventories(vtsa_handle* h, const double* y, double* out, vtsa_stream st);
