@@ -4,9 +4,11 @@ Workload (the metric's configuration): grid Nr=20, Ntheta=32, Nz=100 (64,000 cel
 unknowns), the default cycle adsorption 8,000 s -> blowdown 60 s -> heating 22,400 s -> cooling
 400 s (30,860 s), from the clean bed (feed gas at 288.15 K and 1 atm over bare sorbent, the
 upstream default ``run.initial_state``).  FP64 residual, FD Jacobian, Newton and GMRES(60)
-(Arnoldi orthogonalisation in FP64), line Gauss-Seidel preconditioner (2 symmetric sweeps of the
-z-line block LU), conservation projection, conservation_tol 1e-5 so that every step's mass and
-energy balance closes below 1e-6.
+(Arnoldi orthogonalisation in FP64), line Gauss-Seidel preconditioner (2 forward sweeps over the
+two colours of z-lines, each line solved by its block LU), conservation projection,
+conservation_tol 1e-5 so that every step's mass and energy balance closes below 1e-6.  Every
+form uses the same preconditioner order; they differ in the factors' precision and the learned
+parts only.
 
 One bench "step" is one window of 1,543 s of simulated time (1/20 of the cycle) of the cycle
 schedule run continuously from the clean bed: ``--steps 20`` times exactly one full cycle.  The
@@ -342,8 +344,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="windows of 1,543 simulated s (20 = one cycle)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=2, help="line Gauss-Seidel sweeps of the preconditioner")
+    ap.add_argument("--gs-order", default="forward", choices=("forward", "symmetric"),
+                    help="line Gauss-Seidel colour order (forward: 0 1 0 1; symmetric: 0 1 0 1 0)")
     ap.add_argument("--krylov", type=int, default=60, help="GMRES restart length")
-    ap.add_argument("--lowrank", type=int, default=4, help="rank of the value leg's preconditioner correction (0 off)")
+    ap.add_argument("--lowrank", type=int, default=8, help="rank of the value leg's preconditioner correction (0 off)")
     ap.add_argument("--cpu-budget-s", type=float, default=40.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
@@ -367,7 +371,7 @@ def main():
               "one group per GPU with no communication; no multi-GPU run is part of this build)", file=sys.stderr)
         return 2
     from dac_vtsa_amd.core import residual
-    base_kw = {"gs_sweeps": args.sweeps, "conserve": "project", "krylov_dim": args.krylov}
+    base_kw = {"gs_sweeps": args.sweeps, "gs_order": args.gs_order, "conserve": "project", "krylov_dim": args.krylov}
     cfg, grid, props, bed = setup()
     if args.kernel_probe:
         print(json.dumps(kernel_probes(cfg, grid, props, bed)), flush=True)
@@ -410,7 +414,8 @@ def main():
                    "form": ("learned stage guess + " if net is not None else "") +
                            (f"rank-{args.lowrank} preconditioner correction + " if args.lowrank else "") +
                            "FP32 preconditioner factors",
-                   "solver": f"sdirk2 Newton-GMRES({args.krylov}), line Gauss-Seidel ({args.sweeps} sweeps), projection",
+                   "solver": f"sdirk2 Newton-GMRES({args.krylov}), line Gauss-Seidel ({args.sweeps} {args.gs_order} "
+                             f"sweeps), projection",
                    "rtol": cfg.solver.rtol, "conservation_tol": cfg.solver.conservation_tol},
         "balance": main["balance"], "solver": main["solver"], "steps_detail": main["steps"],
         "roofline": {"bound": "hbm", "kernel": "k_gs_half (line Gauss-Seidel half-sweep)", "achieved": achieved,

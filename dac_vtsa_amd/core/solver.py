@@ -592,6 +592,7 @@ class TRBDF2:
 
     Extra keyword arguments (no upstream counterpart): ``precond`` ("zline_gs" | "zline" |
     "block_jacobi"), ``gs_sweeps`` (symmetric line Gauss-Seidel sweeps of "zline_gs"),
+    ``gs_order`` ("symmetric": colours 0 1 0 1 0 for two sweeps; "forward": 0 1 0 1),
     ``precision`` ("fp64" | "fp32") of the preconditioner factors, ``krylov_dim``, ``conserve``
     ("polish": the upstream conservation polish; "project": project_conservation first;
     "project_strict": the same with no shifted entry below zero; with either projection a stage
@@ -618,7 +619,7 @@ class TRBDF2:
                  precond: str = "zline_gs", precision: str = "fp64", krylov_dim: int = 30,
                  gs_sweeps: int = 2, conserve: str = "polish", guess=None, record=None, h_sequence=None,
                  error_filter: str = "precond", kappa: Optional[float] = None, lowrank: int = 0,
-                 lowrank_window=(1e-3, 0.3)):
+                 lowrank_window=(1e-3, 0.6), gs_order: str = "symmetric"):
         self.m = model
         s = cfg.solver
         self.rtol = s.rtol
@@ -650,6 +651,9 @@ class TRBDF2:
         self.precision = PRECISIONS[precision]
         self.krylov_dim = krylov_dim
         self.gs_sweeps = int(gs_sweeps)
+        if gs_order not in ("symmetric", "forward"):
+            raise ValueError(f"gs_order={gs_order!r}")
+        _abi.call("vtsa_set_option", model.handle, _abi.OPT_GS_FORWARD, int(gs_order == "forward"))
         if conserve not in ("polish", "project", "project_strict"):
             raise ValueError(f"conserve={conserve!r}")
         self.conserve = conserve

@@ -549,6 +549,10 @@ int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {
       if (value < 0 || value > 5) { set_error("vtsa_set_option: staging level must be 0 .. 5"); return -2; }
       H->h.gs_full = value;
       return 0;
+    case VTSA_OPT_GS_FORWARD:
+      if (value < 0 || value > 1) { set_error("vtsa_set_option: sweep order must be 0 or 1"); return -2; }
+      H->h.gs_forward = value;
+      return 0;
     default: set_error("vtsa_set_option: unknown option"); return -2;
   }
 }

@@ -69,6 +69,7 @@ struct Handle {
   std::vector<int> gs_cstart;        // colour c owns gs_order[gs_cstart[c] .. gs_cstart[c + 1])
   int gs_sweeps = 1;
   int gs_full = 1;                   // LDS staging of a half-sweep: 0 segments, 1 whole line
+  int gs_forward = 0;                // line Gauss-Seidel order: 0 symmetric sweeps, 1 forward sweeps only
   int lds_max = 65536;               // LDS bytes one block may use on this device
   int lds_per_cu = 65536;            // LDS bytes of one CU
   int n_cu = 256;                    // compute units of the device

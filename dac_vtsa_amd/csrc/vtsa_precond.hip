@@ -1463,11 +1463,17 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   const bool fold = C <= 2;
   if (!fold) hipLaunchKernelGGL(k_gs_gather, dim3((N + 255) / 256), dim3(256), 0, s, N, a, h.s0, h.gsb, h.gsu);
   std::vector<int> seq;
-  for (int c = 0; c < C; ++c) seq.push_back(c);
-  for (int c = C - 2; c >= 0; --c) seq.push_back(c);
-  for (int sw = 1; sw < h.gs_sweeps; ++sw) {
-    for (int c = 1; c < C; ++c) seq.push_back(c);
+  if (h.gs_forward) {
+    // forward sweeps only: colours 0 .. C-1, gs_sweeps times
+    for (int sw = 0; sw < h.gs_sweeps; ++sw)
+      for (int c = 0; c < C; ++c) seq.push_back(c);
+  } else {
+    for (int c = 0; c < C; ++c) seq.push_back(c);
     for (int c = C - 2; c >= 0; --c) seq.push_back(c);
+    for (int sw = 1; sw < h.gs_sweeps; ++sw) {
+      for (int c = 1; c < C; ++c) seq.push_back(c);
+      for (int c = C - 2; c >= 0; --c) seq.push_back(c);
+    }
   }
   // a colour's last visit writes the apply's outputs for its lines
   std::vector<int> last(C, -1), first(C, -1);

@@ -257,8 +257,9 @@ int vtsa_time_fdjac(vtsa_handle* h, double t, const double* y, const double* f0,
  * first, with FP64 factors as 4 (each when granted); 0 cell factors in segments during the sweep.
  * Automatic tries 5 first for FP32 factors.  Levels 1-4 give bit-identical results, 5 with FP32
  * factors the same recurrences on a right-hand side summed in another order; level 0 solves the
- * same line systems cell by cell (other roundings). */
-enum { VTSA_OPT_GS_WHOLE_LINES = 1 };
+ * same line systems cell by cell (other roundings).  VTSA_OPT_GS_FORWARD: 0 (default) symmetric
+ * sweeps (colours 0 1 0 1 0 for two sweeps of two colours), 1 forward sweeps only (0 1 0 1). */
+enum { VTSA_OPT_GS_WHOLE_LINES = 1, VTSA_OPT_GS_FORWARD = 2 };
 int vtsa_set_option(vtsa_handle* h, int32_t option, int32_t value);
 
 /* Sampled timing of the line Gauss-Seidel half-sweep kernel, the dominant kernel of a step (no

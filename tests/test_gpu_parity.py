@@ -216,11 +216,11 @@ def test_preconditioner_matches_dense_solve(kind, prec, tol):
     assert np.max(np.abs(out - ref) / sc) / np.max(np.abs(ref) / sc) < tol
 
 
-def _line_gs_reference(M, m, r, sweeps):
+def _line_gs_reference(M, m, r, sweeps, forward=False):
     """Line Gauss-Seidel on the dense M (host reference): one line per cell z-column (its six
     variables over Nz cells) and one per wall column; greedy colouring of the line graph in line
-    order; colour sequence 0..C-1, C-2..0, plus 1..C-1, C-2..0 per extra sweep; each line solved
-    exactly against the current values of the other lines."""
+    order; colour sequence 0..C-1, C-2..0, plus 1..C-1, C-2..0 per extra sweep (forward: 0..C-1
+    per sweep); each line solved exactly against the current values of the other lines."""
     g = m.grid
     Nr, Nth, Nz, n = g.Nr, g.Nth, g.Nz, g.ncell
     ncol = Nr * Nth
@@ -248,9 +248,12 @@ def _line_gs_reference(M, m, r, sweeps):
         used = {colour[K] for K in nbrs(L) if K != L}
         colour[L] = next(c for c in range(len(lines)) if c not in used)
     C = max(colour) + 1
-    seq = list(range(C)) + list(range(C - 2, -1, -1))
-    for _ in range(1, sweeps):
-        seq += list(range(1, C)) + list(range(C - 2, -1, -1))
+    if forward:
+        seq = list(range(C)) * sweeps
+    else:
+        seq = list(range(C)) + list(range(C - 2, -1, -1))
+        for _ in range(1, sweeps):
+            seq += list(range(1, C)) + list(range(C - 2, -1, -1))
     x = np.zeros(M.shape[0])
     for c in seq:
         for L in (L for L in range(len(lines)) if colour[L] == c):
@@ -316,6 +319,14 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
         # neighbour first (other roundings of the right-hand side)
         assert np.max(np.abs(outs[0] - outs[3]) / sc) / np.max(np.abs(outs[0]) / sc) < 1e-11
     assert np.max(np.abs(outs[0] - outs[4]) / sc) / np.max(np.abs(ref) / sc) < tol_pair
+    # forward sweeps only (colours 0..C-1 per sweep)
+    ref_f = _line_gs_reference(np.eye(m.n_core) - dh * J, m, r, sweeps, forward=True)
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_FORWARD, 1)
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_WHOLE_LINES, 1)
+    out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
+    assert np.max(np.abs(out - ref_f) / sc) / np.max(np.abs(ref_f) / sc) < tol_pair
+    assert np.max(np.abs(out - outs[0]) / sc) > 0.0     # a different operator from the symmetric one
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_FORWARD, 0)
 
 
 @pytest.mark.parametrize("dh,rtol", [(0.05, 1e-2), (0.05, 1e-4), (0.3, 1e-2)])
