@@ -93,7 +93,12 @@ class TVSAModel:
 
     def __init__(self, cfg, grid, props, step: Dict, state0, T_jacket_prev: Optional[float] = None):
         if not cfg.components.water:
-            raise NotImplementedError("the device model covers humid air (components.water = true)")
+            # upstream builds the dry model (core/residual.py:46-50) but its first residual raises
+            # TypeError in the equilibrium (physics/isotherms.py:118, np.maximum(None, 1e-12)), so
+            # there is no dry-air behaviour to reproduce; refused here before any device work
+            raise NotImplementedError("components.water = false (dry air): the upstream residual of this case "
+                                      "raises TypeError at physics/isotherms.py:118; the device model covers "
+                                      "humid air (components.water = true)")
         self.cfg, self.grid, self.props, self.step = cfg, grid, props, step
         g = grid
         self.species = list(props.species)

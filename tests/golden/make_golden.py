@@ -23,6 +23,9 @@ for the functions that no closed form, invariant or known answer pins on its own
 * ``integ_cfg1_200s.npz`` : the whole 200 s adsorption step of the seeded 4x4x20 state with a
                         tight Newton tolerance (newton_kappa 1e-4, the smallest the configuration allows), sdirk2 and trbdf2: accepted
                         times, counters, snapshots, end state and accumulators;
+* ``dry_air.json``    : the upstream dry-air case (components.water = false) on 3x4x6, stage by
+                        stage (initial bed, model, residual): which complete, and the exception the
+                        first failing one raises with its upstream file:line;
 * ``state_ref.csv``   : a state file written by the upstream save_state_csv (the seeded 3x4x6
                         state), read back by this package's loader.
 
@@ -318,6 +321,36 @@ def make_state_csv(path):
     save_state_csv(small_state(cfg, g, get_backend(cfg)), g, path)
 
 
+def make_dry_air(path):
+    """The upstream dry-air case (config/cases/dry_air.yaml: components.water = false) on a 3x4x6
+    grid, stage by stage: which stages complete and, for the first that does not, the exception
+    type, message and the upstream frame (file:line) it was raised in."""
+    import json
+    import traceback
+    cfg = load_config(os.path.join(REF, "config", "cases", "dry_air.yaml"), override_dict=SMALL)
+    g = Grid.from_config(cfg)
+    props = get_backend(cfg)
+    out = {"case": "config/cases/dry_air.yaml", "grid": [g.Nr, g.Nth, g.Nz], "species": list(props.species),
+           "stages": {}}
+    st = m = None
+    for stage in ("initial_state", "model", "rhs"):
+        try:
+            if stage == "initial_state":
+                st = initial_state(cfg, g, props)
+            elif stage == "model":
+                m = TVSAModel(cfg, g, props, cfg.cycle[0], st)
+            else:
+                m.rhs(1.0, m.pack(st))
+            out["stages"][stage] = "ok"
+        except Exception as e:  # the outcome is the fixture
+            fr = traceback.extract_tb(e.__traceback__)[-1]
+            out["stages"][stage] = {"raised": type(e).__name__, "message": str(e),
+                                    "at": f"{os.path.relpath(fr.filename, REF)}:{fr.lineno}"}
+            break
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 MAKERS = {"closures": make_closures, "rhs_small": make_rhs_small, "rhs_cfg1": make_rhs_cfg1, "rhs_mid": make_rhs_mid,
           "runner_cfg1": make_runner, "integ_cfg1_200s": make_integ_200}
 
@@ -327,6 +360,8 @@ if __name__ == "__main__" and len(sys.argv) > 1:
         t0 = time.time()
         if stem == "state_ref":
             make_state_csv(os.path.join(HERE, "state_ref.csv"))
+        elif stem == "dry_air":
+            make_dry_air(os.path.join(HERE, "dry_air.json"))
         else:
             MAKERS[stem](os.path.join(HERE, stem + ".npz"))
         print(stem, "done in %.1f s" % (time.time() - t0), flush=True)

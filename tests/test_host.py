@@ -246,3 +246,26 @@ def test_descriptor_builds_from_an_upstream_like_property_backend(monkeypatch):
         a = residual.TVSAModel(cfg, g, ours, cfg.cycle[k], st).desc
         b = residual.TVSAModel(cfg, g, up, cfg.cycle[k], st).desc
         assert bytes(a) == bytes(b)
+
+
+def test_dry_air_is_refused_where_upstream_cannot_run_it():
+    """Dry air (components.water = false): the upstream case builds its bed and model but its
+    first residual raises (tests/golden/dry_air.json, written from the upstream code); the device
+    model refuses the case at construction with an error that names why."""
+    import json
+    from dac_vtsa_amd.core.grid import Grid
+    from dac_vtsa_amd.core.residual import TVSAModel
+    from dac_vtsa_amd.core.state import initial_state
+    from dac_vtsa_amd.properties import get_backend
+    with open(os.path.join(os.path.dirname(__file__), "golden", "dry_air.json")) as fh:
+        ref = json.load(fh)
+    assert ref["species"] == ["CO2", "N2"]
+    assert ref["stages"]["initial_state"] == "ok" and ref["stages"]["model"] == "ok"
+    assert ref["stages"]["rhs"]["raised"] == "TypeError"
+    assert ref["stages"]["rhs"]["at"].startswith("physics/isotherms.py:")
+    cfg = load_config(override_dict={"grid.Nr": 3, "grid.Ntheta": 4, "grid.Nz": 6, "components.water": False})
+    g = Grid.from_config(cfg)
+    humid = load_config(override_dict={"grid.Nr": 3, "grid.Ntheta": 4, "grid.Nz": 6})
+    st = initial_state(humid, g, get_backend(humid))
+    with pytest.raises(NotImplementedError, match=r"dry air.*isotherms\.py:118"):
+        TVSAModel(cfg, g, get_backend(humid), cfg.cycle[0], st)
