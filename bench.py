@@ -100,7 +100,7 @@ def run_windows(cfg, grid, props, state, n_windows, integrator_kw, log=None):
             stop = left if left < dur * (1.0 - 1e-12) else None
             state, sr = RUNNERS[step["type"]](state, cfg, grid, props, step=step, cycle_idx=cycle,
                                               T_jacket_prev=T_jac, logger=log, integrator_kw=integrator_kw,
-                                              stop_at=stop, progress_s=60.0)
+                                              stop_at=stop, progress_s=60.0, device_handoff=True)
             t_done += sr.duration
             T_jac = sr.T_jacket if sr.jacket_on else T_jac
             results.append(sr)

@@ -69,6 +69,7 @@ class GmresInfo(C.Structure):
 _SIGS = {
     "vtsa_create": [C.POINTER(ModelDesc)] + [P] * 9 + [C.POINTER(P)],
     "vtsa_destroy": [P],
+    "vtsa_reset": [P, C.POINTER(ModelDesc)] + [P] * 9 + [P],
     "vtsa_sizes": [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
     "vtsa_rhs": [P, D, P, P, P, P],
     "vtsa_stage_residual": [P, D, P, D, P, P, P],
@@ -90,6 +91,7 @@ _SIGS = {
     "vtsa_newton_try": [P, D, P, P, D, D, P, P, P, P, P, C.POINTER(D), C.POINTER(D), C.POINTER(D), P],
     "vtsa_cons": [P, P, P, P, P, P],
     "vtsa_inventories": [P, P, P, P],
+    "vtsa_state_roundtrip": [P, P, P],
     "vtsa_cons_norm2": [P, P, P, P],
     "vtsa_cons_shift": [P, P, P, P, I32, P, P],
     "vtsa_cons_stash": [P, P, P, I32, P],

@@ -163,7 +163,9 @@ def run_batch(cases: List[Dict], workers: int = 16, log=None, runner=None, **kw)
                 results[i] = r
         wall = time.time() - t0
     finally:
+        from .core.residual import release_handles
         for s in streams:
+            release_handles(s)
             dv.free_stream(s)
     sim = sum(r["sim_s"] for r in results)
     return {"cases": results, "wall_s": wall, "sim_s": sim, "value": sim / wall,

@@ -10,6 +10,7 @@ integrator uses the ``*_dev`` variants, which keep vectors resident in HBM.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from typing import Dict, List, Optional
 
 import numpy as np
@@ -34,6 +35,43 @@ KERNEL_PROFILE = {"every": 0, "launches": 0, "ms": 0.0, "bytes": 0.0}
 
 def reset_kernel_profile(every: int):
     KERNEL_PROFILE.update(every=int(every), launches=0, ms=0.0, bytes=0.0)
+
+
+# Device handles kept for the next model of the same grid (no upstream counterpart: upstream
+# builds a new model for every step, vtsa_cycle.py:80-86).  A closed model leaves its handle here,
+# keyed by grid and stream; the next model of that key re-initialises it with vtsa_reset instead of
+# allocating ~1 GB (20x32x100) anew, and no vtsa_destroy (a device-wide synchronisation) runs
+# between the steps of a cycle.  At most HANDLE_POOL_MAX handles are kept, the oldest destroyed.
+HANDLE_POOL_MAX = 4
+_handle_pool: List = []            # [(key, handle)], most recently released last
+_pool_lock = threading.RLock()     # reentrant: TVSAModel.__del__ releases into the pool
+
+
+def _pool_take(key):
+    with _pool_lock:
+        for i in range(len(_handle_pool) - 1, -1, -1):
+            if _handle_pool[i][0] == key:
+                return _handle_pool.pop(i)[1]
+    return None
+
+
+def _pool_put(key, h):
+    with _pool_lock:
+        _handle_pool.append((key, h))
+        evict = _handle_pool[:-HANDLE_POOL_MAX] if len(_handle_pool) > HANDLE_POOL_MAX else []
+        del _handle_pool[:len(evict)]
+    for _, old in evict:
+        _abi.lib().vtsa_destroy(old)
+
+
+def release_handles(stream=None):
+    """Destroy the pooled handles (of the models built on ``stream`` when given)."""
+    sk = None if stream is None else (stream.value if isinstance(stream, C.c_void_p) else stream)
+    with _pool_lock:
+        drop = [e for e in _handle_pool if stream is None or e[0][-1] == sk]
+        _handle_pool[:] = [e for e in _handle_pool if not any(e is d for d in drop)]
+    for _, h in drop:
+        _abi.lib().vtsa_destroy(h)
 
 
 class Thermo:
@@ -277,9 +315,16 @@ class TVSAModel:
     def _create(self):
         arrs = self._device_arrays()
         self._host_arrays = arrs
-        h = C.c_void_p()
         ptrs = [a.ctypes.data_as(C.c_void_p) for a in arrs]
-        _abi.call("vtsa_create", C.byref(self.desc), *ptrs, C.byref(h))
+        g = self.grid
+        self._pool_key = (g.Nr, g.Nth, g.Nz, int(self.desc.wall_on), dv._stream_key())
+        h = _pool_take(self._pool_key)
+        if h is not None and _abi.lib().vtsa_reset(h, C.byref(self.desc), *ptrs, dv.stream()) != 0:
+            _abi.lib().vtsa_destroy(h)
+            h = None
+        if h is None:
+            h = C.c_void_p()
+            _abi.call("vtsa_create", C.byref(self.desc), *ptrs, C.byref(h))
         self._handle = h
         if KERNEL_PROFILE["every"] > 0:
             _abi.call("vtsa_profile", h, int(KERNEL_PROFILE["every"]))
@@ -295,7 +340,7 @@ class TVSAModel:
                 KERNEL_PROFILE["launches"] += n.value
                 KERNEL_PROFILE["ms"] += ms.value
                 KERNEL_PROFILE["bytes"] += b.value
-            _abi.lib().vtsa_destroy(self._handle)
+            _pool_put(self._pool_key, self._handle)
             self._handle = None
 
     def __del__(self):

@@ -46,7 +46,14 @@ class CycleState:
         return self.q[self.adsorbed.index(name)]
 
     def copy(self) -> "CycleState":
-        return copy.deepcopy(self)
+        # a step runner may attach the bed's device copy (_y_device, no upstream counterpart);
+        # a copy is a host state of its own and does not carry it
+        dev = self.__dict__.pop("_y_device", None)
+        try:
+            return copy.deepcopy(self)
+        finally:
+            if dev is not None:
+                self._y_device = dev
 
 
 def initial_state(cfg, grid, props) -> CycleState:

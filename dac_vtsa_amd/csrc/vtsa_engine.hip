@@ -189,6 +189,46 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   return 0;
 }
 
+int vtsa_reset(vtsa_handle* H, const vtsa_model_desc* desc, const double* V, const double* Az, const double* Ath,
+               const double* arc, const double* Ar, const double* drc, const double* feed_flux, const double* N_nom,
+               const double* cov, vtsa_stream s) {
+  if (!H || !desc || !V || !Az || !Ath || !arc || !feed_flux || !N_nom || !cov) {
+    set_error("vtsa_reset: null argument");
+    return -1;
+  }
+  Handle& h = H->h;
+  if (desc->Nr != h.desc.Nr || desc->Nth != h.desc.Nth || desc->Nz != h.desc.Nz ||
+      (desc->wall_on != 0) != (h.desc.wall_on != 0)) {
+    set_error("vtsa_reset: the handle was created for another grid");
+    return -2;
+  }
+  if (desc->Nr > 1 && (!Ar || !drc)) { set_error("vtsa_reset: null argument"); return -1; }
+  const hipStream_t st = S(s);
+  // the previous model's work on this stream reads these arrays: the copies queue behind it
+  const int Nr = desc->Nr;
+  const size_t nrt = (size_t)Nr * desc->Nth, nwall = (size_t)desc->Nth * desc->Nz;
+  struct { double* dst; const double* src; size_t count; } cp[] = {
+      {h.V, V, (size_t)Nr}, {h.Az, Az, (size_t)Nr}, {h.Ath, Ath, (size_t)Nr}, {h.arc, arc, (size_t)Nr},
+      {h.Ar, Nr > 1 ? Ar : nullptr, (size_t)(Nr - 1)}, {h.drc, Nr > 1 ? drc : nullptr, (size_t)(Nr - 1)},
+      {h.feed_flux, feed_flux, nrt}, {h.N_nom, N_nom, nrt}, {h.cov, cov, nwall}};
+  for (const auto& c : cp)
+    if (c.src && c.count &&
+        check(hipMemcpyAsync(c.dst, c.src, sizeof(double) * c.count, hipMemcpyHostToDevice, st), "hipMemcpyAsync"))
+      return -1;
+  // the host arrays may go out of scope when this returns
+  if (check(hipStreamSynchronize(st), "hipStreamSynchronize")) return -1;
+  if (prof_resolve(h)) return -1;
+  h.desc = *desc;
+  h.m.d = h.desc;
+  h.prec_kind = -1;
+  h.prec_ready = 0;
+  h.prec_dh = 0.0;
+  h.defl_k = 0;
+  h.prof_seen = h.prof_launches = 0;
+  h.prof_ms = h.prof_bytes = 0.0;
+  return 0;
+}
+
 int vtsa_destroy(vtsa_handle* H) {
   if (!H) return 0;
   (void)hipDeviceSynchronize();
@@ -417,6 +457,12 @@ int vtsa_outlet_row(vtsa_handle* H, double t, const double* y, int32_t valve_mas
   launch_diag(h, t, y, h.pb, h.dg, h.dg + o_uz, h.dg + o_ur, h.dg + o_uth, h.dg + o_bnd, h.dg + o_wall, S(st));
   launch_outlet_sums(h, h.dg, h.dg + o_bnd, valve_mask, row, S(st));
   if (reduce_weighted_dev(h, y, 1, row + 24, S(st))) return -1;
+  return finish(S(st));
+}
+
+int vtsa_state_roundtrip(vtsa_handle* H, double* y, vtsa_stream st) {
+  if (!H || !y) { set_error("vtsa_state_roundtrip: null argument"); return -1; }
+  launch_energy_roundtrip(H->h, y, S(st));
   return finish(S(st));
 }
 

@@ -109,6 +109,7 @@ int check(hipError_t e, const char* what);
 void launch_props(Handle& h, const double* y, double* pb, double* f, double* qstar, hipStream_t s);
 void launch_flux(Handle& h, double t, const double* y, const double* pb, double* f, hipStream_t s);
 void launch_acc(Handle& h, double t, const double* y, const double* pb, double* acc, hipStream_t s);
+void launch_energy_roundtrip(Handle& h, double* y, hipStream_t s);
 constexpr int OUTLET_ROW = 32;      // doubles per device outlet row
 void launch_outlet_sums(Handle& h, const double* cell, const double* bnd, int valve_mask, double* row, hipStream_t s);
 int reduce_weighted_dev(Handle& h, const double* a, int mode, double* result, hipStream_t st);

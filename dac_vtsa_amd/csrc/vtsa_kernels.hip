@@ -388,6 +388,32 @@ __global__ void k_equilibrium(Model m, int count, const double* __restrict__ pc,
   qc[i] = q_co2(pc[i], T[i], d.water_basis_eq ? we : qw[i], d);
 }
 
+// ------------------------------------------------------------------ step hand-off
+// e <- energy(temperature(e)) per cell: what a hand-off through the state object does, which stores
+// T = (e - Hq + A T_ref) / (A - B) and has the next step form e = A (T - T_ref) - B T + Hq again
+// (physics/energy.py:46-62 in the association order of the host Thermo and of cell_state).  The
+// round trip is not the identity in floating point; applying it on the device keeps a hand-off in
+// device memory bit-identical to one through the host arrays.
+__global__ void k_energy_roundtrip(Model m, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  const vtsa_model_desc& d = m.d;
+  const int n = m.n;
+  const double c0 = y[i], c1 = y[n + i], c2 = y[2 * n + i], q0 = y[3 * n + i], q1 = y[4 * n + i];
+  const double e = y[5 * n + i];
+  const double Ctot = (c0 + c1) + c2;
+  const double gas_cp = (d.cpg[0] * c0 + d.cpg[1] * c1) + d.cpg[2] * c2;
+  const double A = d.eps_t * gas_cp + d.rho_b * (d.cp_s + (d.cpa[0] * q0 + d.cpa[1] * q1));
+  const double B = (d.eps_t * d.R) * Ctot;
+  const double Hq = d.rho_b * (d.dHa[0] * q0 + d.dHa[1] * q1);
+  const double T = ((e - Hq) + A * d.T_ref) / (A - B);
+  y[5 * n + i] = (A * (T - d.T_ref) - B * T) + Hq;
+}
+
+void launch_energy_roundtrip(Handle& h, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_roundtrip, dim3((h.n + BLK - 1) / BLK), dim3(BLK), 0, s, h.m, y);
+}
+
 // ------------------------------------------------------------------ finite-difference Jacobian
 // FD step of one column (core/jacobian.py:75-76): h = (y + 1.5e-8 max(|y|, typ)) - y
 __device__ inline double fd_step(double yv, double typ) {

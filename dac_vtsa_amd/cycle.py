@@ -1,5 +1,6 @@
-"""Cycle harness (upstream vtsa_cycle.py:72-138): steps chained with the state kept in memory
-(the CSV hand-off is optional), balances, KPIs and the cyclic-steady-state test."""
+"""Cycle harness (upstream vtsa_cycle.py:72-138): steps chained with the bed kept in device memory
+and each step's device model re-initialised in place (the CSV hand-off is optional), balances,
+KPIs and the cyclic-steady-state test."""
 from __future__ import annotations
 
 import os
@@ -27,7 +28,11 @@ class CycleRecord:
 
 
 def run_cycle(state, cfg, grid, props, cycle_idx: int = 1, cycle_dir=None, log=None, T_jac_prev=None,
-              handoff_csv: bool = False, **kw):
+              handoff_csv: bool = False, device_handoff: bool = False, **kw):
+    """The steps of cfg.cycle from ``state`` -> (end state, step results, last jacket set-point).
+    Between steps the bed stays in device memory (a CSV hand-off, when asked for, goes through the
+    file); ``device_handoff`` lets the first step start from the device copy ``state`` carries too
+    (a state returned by a step runner and not modified since)."""
     steps_out, t_cycle = [], 0.0
     prev_csv = None
     if handoff_csv and cycle_dir:
@@ -35,8 +40,10 @@ def run_cycle(state, cfg, grid, props, cycle_idx: int = 1, cycle_dir=None, log=N
     for step in cfg.cycle:
         if prev_csv:
             state = load_state_csv(prev_csv, grid, list(props.species), list(state.adsorbed))
+        # the bed goes from step to step in device memory unless it went through a state file
         state, sr = RUNNERS[step["type"]](state, cfg, grid, props, step=step, cycle_idx=cycle_idx,
-                                          t_cycle0=t_cycle, T_jacket_prev=T_jac_prev, logger=log, **kw)
+                                          t_cycle0=t_cycle, T_jacket_prev=T_jac_prev, logger=log,
+                                          device_handoff=not prev_csv and (device_handoff or bool(steps_out)), **kw)
         if handoff_csv and cycle_dir:
             prev_csv = save_state_csv(state, grid, os.path.join(cycle_dir, f"state_after_{step['name']}.csv"))
         t_cycle += sr.duration
@@ -55,7 +62,8 @@ def run_to_css(cfg, state, log=None, max_cycles=None, **kw):
     prev_end = state.copy()
     for n in range(1, (max_cycles or cfg.run.max_cycles) + 1):
         t0 = time.time()
-        state, steps, T_jac_prev = run_cycle(state, cfg, grid, props, n, log=log, T_jac_prev=T_jac_prev, **kw)
+        state, steps, T_jac_prev = run_cycle(state, cfg, grid, props, n, log=log, T_jac_prev=T_jac_prev,
+                                             device_handoff=n > 1, **kw)
         mb, eb, ok = balances.check_cycle(steps, species)
         kp = balances.cycle_kpis(steps, cfg, species)
         ch = balances.state_change(prev_end, state, grid, cfg)

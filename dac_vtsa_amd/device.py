@@ -151,6 +151,13 @@ class DeviceArray:
     def clone(self) -> "DeviceArray":
         return DeviceArray(self.n, self.dtype).copy_from(self)
 
+    # a copy owns memory of its own (two owners of one pointer would both return it to the cache)
+    def __copy__(self):
+        return self.clone()
+
+    def __deepcopy__(self, memo):
+        return self.clone()
+
     def view(self, offset: int, n: int) -> "DeviceView":
         """A window [offset, offset + n) of this array (shares its memory)."""
         if offset < 0 or n < 0 or offset + n > self.n:

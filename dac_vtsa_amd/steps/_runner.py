@@ -134,17 +134,30 @@ class _OutletRing:
 def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle0: float = 0.0,
              T_jacket_prev: Optional[float] = None, logger=None, snapshots: Optional[int] = None,
              record_outlet: bool = True, integrator_kw: Optional[Dict] = None, progress_s: float = 30.0,
-             stop_at: Optional[float] = None):
+             stop_at: Optional[float] = None, device_handoff: bool = False):
     """One cycle step on the GPU -> (state_out, StepResult) (steps/_runner.py:87-188).
 
-    ``record_outlet``, ``integrator_kw`` (preconditioner kind/precision) and ``stop_at`` (end the
+    ``record_outlet``, ``integrator_kw`` (preconditioner kind/precision), ``stop_at`` (end the
     step at the first accepted time step reaching this step time, as a stop criterion does; only
-    for steps without one of their own) have no upstream counterpart; the defaults reproduce the
-    upstream behaviour.  The outlet rows are reduced on the device (record_outlet=True) or, with
-    record_outlet="host", from diagnostic fields copied to the host as upstream does."""
+    for steps without one of their own) and ``device_handoff`` have no upstream counterpart; the
+    defaults reproduce the upstream behaviour.  The outlet rows are reduced on the device
+    (record_outlet=True) or, with record_outlet="host", from diagnostic fields copied to the host
+    as upstream does.  The returned state carries the bed's device copy; with device_handoff=True
+    a state returned by an earlier step on this stream (and not modified since) starts this step
+    from that copy, so the bed stays in HBM from step to step.  The host arrays are filled either
+    way."""
     log = logger or (lambda *a, **k: None)
     model = TVSAModel(cfg, grid, props, step, state_in, T_jacket_prev)
-    y0 = model.pack(state_in)
+    y_in = getattr(state_in, "_y_device", None) if device_handoff else None
+    if y_in is not None and (y_in.n != model.n_core or y_in._key[0] != dv._stream_key()):
+        y_in = None     # another layout, or made on another stream: uploaded from the host arrays
+    if y_in is not None:
+        # the host arrays hold T, from which pack re-forms e; the same round trip on the device
+        # keeps this start bit-identical to one from the host state
+        y0 = y_in.clone()
+        _abi.call("vtsa_state_roundtrip", model.handle, dv.ptr(y0), dv.stream())
+    else:
+        y0 = dv.to_dev(model.pack(state_in))
     dur = float(step["duration"])
     n_snap = snapshots if snapshots is not None else cfg.run.snapshots_per_step
     t_out = np.linspace(0.0, dur, max(2, n_snap))
@@ -168,7 +181,7 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
         if ring is not None:
             ring.record(t, y if not isinstance(y, np.ndarray) else dv.to_dev(y), acc)
             return
-        d = model.diag(t, y)
+        d = model.diag(t, y if isinstance(y, np.ndarray) else dv.to_host(y))
         row = head(t)
         for end in ("top", "bottom"):
             row.update(_end_summary(model, d, end, c_feed_co2))
@@ -211,7 +224,7 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
     integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **kw)
     try:
         res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
-                              stop_direction=direction)
+                              stop_direction=direction, device_result=True)
     except SolverError as e:
         if kw.get("conserve") != "project":
             raise
@@ -231,7 +244,7 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
             integ = TRBDF2(model, cfg, h_max=cfg.solver.h_max_fraction * dur, **kw)
             try:
                 res = integ.integrate(y0, dur, t_out, on_step=on_step, stop_fn=stop_fn, stop_value=stop_val,
-                                      stop_direction=direction)
+                                      stop_direction=direction, device_result=True)
             except SolverError as e2:
                 if level == 2:
                     raise
@@ -250,15 +263,16 @@ def run_step(state_in, cfg, grid, props, step: Dict, cycle_idx: int = 1, t_cycle
         if ring is not None:
             ring.flush()
     wall = time.time() - wall0
-    state_out = model.to_state(res.y, state_in, state_in.t + res.t, step["name"])
+    state_out = model.to_state(dv.to_host(res.y), state_in, state_in.t + res.t, step["name"])
     state_out.cycle = cycle_idx
+    state_out._y_device = res.y
     snaps = []
     for (ts, ys, _a) in res.snapshots:
         st = model.to_state(ys, state_in, state_in.t + ts, step["name"])
         st.cycle = cycle_idx
         snaps.append({"cycle": cycle_idx, "step": step["name"], "t_step": ts, "t_cycle": t_cycle0 + ts,
                       "t_total": state_in.t + ts, "state": st})
-    inv0, inv1 = model.inventories(y0), model.inventories(res.y)
+    inv0, inv1 = model.inventories_dev(y0), model.inventories_dev(res.y)
     T_jac = model.T_jacket(res.t) if model.jacket_on else cfg.ambient.T
     sr = StepResult(name=step["name"], type=step["type"], cycle=cycle_idx, t_start_cycle=t_cycle0,
                     duration_planned=dur, duration=res.t, stopped_early=res.stopped_early,

@@ -81,6 +81,15 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
                 const double* arc, const double* Ar, const double* drc, const double* feed_flux,
                 const double* N_nom, const double* cov, vtsa_handle** out);
 int vtsa_destroy(vtsa_handle* h);
+/* Re-initialise a handle for the next step of a cycle on the same grid (same Nr, Nth, Nz and
+ * wall; -2 otherwise): the descriptor and the nine arrays of vtsa_create are replaced, the
+ * preconditioner factorisation and low-rank correction are dropped, and every device buffer is
+ * kept.  The copies are queued on stream s behind the handle's earlier work and waited for.
+ * Replaces the per-step TVSAModel construction of vtsa_cycle.py:80-86 / core/residual.py:41-123
+ * when the steps of a cycle hand the bed on in device memory. */
+int vtsa_reset(vtsa_handle* h, const vtsa_model_desc* desc, const double* V, const double* Az,
+               const double* Ath, const double* arc, const double* Ar, const double* drc,
+               const double* feed_flux, const double* N_nom, const double* cov, vtsa_stream s);
 int vtsa_sizes(const vtsa_handle* h, int32_t* n_cells, int32_t* n_wall, int32_t* n_unknowns);
 
 /* f = dy/dt at (t, y); acc (36 accumulator rates, device) when non-NULL.
@@ -136,6 +145,13 @@ int vtsa_precond_apply(vtsa_handle* h, const double* r, double* out, vtsa_stream
 int vtsa_gmres(vtsa_handle* h, double t, const double* z, const double* f, const double* s, double dh,
                double zs_norm, const double* b, double* w, double rtol, double atol, int32_t restart,
                vtsa_gmres_info* info, vtsa_stream s_);
+
+/* In place on a state vector y (device, N values): every cell's energy e replaced by
+ * energy(temperature(e)), the round trip a hand-off through CycleState makes (it stores T, the
+ * next step re-forms e: physics/energy.py:46-62 as called by core/residual.py:175-196 to_state /
+ * pack).  Lets the next step start from the device vector with the same bits as from the host
+ * state.  Concentrations, loadings and wall temperatures are untouched. */
+int vtsa_state_roundtrip(vtsa_handle* h, double* y, vtsa_stream s);
 
 /* ---- Newton vector operations (core/solver.py:84-108,169-259); scalars returned on the host. */
 /* s = atol + rtol |z| */
