@@ -224,6 +224,12 @@ int vtsa_reset(vtsa_handle* H, const vtsa_model_desc* desc, const double* V, con
   h.prec_ready = 0;
   h.prec_dh = 0.0;
   h.defl_k = 0;
+  // options and sampling back to their defaults, as for a handle just created
+  const Handle fresh{};
+  h.gs_full = fresh.gs_full;
+  h.gs_forward = fresh.gs_forward;
+  h.gs_sweeps = fresh.gs_sweeps;
+  h.prof_every = 0;
   h.prof_seen = h.prof_launches = 0;
   h.prof_ms = h.prof_bytes = 0.0;
   return 0;

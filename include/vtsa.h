@@ -83,8 +83,8 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
 int vtsa_destroy(vtsa_handle* h);
 /* Re-initialise a handle for the next step of a cycle on the same grid (same Nr, Nth, Nz and
  * wall; -2 otherwise): the descriptor and the nine arrays of vtsa_create are replaced, the
- * preconditioner factorisation and low-rank correction are dropped, and every device buffer is
- * kept.  The copies are queued on stream s behind the handle's earlier work and waited for.
+ * preconditioner factorisation and low-rank correction are dropped, the engine options and the
+ * sampled timing return to their defaults, and every device buffer is kept.  The copies are queued on stream s behind the handle's earlier work and waited for.
  * Replaces the per-step TVSAModel construction of vtsa_cycle.py:80-86 / core/residual.py:41-123
  * when the steps of a cycle hand the bed on in device memory. */
 int vtsa_reset(vtsa_handle* h, const vtsa_model_desc* desc, const double* V, const double* Az,
