@@ -22,6 +22,9 @@ runs that are timed, each over the same K windows from the same bed:
 * ``plain_fp64``: the plain method with FP64 factors, and ``vs_plain_fp64`` = value / its rate;
 * ``fp32``: FP32 factors without the guess.
 
+Beside them, ``batch``: the batched-sensitivity workload (``--batch-cases`` drawn cases of uneven
+grids, one full cycle each, on 16 HIP streams at once), reported as its own aggregate rate.
+
 Per-field errors (c_CO2, c_H2O, c_N2, q_CO2, q_H2O, T, Tw) of the FP32 and learned runs against
 the plain FP64 run are taken at the end of the last cycle step all runs completed.  The roofline
 is that of the dominant kernel (the line Gauss-Seidel half-sweep), from every 8th of its launches
@@ -236,6 +239,31 @@ def kernel_probes(cfg, grid, props, state, n_rhs=200, n_jac=20):
     return out
 
 
+def batch_leg(n_cases, workers, seed=5):
+    """The batched-sensitivity workload on this GPU: n_cases drawn cases (grid Nr 4..12, Ntheta in
+    {1, 3..16}, Nz 20..100; T_heat, P_vacuum, feed RH per case), one full cycle each from the clean
+    bed with FP32 factors, on `workers` HIP streams at once; aggregate simulated seconds per wall
+    second and the per-case KPIs."""
+    from dac_vtsa_amd.batch import draw_cases, run_batch, run_case_to_css
+    cases = draw_cases(n_cases, seed=seed)
+    out = run_batch(cases, workers=workers, runner=run_case_to_css, max_cycles=1,
+                    base_overrides={"solver.conservation_tol": 1e-5},
+                    integrator_kw={"gs_sweeps": 2, "conserve": "project", "precision": "fp32"})
+    ok = [c for c in out["cases"] if c["ok"]]
+    cells = [c["cells"] for c in out["cases"]]
+    _log(f"batch: {n_cases} cases on {workers} streams, {out['sim_s']:.0f} s simulated in {out['wall_s']:.2f} s "
+         f"({out['value']:.0f} sim-s/wall-s), failed {out['failed']}")
+    kp = {k: [c["kpis"][k] for c in ok] for k in ("recovery_pct", "purity_dry_pct", "specific_total_kJ_per_kg")}
+    return {"workload": f"{n_cases} drawn cases ({min(cells)}-{max(cells)} cells), one full cycle each from the "
+                        f"clean bed, FP32 factors, {workers} HIP streams",
+            "value": out["value"], "unit": "sim-s/wall-s (all cases together)", "wall_s": out["wall_s"],
+            "sim_s": out["sim_s"], "failed": out["failed"],
+            "max_mass_rel_err": max(c["max_mass_rel_err"] for c in ok) if ok else None,
+            "max_energy_rel_err": max(c["max_energy_rel_err"] for c in ok) if ok else None,
+            "kpis_min_max": {k: [min(v), max(v)] for k, v in kp.items() if v},
+            "cases": [{"grid": c["grid"], "wall_s": c["wall_s"], **{k: c["kpis"][k] for k in kp}} for c in ok]}
+
+
 def copy_probe(mib=64, repeat=5):
     """Calibration of the HBM traffic counters: `repeat` device-to-device copies of `mib` MiB."""
     from dac_vtsa_amd import device as dv
@@ -348,6 +376,7 @@ def main():
                     help="line Gauss-Seidel colour order (forward: 0 1 0 1; symmetric: 0 1 0 1 0)")
     ap.add_argument("--krylov", type=int, default=60, help="GMRES restart length")
     ap.add_argument("--lowrank", type=int, default=8, help="rank of the value leg's preconditioner correction (0 off)")
+    ap.add_argument("--batch-cases", type=int, default=16, help="cases of the batch leg (0 skips it)")
     ap.add_argument("--cpu-budget-s", type=float, default=40.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
@@ -445,6 +474,8 @@ def main():
                                                      ("fp32" if nm == "fp32" else "value_run")] = \
                         field_errors(ends[nm][key - 1][2], ref[2], cfg)
     line.update(kernel_probes(cfg, grid, props, bed))
+    if args.batch_cases > 0:
+        line["batch"] = batch_leg(args.batch_cases, workers=min(16, args.batch_cases))
     if not args.no_cpu and "plain_fp64" in line:
         cb = cpu_baseline(args.cpu_budget_s, line["plain_fp64"]["solver"], line["plain_fp64"]["sim_s"])
         if cb is not None:

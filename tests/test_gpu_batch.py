@@ -29,3 +29,28 @@ def test_batch_matches_sequential_runs_exactly():
         assert a["max_mass_rel_err"] <= 1e-4 and a["max_energy_rel_err"] <= 1e-4
         assert set(a["kpis"]) >= {"recovery_pct", "purity_dry_pct"}
     print(f"sequential {seq['wall_s']:.2f} s, 4 streams {par['wall_s']:.2f} s")
+
+
+def test_uneven_cases_run_to_cyclic_steady_state():
+    """The batched-sensitivity workload reduced to three drawn cases (grids clipped to at most
+    6x5x30 cells to keep the test short), each run cycle after cycle from the clean bed to the CSS
+    test of validation/cyclic_steady_state.py:22-44 (css 1e-3, CO2 balance 5e-3, 2..12 cycles),
+    concurrently on three streams: every case reaches CSS with its recovery, purity and specific
+    energy (postprocessing/reporting.py:52-110) in range and every step's balances closed."""
+    from dac_vtsa_amd.batch import draw_cases, run_batch, run_case_to_css
+    cases = draw_cases(3, seed=23)
+    for c in cases:
+        o = c["overrides"]
+        o["grid.Nr"], o["grid.Nz"] = min(o["grid.Nr"], 6), min(o["grid.Nz"], 30)
+        o["grid.Ntheta"] = min(o["grid.Ntheta"], 5) if o["grid.Ntheta"] != 2 else 3
+    out = run_batch(cases, workers=3, runner=run_case_to_css, base_overrides={"solver.conservation_tol": 1e-5},
+                    integrator_kw={"gs_sweeps": 2, "conserve": "project", "precision": "fp32"})
+    assert not out["failed"], [c.get("error") for c in out["cases"]]
+    for c in out["cases"]:
+        assert c["css"], (c["id"], c["cycles"])
+        assert 2 <= c["cycles"] <= 12
+        k = c["kpis"]
+        assert 0.0 < k["recovery_pct"] <= 100.0 and 0.0 < k["purity_dry_pct"] <= 100.0
+        assert k["specific_total_kJ_per_kg"] > 0.0
+        assert c["max_mass_rel_err"] <= 1e-4 and c["max_energy_rel_err"] <= 1e-4
+    print([(c["grid"], c["cycles"], round(c["wall_s"], 1)) for c in out["cases"]], f"{out['wall_s']:.1f} s")
