@@ -246,32 +246,106 @@ __global__ void k_vec_op(int op, int N, int npos, double a, double b, double c, 
 }
 
 // ------------------------------------------------------------------ GMRES kernels
-// Initialisation: |b|, tolerances (scipy gmres with x0 = 0, M = I), V[0] unnormalised = b.
+// The low-rank correction's dots folded into a pass that already walks the vector x that the next
+// Arnoldi column is formed from (k_gm_init: x = b; k_gm_finish: x = w after orthogonalisation):
+// per thread cl[j] = sum Ud_j[i] x_i for j < dk, reduced to per-block partial rows 1 .. dk (stride
+// RMAXB; row 0 is the pass's own |x|^2), then, in the last block, c_j = row sums scaled by 1 / |x|
+// and d = Dd c, which the next column's k_defl_axpy uses (Q v = v + Ud d with v = x / |x|).
+__device__ inline void defl_block_partials(double (&cl)[DEFL_MAX], int dk, double* shd, double* partial) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o[DEFL_MAX];
+#pragma unroll
+    for (int j = 0; j < DEFL_MAX; ++j) o[j] = __shfl_xor(cl[j], off, 64);
+#pragma unroll
+    for (int j = 0; j < DEFL_MAX; ++j) cl[j] += o[j];
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int j = 0; j < DEFL_MAX; ++j) shd[wv * DEFL_MAX + j] = cl[j];
+  __syncthreads();
+  if ((int)threadIdx.x < dk) {
+    double bs = 0.0;
+    for (int q = 0; q < RBLK / 64; ++q) bs += shd[q * DEFL_MAX + threadIdx.x];
+    store_partial(partial + (size_t)(1 + threadIdx.x) * RMAXB + blockIdx.x, bs);
+  }
+}
+
+// Last block: csd[j] = total of partial row 1 + j (j < dk), one wavefront per row.
+__device__ inline void defl_last_rows(int dk, const double* __restrict__ partial, double* csd) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = wv; r < dk; r += RBLK / 64) {
+    const double v = wave_sum_strided(partial + (size_t)(1 + r) * RMAXB, gridDim.x);
+    if (lane == 0) csd[r] = v;
+  }
+}
+
+// d = Dd (csd * inv) by threads 0 .. dk-1 (after a barrier that orders csd and inv).
+__device__ inline void defl_form_d(int dk, const double* csd, double inv, const double* __restrict__ Dd,
+                                   double* __restrict__ dvec) {
+  const int r = threadIdx.x;
+  if (r >= dk) return;
+  double acc = 0.0;
+  for (int j = 0; j < dk; ++j) acc = fma(Dd[r * DEFL_MAX + j], csd[j] * inv, acc);
+  dvec[r] = acc;
+}
+
+// Initialisation: |b|, tolerances (scipy gmres with x0 = 0, M = I), V[0] unnormalised = b.  DEFL:
+// the correction's dots of b as above.
+template <bool DEFL>
 __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restrict__ b, double rtol, double atol,
                                                   GmresScalars* __restrict__ gs, double* __restrict__ partial,
-                                                  unsigned* counter, int* __restrict__ progress) {
+                                                  unsigned* counter, int* __restrict__ progress,
+                                                  const double* __restrict__ Ud, int dk,
+                                                  const double* __restrict__ Dd, double* __restrict__ dvec) {
   __shared__ double sh[RBLK];
+  __shared__ double shd[DEFL ? (RBLK / 64) * DEFL_MAX : 1];
+  __shared__ double csd[DEFL_MAX];
+  __shared__ double s_inv;
   __shared__ bool slast;
   double loc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) loc += b[i] * b[i];
+  double cl[DEFL_MAX];
+#pragma unroll
+  for (int j = 0; j < DEFL_MAX; ++j) cl[j] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const double bi = b[i];
+    loc += bi * bi;
+    if constexpr (DEFL) {
+#pragma unroll
+      for (int j = 0; j < DEFL_MAX; ++j)
+        if (j < dk) cl[j] += Ud[(size_t)j * N + i] * bi;
+    }
+  }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
-  double tot;
-  if (last_block_total(counter, &slast, partial, &tot)) {
-    double bn = sqrt(tot);
-    gs->bnorm = bn;
-    double ae = atol > rtol * bn ? atol : rtol * bn;
-    gs->atol_eff = ae;
-    gs->iters = 0;
-    gs->breakdown = 0;
-    gs->presid = 0.0;
-    gs->rnorm = bn;
-    gs->done = (bn == 0.0 || bn < ae) ? 2 : 0;   // 2: exit before any iteration
-    gs->ptol = bn > 0.0 ? bn * (1.0 < ae / bn ? 1.0 : ae / bn) : 0.0;
-    gs->S[0] = bn;
-    gs->inv_h1 = bn > 0.0 ? 1.0 / bn : 0.0;
-    // pacing only: the host reads no data on the strength of this word, so relaxed suffices
-    __hip_atomic_store(progress, gs->done ? -1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if constexpr (DEFL) defl_block_partials(cl, dk, shd, partial);
+  if (!ticket_last(counter, &slast)) return;
+  if constexpr (DEFL) defl_last_rows(dk, partial, csd);
+  if (threadIdx.x < 64) {
+    const double tot = wave_sum_strided(partial, gridDim.x);
+    if (threadIdx.x == 0) {
+      ticket_reset(counter);
+      double bn = sqrt(tot);
+      gs->bnorm = bn;
+      double ae = atol > rtol * bn ? atol : rtol * bn;
+      gs->atol_eff = ae;
+      gs->iters = 0;
+      gs->breakdown = 0;
+      gs->presid = 0.0;
+      gs->rnorm = bn;
+      gs->done = (bn == 0.0 || bn < ae) ? 2 : 0;   // 2: exit before any iteration
+      gs->ptol = bn > 0.0 ? bn * (1.0 < ae / bn ? 1.0 : ae / bn) : 0.0;
+      gs->S[0] = bn;
+      gs->inv_h1 = bn > 0.0 ? 1.0 / bn : 0.0;
+      s_inv = gs->inv_h1;
+      // pacing only: the host reads no data on the strength of this word, so relaxed suffices
+      __hip_atomic_store(progress, gs->done ? -1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+  if constexpr (DEFL) {
+    __syncthreads();
+    defl_form_d(dk, csd, s_inv, Dd, dvec);
   }
 }
 
@@ -398,12 +472,18 @@ __device__ inline void lartg(double f, double g, double& c, double& s, double& r
 }
 
 // Pass 3: w -= V h2; |w|; Hessenberg column update with Givens rotations; convergence test
-// (scipy gmres inner loop).
+// (scipy gmres inner loop).  DEFL: the correction's dots of w folded in (see k_gm_init).
+template <bool DEFL>
 __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double* __restrict__ Vb, size_t ld,
                                                     double* __restrict__ w, GmresScalars* __restrict__ gs,
                                                     double* __restrict__ partial, unsigned* counter,
-                                                    int* __restrict__ progress) {
+                                                    int* __restrict__ progress, const double* __restrict__ Ud,
+                                                    int dk, const double* __restrict__ Dd,
+                                                    double* __restrict__ dvec) {
   __shared__ double sh[RBLK / 64];
+  __shared__ double shd[DEFL ? (RBLK / 64) * DEFL_MAX : 1];
+  __shared__ double csd[DEFL_MAX];
+  __shared__ double s_inv;
   __shared__ double hk[GMW];
   __shared__ double hc[GMW], csl[GMW], snl[GMW];
   __shared__ bool slast;
@@ -412,14 +492,24 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
   if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
   __syncthreads();
   double loc = 0.0;
+  double cl[DEFL_MAX];
+#pragma unroll
+  for (int j = 0; j < DEFL_MAX; ++j) cl[j] = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const double wi = project_out(Vb, ld, hk, K, i, w[i]);
     w[i] = wi;
     loc += wi * wi;
+    if constexpr (DEFL) {
+#pragma unroll
+      for (int j = 0; j < DEFL_MAX; ++j)
+        if (j < dk) cl[j] += Ud[(size_t)j * N + i] * wi;
+    }
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
+  if constexpr (DEFL) defl_block_partials(cl, dk, shd, partial);
   if (!ticket_last(counter, &slast)) return;
+  if constexpr (DEFL) defl_last_rows(dk, partial, csd);
   // last block: |w|, then the Givens sweep of column col on LDS copies
   const int t = threadIdx.x;
   double tot = 0.0;
@@ -435,6 +525,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
     hc[col + 1] = brk ? 0.0 : h1;
     gs->Hr[col + 1][col] = hc[col + 1];
     gs->inv_h1 = brk ? 0.0 : 1.0 / h1;
+    s_inv = gs->inv_h1;
     for (int k = 0; k < col; ++k) {
       double c = csl[k], s = snl[k];
       double n0 = hc[k], n1 = hc[k + 1];
@@ -460,6 +551,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
   }
   __syncthreads();
   for (int k = t; k <= col + 1; k += blockDim.x) gs->H[k][col] = hc[k];
+  if constexpr (DEFL) defl_form_d(dk, csd, s_inv, Dd, dvec);
 }
 
 // Back substitution of the least-squares problem (scipy's pseudo-solve of the triangular system).
@@ -733,14 +825,16 @@ __global__ void __launch_bounds__(RBLK) k_defl_dots(int N, const double* in, con
   }
 }
 
-// out = v + Ud d with v = in * scale (out may alias in).
-__global__ void k_defl_axpy(int N, int k, const double* in, const double* __restrict__ in_scale,
+// out = v + Ud d with v = in * scale, v also stored to `store` when given (out and store may alias
+// in).
+__global__ void k_defl_axpy(int N, int k, const double* in, const double* __restrict__ in_scale, double* store,
                             const double* __restrict__ Ud, const double* __restrict__ d, double* out,
                             const GmresScalars* __restrict__ gs, int respect_done) {
   if (respect_done && gs->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double v = in[i] * (in_scale ? *in_scale : 1.0);
+  if (store) store[i] = v;
   for (int j = 0; j < k; ++j) v = fma(Ud[(size_t)j * N + i], d[j], v);
   out[i] = v;
 }
@@ -811,8 +905,8 @@ void defl_apply_launch(Handle& h, const double* in, const double* in_scale, doub
   }
   const double* v = store ? store : in;
   const double* vs = store ? nullptr : in_scale;
-  hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(h.N)), dim3(256), 0, st, h.N, h.defl_k, v, vs, h.Ud, d, out, h.gs,
-                     respect_done);
+  hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(h.N)), dim3(256), 0, st, h.N, h.defl_k, v, vs, (double*)nullptr, h.Ud, d,
+                     out, h.gs, respect_done);
 }
 
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
@@ -837,8 +931,17 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
                        &h.gs->zs_rms);
   else
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, &h.gs->zs_rms, zs_norm);
-  hipLaunchKernelGGL(k_gm_init, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial, h.red.counter + 5,
-                     h.progress_dev);
+  // with the low-rank correction, the dots of each next Arnoldi vector ride on the pass that forms
+  // it (k_gm_init for b, k_gm_finish for w), so a column's correction is one axpy
+  const int dk = h.defl_k;
+  double* dvec = h.defl_small + DEFL_MAX * DEFL_MAX;
+  if (dk > 0)
+    hipLaunchKernelGGL(k_gm_init<true>, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial,
+                       h.red.counter + 5, h.progress_dev, (const double*)h.Ud, dk, (const double*)h.defl_small, dvec);
+  else
+    hipLaunchKernelGGL(k_gm_init<false>, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial,
+                       h.red.counter + 5, h.progress_dev, (const double*)nullptr, 0, (const double*)nullptr,
+                       (double*)nullptr);
   if (check(hipGetLastError(), "k_gm_init")) return -1;
   for (int col = 0; col < restart; ++col) {
     if (col >= GM_LOOKAHEAD) {
@@ -858,8 +961,9 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
     // u = P^-1 v_col with v_col = w_prev / |w_prev| normalised in place (u = P^-1 Q v_col with the
     // low-rank correction)
     const double* vin = col == 0 ? b : Vb + (size_t)col * ld;
-    if (h.defl_k > 0) {
-      defl_apply_launch(h, vin, &h.gs->inv_h1, Vb + (size_t)col * ld, h.defl_tmp, 1, st);
+    if (dk > 0) {
+      hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(N)), dim3(256), 0, st, N, dk, vin, &h.gs->inv_h1,
+                         Vb + (size_t)col * ld, (const double*)h.Ud, (const double*)dvec, h.defl_tmp, h.gs, 1);
       ApplyArgs a{h.defl_tmp, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 1};
       apply_launch(h, a, st);
     } else {
@@ -872,8 +976,14 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
                        h.red.counter + 5);
     hipLaunchKernelGGL(k_gm_orth, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
                        h.red.counter + 5);
-    hipLaunchKernelGGL(k_gm_finish, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
-                       h.red.counter + 5, h.progress_dev);
+    if (dk > 0)
+      hipLaunchKernelGGL(k_gm_finish<true>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
+                         h.red.counter + 5, h.progress_dev, (const double*)h.Ud, dk, (const double*)h.defl_small,
+                         dvec);
+    else
+      hipLaunchKernelGGL(k_gm_finish<false>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
+                         h.red.counter + 5, h.progress_dev, (const double*)nullptr, 0, (const double*)nullptr,
+                         (double*)nullptr);
   }
   hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, h.gs);
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);
