@@ -133,7 +133,7 @@ def balance_errors(results, species):
 def solver_totals(results):
     keys = ("steps", "rejected", "newton_fail", "jacobians", "lu", "gmres_iters", "nfev", "cons_projected",
             "balance_unclosed", "unclosed_rejected", "guess_used", "guess_kept_predictor", "lowrank_built",
-            "lowrank_dropped")
+            "lowrank_dropped", "gmres_iters_corrected", "gmres_iters_uncorrected")
     return {k: sum(int(sr.solver.get(k) or 0) for sr in results) for k in keys}
 
 

@@ -139,6 +139,8 @@ class DeviceOps:
         self.lowrank_held = False     # dropped until the next factorisation
         self.lowrank_built = 0
         self.lowrank_dropped = 0
+        self.gmres_iters_corrected = 0   # GMRES iterations of cycles run with / without the correction
+        self.gmres_iters_plain = 0
 
     def st(self):
         return dv.stream()
@@ -188,6 +190,10 @@ class DeviceOps:
         _abi.call("vtsa_gmres", self.h, float(t), dv.ptr(z), dv.ptr(f), dv.ptr(s), float(dh), float(zs_norm),
                   dv.ptr(b), dv.ptr(w), float(rtol), float(atol), int(restart), C.byref(info), self.st())
         self.m.nfev += info.iters + 1
+        if self.lowrank_on:
+            self.gmres_iters_corrected += info.iters
+        else:
+            self.gmres_iters_plain += info.iters
         if self.lowrank and not (self.lowrank_on or self.lowrank_held) and info.iters >= self.lowrank_min_iters:
             self.lowrank_from_last_cycle(self.lowrank)
         return info
@@ -951,6 +957,8 @@ class TRBDF2:
         if ops.lowrank:
             self.stats["lowrank_built"] = ops.lowrank_built
             self.stats["lowrank_dropped"] = ops.lowrank_dropped
+            self.stats["gmres_iters_corrected"] = ops.gmres_iters_corrected
+            self.stats["gmres_iters_uncorrected"] = ops.gmres_iters_plain
         res.nfev = m.nfev
         res.wall_time = time.time() - wall0
         return res
