@@ -27,7 +27,8 @@ grids, one full cycle each, on 16 HIP streams at once), reported as its own aggr
 
 Per-field errors (c_CO2, c_H2O, c_N2, q_CO2, q_H2O, T, Tw) of the FP32 and learned runs against
 the plain FP64 run are taken at the end of the last cycle step all runs completed.  The roofline
-is that of the dominant kernel (the line Gauss-Seidel half-sweep), from every 8th of its launches
+is that of the dominant kernel (the line Gauss-Seidel half-sweep), from every 7th of its launches
+(7 is prime to the 4 half-sweeps of an apply, so every position in the apply is sampled alike)
 in the value's timed region bracketed by HIP events on its stream.  The CPU baseline is the
 NumPy/SciPy oracle (the upstream algorithms) on one host thread: its unit costs measured on this
 workload within a bounded budget, extrapolated with the operation counts of the plain FP64 run
@@ -54,10 +55,10 @@ FP64_PEAK_TFLOPS = 78.6
 GRID = (20, 32, 100)
 CYCLE_S = 30860.0
 WINDOW_S = CYCLE_S / 20.0
-# HBM bytes per launch of the dominant kernel (k_gs_half<float, 2>, the value form's half-sweep) on
-# this workload from two rocprofv3 counter passes (FETCH_SIZE x 1.9993 + WRITE_SIZE x 1.0, the
+# HBM bytes per launch of the dominant kernel (k_gs_half<float, 4>, the value form's half-sweep) on
+# this workload from two rocprofv3 counter passes (FETCH_SIZE x 1.9992 + WRITE_SIZE x 1.0, the
 # factors of a 64 MiB device copy), over the first 1,543 s of the cycle.
-TRAFFIC_BYTES_PER_LAUNCH = 57753507
+TRAFFIC_BYTES_PER_LAUNCH = 56435371
 # FP64 operations per launch on the kernel-probe workload (the heating step's model on the clean
 # bed at t = 0) from the rocprofv3 counters SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave
 # instructions x 64 lanes, an FMA counted as 2).
@@ -424,7 +425,7 @@ def main():
     main_kw = {**base_kw, "precision": "fp32", "lowrank": args.lowrank}
     if net is not None:
         main_kw["guess"] = net
-    residual.reset_kernel_profile(8)
+    residual.reset_kernel_profile(7)
     main, ends_main = timed_leg("learned+fp32" if net is not None or args.lowrank else "fp32", cfg, grid, props,
                                 bed, args.steps, main_kw)
     prof = dict(residual.KERNEL_PROFILE)
