@@ -56,9 +56,9 @@ GRID = (20, 32, 100)
 CYCLE_S = 30860.0
 WINDOW_S = CYCLE_S / 20.0
 # HBM bytes per launch of the dominant kernel (k_gs_half<float, 4>, the value form's half-sweep) on
-# this workload from two rocprofv3 counter passes (FETCH_SIZE x 1.9992 + WRITE_SIZE x 1.0, the
+# this workload from two rocprofv3 counter passes (FETCH_SIZE x 1.9993 + WRITE_SIZE x 1.0, the
 # factors of a 64 MiB device copy), over the first 1,543 s of the cycle.
-TRAFFIC_BYTES_PER_LAUNCH = 56435371
+TRAFFIC_BYTES_PER_LAUNCH = 56659456
 # FP64 operations per launch on the kernel-probe workload (the heating step's model on the clean
 # bed at t = 0) from the rocprofv3 counters SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave
 # instructions x 64 lanes, an FMA counted as 2).

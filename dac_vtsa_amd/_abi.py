@@ -16,7 +16,7 @@ I32 = C.c_int32
 P = C.c_void_p
 # vtsa_cons_shift mode bits (include/vtsa.h)
 SHIFT_TRUNCATE, SHIFT_AT_ZERO, SHIFT_NO_REPORT = 1, 2, 4
-OPT_GS_WHOLE_LINES, OPT_GS_FORWARD = 1, 2          # vtsa_set_option
+OPT_GS_WHOLE_LINES, OPT_GS_FORWARD, OPT_GS_TWIST = 1, 2, 3   # vtsa_set_option
 
 
 class ModelDesc(C.Structure):

@@ -40,6 +40,7 @@ static void free_handle(Handle* h) {
     if (b) (void)hipFree(b);
   if (h->F) (void)hipFree(h->F);
   if (h->P) (void)hipFree(h->P);
+  if (h->Fr) (void)hipFree(h->Fr);
   if (h->gs) (void)hipFree(h->gs);
   if (h->red.counter) (void)hipFree(h->red.counter);
   if (h->defl_counter) (void)hipFree(h->defl_counter);
@@ -133,6 +134,7 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   rc |= dalloc(&h.Tw, nwall * 3);
   rc |= check(hipMalloc(&h.F, sizeof(double) * (size_t)h.n * 180), "hipMalloc");
   rc |= check(hipMalloc(&h.P, sizeof(double) * gs_pairs_count(h.ncol, h.desc.Nz)), "hipMalloc");
+  rc |= check(hipMalloc(&h.Fr, sizeof(double) * (size_t)h.n * 180), "hipMalloc");
   rc |= dalloc(&h.Fw, nwall);
   rc |= dalloc(&h.s0, h.N);
   rc |= dalloc(&h.gsb, h.N);
@@ -228,6 +230,8 @@ int vtsa_reset(vtsa_handle* H, const vtsa_model_desc* desc, const double* V, con
   const Handle fresh{};
   h.gs_full = fresh.gs_full;
   h.gs_forward = fresh.gs_forward;
+  h.gs_twist = fresh.gs_twist;
+  h.gs_tw_m = -1;
   h.gs_sweeps = fresh.gs_sweeps;
   h.prof_every = 0;
   h.prof_seen = h.prof_launches = 0;
@@ -604,6 +608,10 @@ int vtsa_set_option(vtsa_handle* H, int32_t option, int32_t value) {
     case VTSA_OPT_GS_FORWARD:
       if (value < 0 || value > 1) { set_error("vtsa_set_option: sweep order must be 0 or 1"); return -2; }
       H->h.gs_forward = value;
+      return 0;
+    case VTSA_OPT_GS_TWIST:
+      if (value < 0 || value > 1) { set_error("vtsa_set_option: twisted solve must be 0 or 1"); return -2; }
+      H->h.gs_twist = value;
       return 0;
     default: set_error("vtsa_set_option: unknown option"); return -2;
   }

@@ -59,6 +59,7 @@ struct Handle {
   // preconditioner factors (FP64 or FP32), scale at factorisation
   void* F = nullptr;          // [n][5][36] blocks (L2, L1, Dinv, U1, U2) in the chosen precision
   void* P = nullptr;          // line Gauss-Seidel pair form of F, gs_pairs_count values
+  void* Fr = nullptr;         // as F for each z-line factored from its last cell upwards (twisted solve)
   double* Fw = nullptr;       // [nw] wall diagonal of M
   double* Tw = nullptr;       // [nw][3] wall-line tridiagonal LU: lo, 1 / den, c (line Gauss-Seidel)
   double* s0 = nullptr;       // [N] scale of the factorisation
@@ -70,6 +71,8 @@ struct Handle {
   int gs_sweeps = 1;
   int gs_full = 1;                   // LDS staging of a half-sweep: 0 segments, 1 whole line
   int gs_forward = 0;                // line Gauss-Seidel order: 0 symmetric sweeps, 1 forward sweeps only
+  int gs_twist = 1;                  // twisted pair solve when the line allows it (option, read at set-up)
+  int gs_tw_m = -1;                  // twist pair of the last set-up's pair form, -1: classic pair form
   int lds_max = 65536;               // LDS bytes one block may use on this device
   int lds_per_cu = 65536;            // LDS bytes of one CU
   int n_cu = 256;                    // compute units of the device

@@ -28,16 +28,27 @@ constexpr int AC = 4;            // apply gather/scatter: elements per lane whos
 // with partial pivoting (the first row of largest |entry| wins), rows in parallel through LDS.
 // Block products accumulate over m in order in T, as C = A B and C -= A B of 6x6 matrices.
 // F blocks per cell, rows of 6: 0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2.
+// Blocks from nbt on factor the same lines from their last cell upwards (the line with its cells
+// in reverse order: the blocks of slots z+1, z+2 become the lower ones), cells 0 .. kend_rev - 1
+// of the reversed line, into Fr at the reversed cell index.
 constexpr int FSTR = 258;   // LDS doubles per column: Dinv/U1/U2 ring of two cells + pivot scratch
 
 template <typename T>
 __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int kind,
                                                     const double* __restrict__ Jc, const double* __restrict__ s0,
-                                                    double dh, T* __restrict__ F) {
+                                                    double dh, T* __restrict__ F, T* __restrict__ Fr, int nbt,
+                                                    int kend_rev) {
   __shared__ double lds[PCOLS * FSTR];
   const int lane = threadIdx.x;
   const int grp = lane / 6, r = lane - grp * 6;
-  const int col0 = blockIdx.x * PCOLS;
+  const bool rev = (int)blockIdx.x >= nbt;
+  const int col0 = ((int)blockIdx.x - (rev ? nbt : 0)) * PCOLS;
+  const int kend = rev ? kend_rev : Nz;
+  T* const Fo = rev ? Fr : F;
+  // slots and cell offsets of the blocks k-2, k-1, k+1, k+2 along the factorisation order
+  const int sm2 = rev ? S_ZP2 : S_ZM2, sm1 = rev ? S_ZP1 : S_ZM1, sp1 = rev ? S_ZM1 : S_ZP1,
+            sp2 = rev ? S_ZM2 : S_ZP2;
+  const int dd = rev ? -1 : 1;
   const int nc = min(PCOLS, ncol - col0);
   const bool active = grp < nc;
   const int col = col0 + (active ? grp : 0);
@@ -49,8 +60,8 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
   double* sw = Lc + 222;     // [24] row swap
   double* prow = Lc + 246;   // [12] normalised pivot row of D and of its inverse
   const bool zl = kind == VTSA_PREC_ZLINE;
-  for (int k = 0; k < Nz; ++k) {
-    const int X = col * Nz + k;
+  for (int k = 0; k < kend; ++k) {
+    const int X = col * Nz + (rev ? Nz - 1 - k : k);
     const double sr = s0[r * n + X];
     // row r of the block of M~ = S0^-1 (I - dh J) S0 between cell X and cell X + dk
     // loaded first (jr: row r of J's block, sc: the scales of the block's columns), formed after
@@ -75,17 +86,17 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
     for (int c = 0; c < 6; ++c) { l2[c] = T(0); l1[c] = T(0); u1[c] = T(0); u2[c] = T(0); }
     // the cell's five block rows are requested together, one round trip per cell
     double jr[5][6], sc[5][6];
-    if (zl && k >= 2) mload(S_ZM2, -2, jr[0], sc[0]);
-    if (zl && k >= 1) mload(S_ZM1, -1, jr[1], sc[1]);
+    if (zl && k >= 2) mload(sm2, -2 * dd, jr[0], sc[0]);
+    if (zl && k >= 1) mload(sm1, -dd, jr[1], sc[1]);
     mload(S_SELF, 0, jr[2], sc[2]);
-    if (zl && k + 1 < Nz) mload(S_ZP1, 1, jr[3], sc[3]);
-    if (zl && k + 2 < Nz) mload(S_ZP2, 2, jr[4], sc[4]);
+    if (zl && k + 1 < Nz) mload(sp1, dd, jr[3], sc[3]);
+    if (zl && k + 2 < Nz) mload(sp2, 2 * dd, jr[4], sc[4]);
     T a2[6], a1[6];
-    if (zl && k >= 2) mform(S_ZM2, jr[0], sc[0], a2);
-    if (zl && k >= 1) mform(S_ZM1, jr[1], sc[1], a1);
+    if (zl && k >= 2) mform(sm2, jr[0], sc[0], a2);
+    if (zl && k >= 1) mform(sm1, jr[1], sc[1], a1);
     mform(S_SELF, jr[2], sc[2], d);
-    if (zl && k + 1 < Nz) mform(S_ZP1, jr[3], sc[3], u1);
-    if (zl && k + 2 < Nz) mform(S_ZP2, jr[4], sc[4], u2);
+    if (zl && k + 1 < Nz) mform(sp1, jr[3], sc[3], u1);
+    if (zl && k + 2 < Nz) mform(sp2, jr[4], sc[4], u2);
     if (zl && k >= 2) {   // L2 = A_{k,k-2} Dinv_{k-2}
       const T* a = a2;
 #pragma unroll
@@ -147,7 +158,7 @@ __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int
     }
     // U2 = A_{k,k+2} as loaded
     lds_barrier();   // every lane has read ring slot q (cell k-2) before cell k overwrites it
-    T* FX = F + (size_t)X * 180 + r * 6;
+    T* FX = Fo + (size_t)(col * Nz + k) * 180 + r * 6;
     if (active && zl) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
@@ -460,8 +471,13 @@ constexpr int PGH = 288;
 constexpr int PO_G = 0, PO_H = 144, PO_C = 288, PO_D = 324, PO_U = 396;
 
 __host__ __device__ constexpr int npairs(int Nz) { return (Nz + 1) / 2; }
-// xs and zs, [2 npairs + 1][6] each (the last cell a spare slot)
-__host__ __device__ constexpr size_t pair_vec_doubles(int Nz) { return (size_t)(2 * npairs(Nz) + 1) * 6; }
+// xs and zs, [2 npairs + 3][6] each: the last cell a spare slot, then a junk pair for the twisted
+// recurrences' padding steps; at least TW_AREA doubles, since zs holds the twist record until the
+// forward recurrences end
+constexpr int TW_SIG = 0, TW_SIGR = 144, TW_GINV = 288, TW_XCH = 432, TW_AREA = 456;
+__host__ __device__ constexpr size_t pair_vec_doubles(int Nz) {
+  return (size_t)(2 * npairs(Nz) + 3) * 6 > (size_t)TW_AREA ? (size_t)(2 * npairs(Nz) + 3) * 6 : (size_t)TW_AREA;
+}
 template <typename T>
 __host__ __device__ constexpr size_t pair_buf_bytes(int Nz) {   // whole copies of GSW x 16 bytes
   return ((size_t)npairs(Nz) * PP * sizeof(T) + GSW * 16 - 1) / (GSW * 16) * (GSW * 16);
@@ -490,18 +506,25 @@ __host__ __device__ constexpr size_t gs_g_lds_bytes(int Nz) {
 size_t gs_pairs_count(int ncol, int Nz) { return (size_t)ncol * npairs(Nz) * PP; }
 
 // Pair form of the factors, one thread per (line, pair, row r of 6).  Products accumulate over m
-// in order in FP64 and are stored in T.
+// in order in FP64 and are stored in T.  Twisted form (tw_m >= 0): pairs K < tw_m from F, pairs
+// K > tw_m from the reversed factor Fr (pair NP - 1 - K of the reversed line, its two cells in
+// reverse order), pair tw_m written by k_gs_twist.
 template <typename T>
-__global__ void __launch_bounds__(256) k_gs_pairs(int Nz, int ncol, const T* __restrict__ F, T* __restrict__ P) {
+__global__ void __launch_bounds__(256) k_gs_pairs(int Nz, int ncol, const T* __restrict__ F, T* __restrict__ P,
+                                                  const T* __restrict__ Fr, int tw_m) {
   const int NP = npairs(Nz);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncol * NP * 6) return;
   const int r = t % 6, pr = t / 6;
   const int L = pr / NP, K = pr - L * NP;
-  const int k0 = 2 * K, k1 = 2 * K + 1;
+  if (tw_m >= 0 && K == tw_m) return;
+  const bool bot = tw_m >= 0 && K > tw_m;
+  const T* src = bot ? Fr : F;
+  const int Ks = bot ? NP - 1 - K : K;
+  const int k0 = 2 * Ks, k1 = 2 * Ks + 1;
   // entry (i, c) of block b (0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2) of cell k; zero for the phantom
   auto fb = [&](int k, int b, int i, int c) -> double {
-    return k < Nz ? (double)F[(size_t)(L * Nz + k) * 180 + b * 36 + i * 6 + c] : 0.0;
+    return k < Nz ? (double)src[(size_t)(L * Nz + k) * 180 + b * 36 + i * 6 + c] : 0.0;
   };
   double u1[36], u2[36];   // ^U1_{2K+1}, ^U2_{2K+1}
   for (int i = 0; i < 6; ++i)
@@ -682,6 +705,238 @@ __device__ inline void pair_recurrence(const T* Pl, int NP, int off, const doubl
     }
     if (left == 1) out[K * 12 + rho] = bsub_pair(ca.b, s, ca.g);
   }
+}
+
+// ------------------------------------------------------------------ twisted pair solve
+// A line of an even number of cells is also factored from its last cell upwards (k_prec_factor,
+// reversed blocks), and the two factorisations meet in the twist pair m = (NP - 1) / 2.  At pair
+// level (pair K = cells 2K, 2K+1) the line matrix is block tridiagonal; with the top factor's
+//   Sigma_K = [L2_{2K} L1_{2K}; 0 L2_{2K+1}]          (L, pair K on pair K-1)
+//   Ups_K   = [U2_{2K} 0; U1_{2K+1} U2_{2K+1}]         (U, pair K on pair K+1)
+// and ~Sigma, ~Ups the same of the reversed line, the forward pair recurrence runs from the top
+// over pairs 0 .. m-1 and from the bottom over pairs NP-1 .. m+1 (in reversed cell order), then
+//   x_m = Gamma^-1 (b_m - Sigma_m z_{m-1} - P ~Sigma_{m'} ~z_{m'-1}),
+//   Gamma = A_m - Sigma_m Ups_{m-1} - P ~Sigma_{m'} ~Ups_{m'-1} P,
+// with A_m pair m's diagonal block, m' = NP - 1 - m and P the swap of a pair's two cells, and the
+// backward recurrences run from x_m outwards, top over pairs m-1 .. 0 and bottom over m+1 .. NP-1.
+// Each half is the classic LU solve of its end of the line, so the pairs other than m keep their
+// pair form (from Fr, in reversed cell order, below the twist); record m holds Sigma_m, ~Sigma_{m'}
+// and Gamma^-1 (TW_SIG, TW_SIGR, TW_GINV).  Half as many dependent steps as the classic solve.
+
+// element of a line vector holding component rho of a pair taken in reversed cell order
+__device__ inline int tw_perm(int rho) { return rho < 6 ? rho + 6 : rho - 6; }
+
+// Record m of every line: one 64-lane block per line, lanes 0..11 owning rows; Gamma and its
+// Gauss-Jordan inverse (partial pivoting, the first row of largest |entry| wins) in FP64.
+template <typename T>
+__global__ void __launch_bounds__(64) k_gs_twist(int n, int Nz, const double* __restrict__ Jc,
+                                                 const double* __restrict__ s0, double dh, const T* __restrict__ F,
+                                                 const T* __restrict__ Fr, int m, T* __restrict__ P) {
+  __shared__ double Sg[144], Up[144], Sr[144], Ur[144], Am[144];
+  __shared__ double colp[12], sw[48], prow[24];
+  const int L = blockIdx.x, lane = threadIdx.x;
+  const int NP = npairs(Nz), mr = NP - 1 - m;
+  auto fe = [&](const T* src, int k, int b, int i, int c) -> double {
+    return (double)src[(size_t)(L * Nz + k) * 180 + b * 36 + i * 6 + c];
+  };
+  for (int e = lane; e < 144; e += 64) {
+    const int row = e / 12, col = e - row * 12;
+    const int ci = row / 6, i = row - ci * 6, cj = col / 6, c = col - cj * 6;
+    auto sig = [&](const T* src, int K) -> double {   // Sigma_K (rows: cells 2K, 2K+1; columns: 2K-2, 2K-1)
+      if (ci == 0) return fe(src, 2 * K, cj == 0 ? 0 : 1, i, c);
+      return cj == 0 ? 0.0 : fe(src, 2 * K + 1, 0, i, c);
+    };
+    auto ups = [&](const T* src, int K) -> double {   // Ups_K (rows: cells 2K, 2K+1; columns: 2K+2, 2K+3)
+      if (ci == 0) return cj == 0 ? fe(src, 2 * K, 4, i, c) : 0.0;
+      return fe(src, 2 * K + 1, cj == 0 ? 3 : 4, i, c);
+    };
+    Sg[e] = sig(F, m);
+    Up[e] = ups(F, m - 1);
+    Sr[e] = sig(Fr, mr);
+    Ur[e] = ups(Fr, mr - 1);
+    // A_m = S0^-1 (I - dh J) S0 between cells 2m + ci and 2m + cj
+    const int X = L * Nz + 2 * m + ci, dk = cj - ci;
+    const int slot = dk == 0 ? S_SELF : dk > 0 ? S_ZP1 : S_ZM1;
+    const double v = (dk == 0 && i == c ? 1.0 : 0.0) - dh * Jc[((size_t)X * NSLOT + slot) * 36 + i * 6 + c];
+    Am[e] = v * (s0[c * n + X + dk] / s0[i * n + X]);
+  }
+  __syncthreads();
+  const int rho = lane < 12 ? lane : 0, pr = tw_perm(rho);
+  double g[12], iv[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    const int pc = tw_perm(c);
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < 12; ++q) {
+      a += Sg[rho * 12 + q] * Up[q * 12 + c];
+      b += Sr[pr * 12 + q] * Ur[q * 12 + pc];
+    }
+    g[c] = (Am[rho * 12 + c] - a) - b;
+    iv[c] = rho == c ? 1.0 : 0.0;
+  }
+  const bool own = lane < 12;
+#pragma unroll
+  for (int p = 0; p < 12; ++p) {
+    if (own) colp[lane] = fabs(g[p]);
+    __syncthreads();
+    int piv = p;
+    double best = colp[p];
+    for (int rr = p + 1; rr < 12; ++rr) {
+      const double v = colp[rr];
+      if (v > best) { best = v; piv = rr; }
+    }
+    if (own && (lane == p || lane == piv)) {
+      double* dst = sw + (lane == p ? 0 : 24);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) { dst[c] = g[c]; dst[12 + c] = iv[c]; }
+    }
+    __syncthreads();
+    if (own && piv != p && (lane == p || lane == piv)) {
+      const double* src = sw + (lane == p ? 24 : 0);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) { g[c] = src[c]; iv[c] = src[12 + c]; }
+    }
+    if (lane == p) {
+      const double inv = 1.0 / g[p];
+#pragma unroll
+      for (int c = 0; c < 12; ++c) { g[c] *= inv; iv[c] *= inv; prow[c] = g[c]; prow[12 + c] = iv[c]; }
+    }
+    __syncthreads();
+    if (own && lane != p) {
+      const double f = g[p];
+      if (f != 0.0) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) { g[c] -= f * prow[c]; iv[c] -= f * prow[12 + c]; }
+      }
+    }
+    __syncthreads();
+  }
+  T* R = P + ((size_t)L * NP + m) * PP;
+  for (int e = lane; e < 144; e += 64) {
+    R[TW_SIG + e] = (T)Sg[e];
+    R[TW_SIGR + e] = (T)Sr[e];
+  }
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < 12; ++c) R[TW_GINV + lane * 12 + c] = (T)iv[c];
+  }
+}
+
+// The twisted recurrences, forward (top pairs 0 .. m-1, bottom NP-1 .. m+1, from zero) or backward
+// (top m-1 .. 0, bottom m+1 .. NP-1, from s = x_m): lanes 0..11 the top rows, lanes 16..27 the
+// bottom rows (reversed cell order), each group in its own 16-lane DPP row, so one instruction
+// stream advances both.  The bottom group has NP-1-m steps, the top m, one fewer when NP is even:
+// the top's extra step re-reads its last pair and stores into the junk pair after the spare slot.
+// Loads run ahead as in pair_recurrence; the caller synchronises.
+template <typename T, bool FORWARD, int STRIDE>
+__device__ inline double pair_recurrence_tw(const T* Pl, int NP, int m, int off, const double* rhs, double* out,
+                                            double s) {
+  const int lane = threadIdx.x;
+  const bool bot = lane >= 16;
+  const int rho = lane & 15;
+  const int S = NP - 1 - m, cnt = bot ? S : m;
+  const int K0 = FORWARD ? (bot ? NP - 1 : 0) : (bot ? m + 1 : m - 1);
+  const int dK = (FORWARD != bot) ? 1 : -1;
+  const int el = bot ? tw_perm(rho) : rho;
+  const int junk = NP * 12 + 6 + rho;
+  auto pair_of = [&](int i) { return K0 + dK * (i < cnt ? i : cnt - 1); };
+  auto oidx = [&](int i) { return i < cnt ? (K0 + dK * i) * 12 + el : junk; };
+  struct Row { double g[12]; double b; };
+  if constexpr (sizeof(T) == sizeof(double)) {
+    auto ld = [&](int i, Row& w) {
+      const int K = pair_of(i);
+      const T* R = Pl + (size_t)K * STRIDE + off + rho * 12;
+#pragma unroll
+      for (int c = 0; c < 12; ++c) w.g[c] = (double)R[c];
+      w.b = rhs[K * 12 + el];
+    };
+    Row wa, wb;
+    ld(0, wa);
+    int i = 0;
+    for (; i + 2 <= S; i += 2) {
+      ld(i + 1, wb);
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(wa.b, s, wa.g);
+      out[oidx(i)] = s;
+      __builtin_amdgcn_sched_barrier(0);
+      ld(i + 2 < S ? i + 2 : i + 1, wa);
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(wb.b, s, wb.g);
+      out[oidx(i + 1)] = s;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (i < S) {
+      s = bsub_pair(wa.b, s, wa.g);
+      out[oidx(i)] = s;
+    }
+  } else {
+    // narrow rows: loaded two steps ahead, widened one step ahead (as in pair_recurrence)
+    struct Raw { T g[12]; double b; };
+    auto issue = [&](int i, Raw& w) {
+      const int K = pair_of(i < S ? i : S - 1);
+      const T* R = Pl + (size_t)K * STRIDE + off + rho * 12;
+#pragma unroll
+      for (int c = 0; c < 12; ++c) w.g[c] = R[c];
+      w.b = rhs[K * 12 + el];
+    };
+    auto widen = [&](const Raw& r, Row& w) {
+#pragma unroll
+      for (int c = 0; c < 12; ++c) w.g[c] = (double)r.g[c];
+      w.b = r.b;
+    };
+    Raw ra, rb;
+    Row ca, cb;
+    issue(0, ra);
+    widen(ra, ca);
+    issue(1, rb);
+    int i = 0;
+    for (; i + 2 <= S; i += 2) {
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(ca.b, s, ca.g);
+      out[oidx(i)] = s;
+      __builtin_amdgcn_sched_barrier(0);
+      widen(rb, cb);
+      issue(i + 2, ra);
+      __builtin_amdgcn_sched_barrier(0);
+      s = bsub_pair(cb.b, s, cb.g);
+      out[oidx(i + 1)] = s;
+      __builtin_amdgcn_sched_barrier(0);
+      widen(ra, ca);
+      issue(i + 3, rb);
+    }
+    if (i < S) {
+      s = bsub_pair(ca.b, s, ca.g);
+      out[oidx(i)] = s;
+    }
+  }
+  return s;
+}
+
+// The twist pair, right after the forward recurrences, by the same lanes (zs holds record m):
+// top rows y = b_m - Sigma_m z_{m-1}, bottom rows -~Sigma ~z_{m'-1} exchanged through zs in
+// pair m's cell order, then x_m = Gamma^-1 (y + exchanged) into xs by the top rows.  Every lane
+// stores its y (the top rows after the exchange slots) and every lane runs the second product:
+// lanes that exchange through LDS must not sit in branches of their own, whose order the
+// compiler may choose.
+__device__ inline void twist_pair(int m, double* xs, double* zs) {
+  const int lane = threadIdx.x;
+  const bool bot = lane >= 16;
+  const int rho = lane & 15;
+  wave_lds_sync();
+  const double v = bot ? xs[(m + 1) * 12 + tw_perm(rho)] : xs[(m - 1) * 12 + rho];
+  const double b = bot ? 0.0 : xs[m * 12 + rho];
+  double g[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) g[c] = zs[(bot ? TW_SIGR : TW_SIG) + rho * 12 + c];
+  const double y = bsub_pair(b, v, g);
+  zs[TW_XCH + (bot ? tw_perm(rho) : 12 + rho)] = y;
+  wave_lds_sync();
+  __builtin_amdgcn_wave_barrier();
+  const double r = y + zs[TW_XCH + rho];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) g[c] = zs[TW_GINV + rho * 12 + c];
+  const double x = bsub_pair(0.0, -r, g);
+  if (!bot) xs[m * 12 + rho] = x;
 }
 
 // FP32 recurrence matrices widened to FP64 in LDS, so that the pair recurrences run on FP64
@@ -949,7 +1204,9 @@ __global__ void __launch_bounds__(256) k_gs_offline(int n, const double* __restr
 // MODE 3: as MODE 2 with G staged at the start and H copied over it after the forward
 //         recurrence (1/3 of the LDS), the copies in flight during the two passes between the
 //         recurrences.
-// Both solve the same line systems; the roundings differ.
+// MODE 4: as MODE 3 with FP32 matrices widened to FP64 in LDS.
+// All solve the same line systems; the roundings differ.  MODE 1 .. 4 with tw_m >= 0: the twisted
+// pair form (twist pair tw_m), both halves of the line at once.
 // The apply's outputs, on a line's last visit: out = U / post for the line's unknowns, and the
 // rms of out / red_s over all unknowns (the sum of squares per line into partial[L]; the last
 // line to finish sums the lines' partials in line order).  Returns this thread's sum of squares.
@@ -1003,7 +1260,7 @@ template <typename T, int MODE>
 __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     k_gs_half(int n, int Nr, int Nth, int Nz, int ncol, int nw, const T* __restrict__ Fac, GsArgs g, int cbeg,
               const GmresScalars* __restrict__ gs, int respect_done, ApplyArgs a, int first_visit, int final_visit,
-              int nlines, int N, double* __restrict__ partial, unsigned* counter) {
+              int nlines, int N, double* __restrict__ partial, unsigned* counter, int tw_m) {
   extern __shared__ __align__(16) double lds[];
   __shared__ double sh_norm[(MODE == 0 ? PW : GSW) / 64];
   __shared__ bool s_last;
@@ -1029,6 +1286,10 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
     else if (MODE == 3) stage_pairs_part<T, PO_G, 144>(Fac, L, NP, pb);
     else if (MODE == 0) zline_stage_first<T>(Fac, X0, Nz, fb0);
     if constexpr (MODE == 4) stage_narrow<PO_G>(Fac, L, NP, Nz, pbw);
+    // twisted: the twist record, one value per thread, loaded now and put into zs after the gather
+    const bool twisted = MODE >= 1 && tw_m >= 0;
+    T twv = T(0);
+    if (twisted && lane < TW_XCH) twv = Fac[((size_t)L * NP + tw_m) * PP + lane];
     const int jm = j == 0 ? Nth - 1 : j - 1, jp = j + 1 == Nth ? 0 : j + 1;
     // neighbour z-lines' first cells for the off-line slots r-1, r+1, theta-1, theta+1
     const int nb[4] = {i > 0 ? ((i - 1) * Nth + j) * Nz : -1, i + 1 < Nr ? ((i + 1) * Nth + j) * Nz : -1,
@@ -1122,19 +1383,30 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       // widened to FP64); the recurrences read G and H from LDS
       const T* pd = MODE == 1 ? pb : Fac + (size_t)L * NP * PP;
       stage_wait();
+      if (twisted && lane < TW_XCH) zs[lane] = (double)twv;
       if constexpr (MODE == 4) widen_in_place(NP, Nz, pbw);
-      // b'_{2K+1} = b_{2K+1} - L1_{2K+1} b_{2K}
+      // b'_{2K+1} = b_{2K+1} - L1_{2K+1} b_{2K} (twisted, below the twist pair: the cells in
+      // reverse order; none for the twist pair)
       for (int e = lane; e < NP * 6; e += NT) {
         const int K = e / 6, r = e - K * 6;
+        if (twisted && K == tw_m) continue;
+        const bool rv = twisted && K > tw_m;
+        const int e0 = K * 12 + (rv ? 6 : 0), e1 = K * 12 + (rv ? 0 : 6);
         const T* C = pd + (size_t)K * PP + PO_C + r * 6;
-        double v = xs[K * 12 + 6 + r];
+        double v = xs[e1 + r];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v = fma(-(double)C[c], xs[K * 12 + c], v);
-        xs[K * 12 + 6 + r] = v;
+        for (int c = 0; c < 6; ++c) v = fma(-(double)C[c], xs[e0 + c], v);
+        xs[e1 + r] = v;
       }
       lds_barrier();
       constexpr int SG = MODE == 1 ? PP : MODE == 2 ? PGH : 144;   // LDS stride of a pair's matrices
-      if (lane < 12) {
+      if (twisted) {
+        if (lane < 32 && (lane & 15) < 12) {
+          if constexpr (MODE == 4) pair_recurrence_tw<double, true, 144>(pbw, NP, tw_m, 0, xs, xs, 0.0);
+          else pair_recurrence_tw<T, true, SG>(pb, NP, tw_m, PO_G, xs, xs, 0.0);
+          twist_pair(tw_m, xs, zs);
+        }
+      } else if (lane < 12) {
         if constexpr (MODE == 4) pair_recurrence<double, true, 144>(pbw, NP, 0, xs, xs);
         else pair_recurrence<T, true, SG>(pb, NP, PO_G, xs, xs);
       }
@@ -1145,26 +1417,37 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       // h_k = Dinv_k z_k
       for (int e = lane; e < NP * 12; e += NT) {
         const int K = e / 12, rr = e - K * 12, hh = rr / 6;
+        if (twisted && K == tw_m) continue;
+        const int cell = twisted && K > tw_m ? 1 - hh : hh;
         const T* D = pd + (size_t)K * PP + PO_D + rr * 6;
         double v = 0.0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v = fma((double)D[c], xs[K * 12 + hh * 6 + c], v);
-        zs[e] = v;
+        for (int c = 0; c < 6; ++c) v = fma((double)D[c], xs[K * 12 + cell * 6 + c], v);
+        zs[K * 12 + cell * 6 + (rr - hh * 6)] = v;
       }
       lds_barrier();
       // z'_{2K} = h_{2K} - ^U1_{2K} h_{2K+1}
       for (int e = lane; e < NP * 6; e += NT) {
         const int K = e / 6, r = e - K * 6;
+        if (twisted && K == tw_m) continue;
+        const bool rv = twisted && K > tw_m;
+        const int e0 = K * 12 + (rv ? 6 : 0), e1 = K * 12 + (rv ? 0 : 6);
         const T* U = pd + (size_t)K * PP + PO_U + r * 6;
-        double v = zs[K * 12 + r];
+        double v = zs[e0 + r];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v = fma(-(double)U[c], zs[K * 12 + 6 + c], v);
-        zs[K * 12 + r] = v;
+        for (int c = 0; c < 6; ++c) v = fma(-(double)U[c], zs[e1 + c], v);
+        zs[e0 + r] = v;
       }
       if (MODE == 3 || MODE == 4) stage_wait();
       if constexpr (MODE == 4) widen_in_place(NP, Nz, pbw);
       lds_barrier();
-      if (lane < 12) {
+      if (twisted) {
+        if (lane < 32 && (lane & 15) < 12) {
+          const double xm = xs[tw_m * 12 + (lane >= 16 ? tw_perm(lane & 15) : lane)];
+          if constexpr (MODE == 4) pair_recurrence_tw<double, false, 144>(pbw, NP, tw_m, 0, zs, xs, xm);
+          else pair_recurrence_tw<T, false, SG>(pb, NP, tw_m, MODE == 3 ? 0 : PO_H, zs, xs, xm);
+        }
+      } else if (lane < 12) {
         if constexpr (MODE == 4) pair_recurrence<double, false, 144>(pbw, NP, 0, zs, xs);
         else pair_recurrence<T, false, SG>(pb, NP, MODE == 3 ? 0 : PO_H, zs, xs);
       }
@@ -1301,12 +1584,18 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
   (void)hipMemcpyAsync(h.s0, scale, sizeof(double) * h.N, hipMemcpyDeviceToDevice, s);
   const int fkind = kind == VTSA_PREC_BLOCK_JACOBI ? VTSA_PREC_BLOCK_JACOBI : VTSA_PREC_ZLINE;
   const int nb = (h.ncol + PCOLS - 1) / PCOLS;
+  // twisted pair form (line Gauss-Seidel, an even number of cells, at least three pairs): the
+  // reversed factor of each line down to the twist pair, in the same launch as the full top one
+  const int Nz = h.desc.Nz, NP = npairs(Nz);
+  const int tw_m = kind == VTSA_PREC_ZLINE_GS && h.gs_twist && Nz % 2 == 0 && NP >= 3 ? (NP - 1) / 2 : -1;
+  const int nbr = tw_m >= 0 ? nb : 0;
+  const int kend_rev = tw_m >= 0 ? 2 * (NP - 1 - tw_m) + 2 : 0;
   if (prec == VTSA_FP32)
-    hipLaunchKernelGGL(k_prec_factor<float>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, fkind, h.Jc, h.s0, dh,
-                       (float*)h.F);
+    hipLaunchKernelGGL(k_prec_factor<float>, dim3(nb + nbr), dim3(PW), 0, s, h.n, Nz, h.ncol, fkind, h.Jc, h.s0, dh,
+                       (float*)h.F, (float*)h.Fr, nb, kend_rev);
   else
-    hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb), dim3(PW), 0, s, h.n, h.desc.Nz, h.ncol, fkind, h.Jc, h.s0,
-                       dh, (double*)h.F);
+    hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb + nbr), dim3(PW), 0, s, h.n, Nz, h.ncol, fkind, h.Jc, h.s0, dh,
+                       (double*)h.F, (double*)h.Fr, nb, kend_rev);
   if (h.nw > 0) hipLaunchKernelGGL(k_prec_wall, dim3((h.nw + 255) / 256), dim3(256), 0, s, h.nw, h.Jw, dh, h.Fw);
   if (kind == VTSA_PREC_ZLINE_GS) {
     // the off-line blocks are stored in the precision of the factors (the FP32 form halves the
@@ -1318,14 +1607,23 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
     if (h.nw > 0)
       hipLaunchKernelGGL(k_gs_wall_factor, dim3((h.desc.Nth + 63) / 64), dim3(64), 0, s, h.desc.Nth, h.desc.Nz, h.n,
                          h.Jw, h.s0, dh, h.Tw);
-    const int nt = h.ncol * npairs(h.desc.Nz) * 6;
+    const int nt = h.ncol * NP * 6;
     if (prec == VTSA_FP32)
-      hipLaunchKernelGGL(k_gs_pairs<float>, dim3((nt + 255) / 256), dim3(256), 0, s, h.desc.Nz, h.ncol,
-                         (const float*)h.F, (float*)h.P);
+      hipLaunchKernelGGL(k_gs_pairs<float>, dim3((nt + 255) / 256), dim3(256), 0, s, Nz, h.ncol, (const float*)h.F,
+                         (float*)h.P, (const float*)h.Fr, tw_m);
     else
-      hipLaunchKernelGGL(k_gs_pairs<double>, dim3((nt + 255) / 256), dim3(256), 0, s, h.desc.Nz, h.ncol,
-                         (const double*)h.F, (double*)h.P);
+      hipLaunchKernelGGL(k_gs_pairs<double>, dim3((nt + 255) / 256), dim3(256), 0, s, Nz, h.ncol,
+                         (const double*)h.F, (double*)h.P, (const double*)h.Fr, tw_m);
+    if (tw_m >= 0) {
+      if (prec == VTSA_FP32)
+        hipLaunchKernelGGL(k_gs_twist<float>, dim3(h.ncol), dim3(64), 0, s, h.n, Nz, h.Jc, h.s0, dh,
+                           (const float*)h.F, (const float*)h.Fr, tw_m, (float*)h.P);
+      else
+        hipLaunchKernelGGL(k_gs_twist<double>, dim3(h.ncol), dim3(64), 0, s, h.n, Nz, h.Jc, h.s0, dh,
+                           (const double*)h.F, (const double*)h.Fr, tw_m, (double*)h.P);
+    }
   }
+  h.gs_tw_m = kind == VTSA_PREC_ZLINE_GS ? tw_m : -1;
   h.prec_kind = kind;
   h.prec_prec = prec;
   h.prec_dh = dh;
@@ -1502,23 +1800,23 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
       if constexpr (narrow)
         hipLaunchKernelGGL((k_gs_half<T, 4>), dim3(cnt), dim3(GSW), lds4, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                            h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
-                           partial, counter);
+                           partial, counter, h.gs_tw_m);
     } else if (mode == 1)
       hipLaunchKernelGGL((k_gs_half<T, 1>), dim3(cnt), dim3(GSW), lds1, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
-                         partial, counter);
+                         partial, counter, h.gs_tw_m);
     else if (mode == 2)
       hipLaunchKernelGGL((k_gs_half<T, 2>), dim3(cnt), dim3(GSW), lds2, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
-                         partial, counter);
+                         partial, counter, h.gs_tw_m);
     else if (mode == 3)
       hipLaunchKernelGGL((k_gs_half<T, 3>), dim3(cnt), dim3(GSW), lds3, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.P, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
-                         partial, counter);
+                         partial, counter, h.gs_tw_m);
     else
       hipLaunchKernelGGL((k_gs_half<T, 0>), dim3(cnt), dim3(PW), lds0, s, h.n, h.desc.Nr, h.desc.Nth, Nz, h.ncol,
                          h.nw, (const T*)h.F, g, h.gs_cstart[c], a.gs, a.respect_done, a, fv, fin, nlines, N,
-                         partial, counter);
+                         partial, counter, h.gs_tw_m);
     if (sample && e0 && e1) {
       (void)hipEventRecord(e1, s);
       h.prof_pending.push_back(e0);

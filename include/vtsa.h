@@ -274,8 +274,13 @@ int vtsa_time_fdjac(vtsa_handle* h, double t, const double* y, const double* f0,
  * Automatic tries 5 first for FP32 factors.  Levels 1-4 give bit-identical results, 5 with FP32
  * factors the same recurrences on a right-hand side summed in another order; level 0 solves the
  * same line systems cell by cell (other roundings).  VTSA_OPT_GS_FORWARD: 0 (default) symmetric
- * sweeps (colours 0 1 0 1 0 for two sweeps of two colours), 1 forward sweeps only (0 1 0 1). */
-enum { VTSA_OPT_GS_WHOLE_LINES = 1, VTSA_OPT_GS_FORWARD = 2 };
+ * sweeps (colours 0 1 0 1 0 for two sweeps of two colours), 1 forward sweeps only (0 1 0 1).
+ * VTSA_OPT_GS_TWIST: 1 (default) lines of an even number of cells (at least 6) are solved by a
+ * twisted factorisation, top half from the first cell down and bottom half from the last cell up,
+ * meeting in a middle pair (half as many dependent steps; other roundings), 0 the classic top-down
+ * factorisation; read by the next preconditioner set-up.  Staging levels 1-5 then run the twisted
+ * form, level 0 the classic cell factors, which every set-up keeps. */
+enum { VTSA_OPT_GS_WHOLE_LINES = 1, VTSA_OPT_GS_FORWARD = 2, VTSA_OPT_GS_TWIST = 3 };
 int vtsa_set_option(vtsa_handle* h, int32_t option, int32_t value);
 
 /* Sampled timing of the line Gauss-Seidel half-sweep kernel, the dominant kernel of a step (no

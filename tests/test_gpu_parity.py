@@ -281,7 +281,10 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     tol is the cell-by-cell solve's (segmented staging); tol_pair the whole-line solve's, which
     advances two cells per step with products of neighbouring cells' factors (L1 L1, L1 L2, ^U ^U):
     on the 4x4x20 lines (condition numbers to 3e11, so cond x eps ~ 3e-5) their rounding shows at
-    1.7e-7 of the solution, where the cell-by-cell order stays below 1e-9."""
+    1.7e-7 of the solution in the classic top-down pair form, where the cell-by-cell order stays
+    below 1e-9.  Lines of an even number of cells run the twisted pair form by default (each half
+    factored from its own end, half as many dependent steps, 1.2e-10 on the 4x4x20 lines); the
+    classic form is checked as well, and odd lines (never twisted) give the same bits either way."""
     from dac_vtsa_amd import _abi
     from dac_vtsa_amd import device as dv
     from dac_vtsa_amd.core.jacobian import FDJacobian
@@ -327,6 +330,17 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     assert np.max(np.abs(out - ref_f) / sc) / np.max(np.abs(ref_f) / sc) < tol_pair
     assert np.max(np.abs(out - outs[0]) / sc) > 0.0     # a different operator from the symmetric one
     _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_FORWARD, 0)
+    # the classic top-down pair form (read at set-up)
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_TWIST, 0)
+    ops.precond_setup(dh, s0, PRECONDITIONERS["zline_gs"], prec, sweeps)
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_WHOLE_LINES, 3)
+    classic = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
+    assert np.max(np.abs(classic - ref) / sc) / np.max(np.abs(ref) / sc) < tol_pair
+    if over == "odd":
+        assert np.array_equal(classic, outs[0])
+    else:
+        assert not np.array_equal(classic, outs[0])
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_TWIST, 1)
 
 
 @pytest.mark.parametrize("dh,rtol", [(0.05, 1e-2), (0.05, 1e-4), (0.3, 1e-2)])
