@@ -12,6 +12,9 @@ feed y_CO2 = 400 ppm, drawn per case from one seed.
 """
 from __future__ import annotations
 
+import argparse
+import json
+import sys
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -99,11 +102,12 @@ def run_case(case: Dict, mode: str = "cycle", state_kind: str = "small", base_ov
 
 
 def run_case_to_css(case: Dict, base_overrides: Optional[Dict] = None, integrator_kw: Optional[Dict] = None,
-                    state=None, max_cycles: Optional[int] = None) -> Dict:
+                    state=None, max_cycles: Optional[int] = None, log=None) -> Dict:
     """One case run cycle after cycle to cyclic steady state (vtsa_cycle.py:93-138, the CSS test of
     validation/cyclic_steady_state.py:22-44) on the calling thread's current stream, from
     ``state`` (default: the configuration's initial bed).  Returns per-cycle balances, the last
-    cycle's KPIs, whether CSS was reached, and the end state."""
+    cycle's KPIs, whether CSS was reached, and the end state.  ``log`` (optional) receives the step
+    runners' progress lines, prefixed with the case id."""
     from .balances import is_css
     from .config import load_config
     from .core.grid import Grid
@@ -116,7 +120,9 @@ def run_case_to_css(case: Dict, base_overrides: Optional[Dict] = None, integrato
     t0 = time.time()
     try:
         st0 = state if state is not None else initial_state(cfg, grid, get_backend(cfg))
-        records, st = run_to_css(cfg, st0, max_cycles=max_cycles, integrator_kw=dict(integrator_kw or {}))
+        case_log = (lambda msg: log(f"[case {case['id']}] {msg}")) if log else None
+        records, st = run_to_css(cfg, st0, log=case_log, max_cycles=max_cycles,
+                                 integrator_kw=dict(integrator_kw or {}))
         dv.synchronize()
     except Exception as e:                     # a failed case is reported, the batch goes on
         out.update(ok=False, error=f"{type(e).__name__}: {e}", wall_s=time.time() - t0, sim_s=0.0)
@@ -150,8 +156,9 @@ def run_batch(cases: List[Dict], workers: int = 16, log=None, runner=None, **kw)
         with dv.use_stream(s):
             r = runner(cases[i], **kw)
         if log:
+            cyc = f", {r['cycles']} cycles{' (CSS)' if r.get('css') else ''}" if "cycles" in r else ""
             log(f"case {r['id']} {r['grid']}: {'ok' if r['ok'] else r['error']}, {r['sim_s']:.0f} s simulated "
-                f"in {r['wall_s']:.2f} s")
+                f"in {r['wall_s']:.2f} s{cyc}")
         return i, r
 
     results = [None] * len(cases)
@@ -170,3 +177,63 @@ def run_batch(cases: List[Dict], workers: int = 16, log=None, runner=None, **kw)
     sim = sum(r["sim_s"] for r in results)
     return {"cases": results, "wall_s": wall, "sim_s": sim, "value": sim / wall,
             "failed": [r["id"] for r in results if not r["ok"]]}
+
+
+def main(argv=None) -> int:
+    """Command line: drawn cases on this GPU, one cycle each or each to cyclic steady state; one
+    progress line per finished step and per finished case, then one JSON line with the aggregate
+    rate, per-case grids, cycles, KPIs and balances."""
+    ap = argparse.ArgumentParser(prog="python -m dac_vtsa_amd.batch", description=main.__doc__)
+    ap.add_argument("--cases", type=int, default=64, help="number of drawn cases")
+    ap.add_argument("--seed", type=int, default=5, help="seed of the draw")
+    ap.add_argument("--workers", type=int, default=16, help="worker threads, one HIP stream each")
+    ap.add_argument("--css", action="store_true", help="run every case to cyclic steady state (else one cycle)")
+    ap.add_argument("--max-cycles", type=int, default=None, help="cycle limit per case (default: run.max_cycles)")
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32", help="preconditioner factors")
+    ap.add_argument("--lowrank", type=int, default=0, help="rank of the preconditioner correction (0 off)")
+    ap.add_argument("--json", default=None, help="also write the result line to this file")
+    args = ap.parse_args(argv)
+
+    def say(msg):
+        print(msg, flush=True)
+    cases = draw_cases(args.cases, seed=args.seed)
+    ikw = {"gs_sweeps": 2, "conserve": "project", "precision": args.precision}
+    if args.lowrank:
+        ikw["lowrank"] = args.lowrank
+    out = run_batch(cases, workers=args.workers, log=say, runner=lambda case, **kw: run_case_to_css(case, log=say, **kw),
+                    max_cycles=args.max_cycles if args.css else 1,
+                    base_overrides={"solver.conservation_tol": 1e-5}, integrator_kw=ikw)
+    keys = ("recovery_pct", "purity_dry_pct", "specific_total_kJ_per_kg")
+    rows = []
+    for c in out["cases"]:
+        row = {"id": c["id"], "grid": c["grid"], "cells": c["cells"], "ok": c["ok"], "wall_s": round(c["wall_s"], 2)}
+        if c["ok"]:
+            row.update(cycles=c["cycles"], css=c["css"], sim_s=c["sim_s"],
+                       max_mass_rel_err=c["max_mass_rel_err"], max_energy_rel_err=c["max_energy_rel_err"],
+                       **{k: c["kpis"][k] for k in keys},
+                       **{k: v for k, v in cases[c["id"]]["overrides"].items() if k.startswith(("operation.", "feed.RH"))})
+        else:
+            row["error"] = c["error"]
+        rows.append(row)
+    ok = [r for r in rows if r["ok"]]
+    line = {"workload": f"{args.cases} drawn cases (seed {args.seed}), "
+                        + ("each to cyclic steady state" if args.css else "one cycle each")
+                        + f" from the clean bed, {args.precision} factors"
+                        + (f", rank-{args.lowrank} correction" if args.lowrank else "") + f", {args.workers} HIP streams",
+            "value": out["value"], "unit": "sim-s/wall-s (all cases together)", "wall_s": out["wall_s"],
+            "sim_s": out["sim_s"], "failed": out["failed"],
+            "css_reached": sum(1 for r in ok if r.get("css")), "cycles_total": sum(r["cycles"] for r in ok),
+            "kpis_min_max": {k: [min(r[k] for r in ok), max(r[k] for r in ok)] for k in keys} if ok else {},
+            "max_mass_rel_err": max((r["max_mass_rel_err"] for r in ok), default=None),
+            "max_energy_rel_err": max((r["max_energy_rel_err"] for r in ok), default=None),
+            "cases": rows}
+    text = json.dumps(line)
+    if args.json:
+        with open(args.json, "w") as f:
+            f.write(text + "\n")
+    print(text, flush=True)
+    return 0 if not out["failed"] else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())
