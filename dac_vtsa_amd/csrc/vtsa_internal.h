@@ -99,7 +99,8 @@ struct Handle {
   // Ud [defl_k][N] orthonormal columns, Dd [defl_k][defl_k]; defl_k = 0: none
   int defl_k = 0;
   double* Ud = nullptr;             // [DEFL_MAX][N] (allocated on first use)
-  double* defl_small = nullptr;     // device: Dd [DEFL_MAX^2] | d [DEFL_MAX] | G [MAX_RESTART * DEFL_MAX]
+  double* defl_small = nullptr;     // device: Dd [DEFL_MAX^2] | d [DEFL_MAX] | G [MAX_RESTART * DEFL_MAX] |
+                                    //   uv [DEFL_MAX][MAX_RESTART + 1] (Ud_j . v_k of the current cycle)
   double* defl_partial = nullptr;   // [DEFL_MAX][RMAXB] reduction partials
   unsigned* defl_counter = nullptr; // [2]
   double* defl_tmp = nullptr;       // [N] the corrected Arnoldi vector

@@ -247,11 +247,17 @@ __global__ void k_vec_op(int op, int N, int npos, double a, double b, double c, 
 
 // ------------------------------------------------------------------ GMRES kernels
 // The low-rank correction's dots folded into a pass that already walks the vector x that the next
-// Arnoldi column is formed from (k_gm_init: x = b; k_gm_finish: x = w after orthogonalisation):
-// per thread cl[j] = sum Ud_j[i] x_i for j < dk, reduced to per-block partial rows 1 .. dk (stride
-// RMAXB; row 0 is the pass's own |x|^2), then, in the last block, c_j = row sums scaled by 1 / |x|
-// and d = Dd c, which the next column's k_defl_axpy uses (Q v = v + Ud d with v = x / |x|).
-__device__ inline void defl_block_partials(double (&cl)[DEFL_MAX], int dk, double* shd, double* partial) {
+// Arnoldi column is formed from (k_gm_init: x = b; k_gm_orth: x = w after orthogonalisation):
+// per thread cl[j] = sum Ud_j[i] x_i for j < dk, reduced to per-block partial rows row0 .. row0 +
+// dk - 1 (stride RMAXB; the rows before are the pass's own sums), then, in the last block, c_j =
+// row sums scaled by 1 / |x| and d = Dd c, which the next column's k_defl_axpy uses (Q v = v + Ud d
+// with v = x / |x|).  c_j = Ud_j . v is also kept per basis vector (uv[j][k], UVS per row), so
+// that the pass that ends an Arnoldi column can form the next vector's dots without a pass of its
+// own.
+constexpr int UVS = MAX_RESTART + 1;   // row stride of uv (Ud_j . v_k per basis vector k)
+
+__device__ inline void defl_block_partials(double (&cl)[DEFL_MAX], int dk, double* shd, double* partial,
+                                           int row0) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double o[DEFL_MAX];
@@ -268,7 +274,7 @@ __device__ inline void defl_block_partials(double (&cl)[DEFL_MAX], int dk, doubl
   if ((int)threadIdx.x < dk) {
     double bs = 0.0;
     for (int q = 0; q < RBLK / 64; ++q) bs += shd[q * DEFL_MAX + threadIdx.x];
-    store_partial(partial + (size_t)(1 + threadIdx.x) * RMAXB + blockIdx.x, bs);
+    store_partial(partial + (size_t)(row0 + threadIdx.x) * RMAXB + blockIdx.x, bs);
   }
 }
 
@@ -292,13 +298,14 @@ __device__ inline void defl_form_d(int dk, const double* csd, double inv, const 
 }
 
 // Initialisation: |b|, tolerances (scipy gmres with x0 = 0, M = I), V[0] unnormalised = b.  DEFL:
-// the correction's dots of b as above.
+// the correction's dots of b as above (uv[j][0] = Ud_j . v_0).
 template <bool DEFL>
 __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restrict__ b, double rtol, double atol,
                                                   GmresScalars* __restrict__ gs, double* __restrict__ partial,
                                                   unsigned* counter, int* __restrict__ progress,
                                                   const double* __restrict__ Ud, int dk,
-                                                  const double* __restrict__ Dd, double* __restrict__ dvec) {
+                                                  const double* __restrict__ Dd, double* __restrict__ dvec,
+                                                  double* __restrict__ uv) {
   __shared__ double sh[RBLK];
   __shared__ double shd[DEFL ? (RBLK / 64) * DEFL_MAX : 1];
   __shared__ double csd[DEFL_MAX];
@@ -319,7 +326,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restric
   }
   double bs = wave_block_sum(loc, sh);
   if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
-  if constexpr (DEFL) defl_block_partials(cl, dk, shd, partial);
+  if constexpr (DEFL) defl_block_partials(cl, dk, shd, partial, 1);
   if (!ticket_last(counter, &slast)) return;
   if constexpr (DEFL) defl_last_rows(dk, partial, csd);
   if (threadIdx.x < 64) {
@@ -346,13 +353,17 @@ __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restric
   if constexpr (DEFL) {
     __syncthreads();
     defl_form_d(dk, csd, s_inv, Dd, dvec);
+    if ((int)threadIdx.x < dk) uv[threadIdx.x * UVS] = csd[threadIdx.x] * s_inv;
   }
 }
 
-// Classical Gram-Schmidt with one reorthogonalisation (CGS2), three passes per Arnoldi step.
-// Every pass reads the K = col + 1 basis vectors once per element; per-vector block sums go
-// through wavefront butterflies (no per-thread arrays indexed by K), and the last block sums the
-// per-block partials with one wavefront per basis vector.
+// Classical Gram-Schmidt with one reorthogonalisation (CGS2) in three passes per Arnoldi step, two
+// of them grid reductions: k_gm_dots (h = V^T w, |w|^2), k_gm_orth (w' = w - V h, h2 = V^T w',
+// |w'|^2, and in its last block the end of the column), k_gm_project (w'' = w' - V h2, no
+// reduction).  The norm of w'' comes from |w''|^2 = |w'|^2 - |h2|^2 (V orthonormal), so no pass
+// reduces over w''.  Every pass reads the K = col + 1 basis vectors once per element; per-vector
+// block sums go through wavefront butterflies (no per-thread arrays indexed by K), and the last
+// block sums the per-block partials with one wavefront per row.
 constexpr int GMW = MAX_RESTART + 2;   // partial rows per reduction (K dots + |w|^2)
 constexpr int GMC = 8;                 // basis vectors whose loads are in flight together
 
@@ -432,35 +443,6 @@ __global__ void __launch_bounds__(RBLK) k_gm_dots(int N, int col, const double* 
   }
 }
 
-// Pass 2: w -= V h_tmp; h2 = V . w (reorthogonalisation); H[:, col] += h2.
-__global__ void __launch_bounds__(RBLK) k_gm_orth(int N, int col, const double* __restrict__ Vb, size_t ld,
-                                                  double* __restrict__ w, GmresScalars* __restrict__ gs,
-                                                  double* __restrict__ partial, unsigned* counter) {
-  __shared__ double sh[(RBLK / 64) * GMW];
-  __shared__ double hk[GMW];
-  __shared__ double tot[GMW];
-  __shared__ bool slast;
-  if (gs->done) return;
-  const int K = col + 1;
-  const int wv = threadIdx.x >> 6;
-  if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
-  __syncthreads();
-  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-  for (int i = i0; i < N; i += stride) w[i] = project_out(Vb, ld, hk, K, i, w[i]);
-  block_dots(Vb, ld, w, K, N, i0, stride, sh + wv * GMW);
-  __syncthreads();
-  if ((int)threadIdx.x < K) {
-    double s = 0.0;
-    for (int q = 0; q < RBLK / 64; ++q) s += sh[q * GMW + threadIdx.x];
-    store_partial(partial + threadIdx.x * RMAXB + blockIdx.x, s);
-  }
-  if (last_block_totals(counter, &slast, partial, K, tot) && (int)threadIdx.x < K) {
-    const int k = threadIdx.x;
-    gs->h_tmp[k] = tot[k];
-    gs->H[k][col] += tot[k];
-  }
-}
-
 // LAPACK-style Givens rotation: [c s; -s c] [f; g] = [r; 0]
 __device__ inline void lartg(double f, double g, double& c, double& s, double& r) {
   if (g == 0.0) { c = 1.0; s = 0.0; r = f; return; }
@@ -471,57 +453,83 @@ __device__ inline void lartg(double f, double g, double& c, double& s, double& r
   s = g / r;
 }
 
-// Pass 3: w -= V h2; |w|; Hessenberg column update with Givens rotations; convergence test
-// (scipy gmres inner loop).  DEFL: the correction's dots of w folded in (see k_gm_init).
+// Pass 2: w' = w - V h_tmp (stored over w); h2 = V . w' and |w'|^2; DEFL: Ud . w' folded in.
+// Its last block then ends the column (scipy gmres inner loop): H[:, col] = h + h2 (kept raw in
+// Hr), |w''| = sqrt(|w'|^2 - sum_k h2_k^2) with w'' = w' - V h2 (k_gm_project forms it), the
+// Givens sweep of the column, the convergence test and the host-visible progress; DEFL: the next
+// vector's dots Ud . w'' = Ud . w' - sum_k (Ud . v_k) h2_k, scaled by 1 / |w''|, into uv[., col + 1]
+// and d = Dd c.  Everything the tail reads from gs is loaded at kernel start with the
+// coefficients, so that the serial tail waits for no load.
 template <bool DEFL>
-__global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double* __restrict__ Vb, size_t ld,
-                                                    double* __restrict__ w, GmresScalars* __restrict__ gs,
-                                                    double* __restrict__ partial, unsigned* counter,
-                                                    int* __restrict__ progress, const double* __restrict__ Ud,
-                                                    int dk, const double* __restrict__ Dd,
-                                                    double* __restrict__ dvec) {
-  __shared__ double sh[RBLK / 64];
+__global__ void __launch_bounds__(RBLK) k_gm_orth(int N, int col, const double* __restrict__ Vb, size_t ld,
+                                                  double* __restrict__ w, GmresScalars* __restrict__ gs,
+                                                  double* __restrict__ partial, unsigned* counter,
+                                                  int* __restrict__ progress, const double* __restrict__ Ud,
+                                                  int dk, const double* __restrict__ Dd, double* __restrict__ dvec,
+                                                  double* __restrict__ uv) {
+  __shared__ double sh[(RBLK / 64) * GMW];
   __shared__ double shd[DEFL ? (RBLK / 64) * DEFL_MAX : 1];
-  __shared__ double csd[DEFL_MAX];
-  __shared__ double s_inv;
   __shared__ double hk[GMW];
+  __shared__ double tot[GMW + DEFL_MAX];
   __shared__ double hc[GMW], csl[GMW], snl[GMW];
+  __shared__ double csd[DEFL_MAX];
+  __shared__ double s_h0sq, s_S, s_ptol, s_inv;
   __shared__ bool slast;
   if (gs->done) return;
   const int K = col + 1;
-  if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
+  const int t = threadIdx.x, wv = t >> 6;
+  if (t < K) {
+    hk[t] = gs->h_tmp[t];
+    hc[t] = gs->H[t][col];
+  }
+  if (t < col) {
+    csl[t] = gs->cs[t];
+    snl[t] = gs->sn[t];
+  }
+  if (t == 0) {
+    s_h0sq = gs->h0sq;
+    s_S = gs->S[col];
+    s_ptol = gs->ptol;
+  }
   __syncthreads();
-  double loc = 0.0;
+  const int i0 = blockIdx.x * blockDim.x + t, stride = gridDim.x * blockDim.x;
   double cl[DEFL_MAX];
 #pragma unroll
   for (int j = 0; j < DEFL_MAX; ++j) cl[j] = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+  for (int i = i0; i < N; i += stride) {
     const double wi = project_out(Vb, ld, hk, K, i, w[i]);
     w[i] = wi;
-    loc += wi * wi;
     if constexpr (DEFL) {
 #pragma unroll
       for (int j = 0; j < DEFL_MAX; ++j)
         if (j < dk) cl[j] += Ud[(size_t)j * N + i] * wi;
     }
   }
-  double bs = wave_block_sum(loc, sh);
-  if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
-  if constexpr (DEFL) defl_block_partials(cl, dk, shd, partial);
-  if (!ticket_last(counter, &slast)) return;
-  if constexpr (DEFL) defl_last_rows(dk, partial, csd);
-  // last block: |w|, then the Givens sweep of column col on LDS copies
-  const int t = threadIdx.x;
-  double tot = 0.0;
-  if (t < 64) tot = wave_sum_strided(partial, gridDim.x);
-  for (int k = t; k <= col; k += blockDim.x) hc[k] = gs->Hr[k][col] = gs->H[k][col];
-  for (int k = t; k < col; k += blockDim.x) { csl[k] = gs->cs[k]; snl[k] = gs->sn[k]; }
+  // rows 0 .. K-1: V_k . w'; row K: |w'|^2 (w' is stored as column K); DEFL rows K+1 ..: Ud . w'
+  block_dots(Vb, ld, w, K + 1, N, i0, stride, sh + wv * GMW);
+  __syncthreads();
+  if (t <= K) {
+    double sm = 0.0;
+    for (int q = 0; q < RBLK / 64; ++q) sm += sh[q * GMW + t];
+    store_partial(partial + t * RMAXB + blockIdx.x, sm);
+  }
+  if constexpr (DEFL) defl_block_partials(cl, dk, shd, partial, K + 1);
+  if (!last_block_totals(counter, &slast, partial, K + 1 + (DEFL ? dk : 0), tot)) return;
+  // last block: H[:, col] = h + h2 (raw copy in Hr), h2 for k_gm_project
+  for (int k = t; k < K; k += blockDim.x) {
+    const double hn = hc[k] + tot[k];
+    gs->h_tmp[k] = tot[k];
+    gs->Hr[k][col] = hn;
+    hc[k] = hn;
+  }
   __syncthreads();
   if (t == 0) {
-    ticket_reset(counter);
-    double h1 = sqrt(tot);
-    double h0 = sqrt(gs->h0sq);
-    bool brk = h1 <= EPS64 * h0;
+    double s2 = 0.0;
+    for (int k = 0; k < K; ++k) s2 += tot[k] * tot[k];
+    const double h1sq = tot[K] - s2;
+    const double h1 = h1sq > 0.0 ? sqrt(h1sq) : 0.0;
+    const double h0 = sqrt(s_h0sq);
+    const bool brk = h1 <= EPS64 * h0;
     hc[col + 1] = brk ? 0.0 : h1;
     gs->Hr[col + 1][col] = hc[col + 1];
     gs->inv_h1 = brk ? 0.0 : 1.0 / h1;
@@ -538,20 +546,43 @@ __global__ void __launch_bounds__(RBLK) k_gm_finish(int N, int col, const double
     gs->sn[col] = s;
     hc[col] = mag;
     hc[col + 1] = 0.0;
-    double tmp = -s * gs->S[col];
-    gs->S[col] = c * gs->S[col];
+    double tmp = -s * s_S;
+    gs->S[col] = c * s_S;
     gs->S[col + 1] = tmp;
-    gs->presid = fabs(tmp);
+    const double presid = fabs(tmp);
+    gs->presid = presid;
     gs->iters = col + 1;
     if (brk) gs->breakdown = 1;
-    const bool fin = gs->presid <= gs->ptol || brk;
+    const bool fin = presid <= s_ptol || brk;
     if (fin) gs->done = 1;
     // host-visible progress: completed columns, negative once the cycle is done
     __hip_atomic_store(progress, fin ? -(col + 1) : (col + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   __syncthreads();
   for (int k = t; k <= col + 1; k += blockDim.x) gs->H[k][col] = hc[k];
-  if constexpr (DEFL) defl_form_d(dk, csd, s_inv, Dd, dvec);
+  if constexpr (DEFL) {
+    if (t < dk) {
+      double a = tot[K + 1 + t];
+      for (int k = 0; k < K; ++k) a -= uv[t * UVS + k] * tot[k];
+      csd[t] = a;
+    }
+    __syncthreads();
+    defl_form_d(dk, csd, s_inv, Dd, dvec);
+    if (t < dk && col + 1 < UVS) uv[t * UVS + col + 1] = csd[t] * s_inv;
+  }
+}
+
+// Pass 3: w'' = w' - V h2 (the reorthogonalised vector the next column starts from), elementwise;
+// nothing to do once the cycle is done.
+__global__ void __launch_bounds__(256) k_gm_project(int N, int col, const double* __restrict__ Vb, size_t ld,
+                                                    double* __restrict__ w, const GmresScalars* __restrict__ gs) {
+  __shared__ double hk[GMW];
+  if (gs->done) return;
+  const int K = col + 1;
+  if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) w[i] = project_out(Vb, ld, hk, K, i, w[i]);
 }
 
 // Back substitution of the least-squares problem (scipy's pseudo-solve of the triangular system).
@@ -864,7 +895,9 @@ int defl_set_impl(Handle& h, int k, int m, const double* G, const double* D, hip
   if (!h.Ud) {
     if (check(hipMalloc((void**)&h.Ud, sizeof(double) * (size_t)DEFL_MAX * h.N), "hipMalloc")) return -1;
     if (check(hipMalloc((void**)&h.defl_small, sizeof(double) * (DEFL_MAX * DEFL_MAX + DEFL_MAX +
-                                                                  MAX_RESTART * DEFL_MAX)), "hipMalloc")) return -1;
+                                                                  MAX_RESTART * DEFL_MAX + DEFL_MAX * UVS)),
+              "hipMalloc"))
+      return -1;
     if (check(hipMalloc((void**)&h.defl_partial, sizeof(double) * DEFL_MAX * RMAXB), "hipMalloc")) return -1;
     if (check(hipMalloc((void**)&h.defl_counter, sizeof(unsigned) * 2), "hipMalloc")) return -1;
     if (check(hipMemset(h.defl_counter, 0, sizeof(unsigned) * 2), "hipMemset")) return -1;
@@ -920,7 +953,7 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   double* U = h.vtmp2;
   const int g = rgrid(N);
   // The host queues at most GM_LOOKAHEAD Arnoldi columns beyond the last one the device reports
-  // complete (a coherent host word written by k_gm_init / k_gm_finish), and stops queueing once
+  // complete (a coherent host word written by k_gm_init / k_gm_orth), and stops queueing once
   // the device reports convergence; queued columns past convergence return at once.
   constexpr int GM_LOOKAHEAD = 2;
   constexpr int NOT_STARTED = -0x7fffffff;
@@ -932,16 +965,18 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   else
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, &h.gs->zs_rms, zs_norm);
   // with the low-rank correction, the dots of each next Arnoldi vector ride on the pass that forms
-  // it (k_gm_init for b, k_gm_finish for w), so a column's correction is one axpy
+  // it (k_gm_init for b, k_gm_orth for w), so a column's correction is one axpy
   const int dk = h.defl_k;
   double* dvec = h.defl_small + DEFL_MAX * DEFL_MAX;
+  double* uv = dk > 0 ? h.defl_small + DEFL_MAX * DEFL_MAX + DEFL_MAX + MAX_RESTART * DEFL_MAX : nullptr;
   if (dk > 0)
     hipLaunchKernelGGL(k_gm_init<true>, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial,
-                       h.red.counter + 5, h.progress_dev, (const double*)h.Ud, dk, (const double*)h.defl_small, dvec);
+                       h.red.counter + 5, h.progress_dev, (const double*)h.Ud, dk, (const double*)h.defl_small, dvec,
+                       uv);
   else
     hipLaunchKernelGGL(k_gm_init<false>, dim3(g), dim3(RBLK), 0, st, N, b, rtol, atol, h.gs, h.red.partial,
                        h.red.counter + 5, h.progress_dev, (const double*)nullptr, 0, (const double*)nullptr,
-                       (double*)nullptr);
+                       (double*)nullptr, (double*)nullptr);
   if (check(hipGetLastError(), "k_gm_init")) return -1;
   for (int col = 0; col < restart; ++col) {
     if (col >= GM_LOOKAHEAD) {
@@ -974,16 +1009,15 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
     launch_matvec_residual(h, t, z, U, s, f, dh, wcol, true, st);
     hipLaunchKernelGGL(k_gm_dots, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
                        h.red.counter + 5);
-    hipLaunchKernelGGL(k_gm_orth, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
-                       h.red.counter + 5);
     if (dk > 0)
-      hipLaunchKernelGGL(k_gm_finish<true>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
+      hipLaunchKernelGGL(k_gm_orth<true>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
                          h.red.counter + 5, h.progress_dev, (const double*)h.Ud, dk, (const double*)h.defl_small,
-                         dvec);
+                         dvec, uv);
     else
-      hipLaunchKernelGGL(k_gm_finish<false>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
+      hipLaunchKernelGGL(k_gm_orth<false>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
                          h.red.counter + 5, h.progress_dev, (const double*)nullptr, 0, (const double*)nullptr,
-                         (double*)nullptr);
+                         (double*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL(k_gm_project, dim3(eblk(N)), dim3(256), 0, st, N, col, Vb, ld, wcol, h.gs);
   }
   hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, h.gs);
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);

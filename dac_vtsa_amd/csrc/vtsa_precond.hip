@@ -1635,7 +1635,7 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
 bool zline_fits(int Nz) { return zline_lds_bytes<double>(Nz) <= 65536; }
 
 size_t partial_size(int ncol, int nw) {
-  size_t a = (size_t)RMAXB * (MAX_RESTART + 2);
+  size_t a = (size_t)RMAXB * (MAX_RESTART + 2 + DEFL_MAX);   // GMRES: K dots + |w|^2 + correction dots
   size_t pg = (size_t)ncol + nw + 1;   // apply grid: a block per column; line Gauss-Seidel: a partial per line
   if (pg < (size_t)RMAXB) pg = RMAXB;                            // line Gauss-Seidel final reduction
   size_t b = (size_t)RMAXB * 8 + pg;

@@ -791,8 +791,9 @@ def test_replayed_full_adsorption_step_fields_within_1e6(scheme):
     The upstream run solves its stages to newton_kappa 1e-4, the configuration's smallest; the
     oracle, which reproduces that run to 1e-9 at that setting (tests/test_oracle.py), shows that its
     own Newton error then reaches 2e-6 relative in the CO2 concentration of nearly depleted cells.
-    So the device run (stages to 1e-6) is compared with the oracle replaying the same steps with
-    stages to 1e-7, i.e. both with the same discrete solution: every field (c_k, q_j, e, T, Tw)
+    So the device run and the oracle replay the same steps with stages to 1e-7, i.e. both with the
+    same discrete solution (at 1e-6 the step's N2 balance still moves by factors of 3 with the
+    rounding of the Krylov solves; at 1e-7 it settles at 1e-7): every field (c_k, q_j, e, T, Tw)
     within 1e-6 relative at every snapshot (entries above 1e-3 of the field's maximum and above
     atol); against the upstream run itself, within 3e-6 (its Newton tolerance); and the species
     balances of the step no worse than the upstream run's."""
@@ -805,7 +806,7 @@ def test_replayed_full_adsorption_step_fields_within_1e6(scheme):
     y0 = m.pack(st)
     seq = z[f"{scheme}/t_acc"]
     t_out = np.linspace(0, 200.0, 12)
-    res = TRBDF2(m, cfg, h_max=4.0, h_sequence=seq, kappa=1e-6).integrate(y0, 200.0, t_out)
+    res = TRBDF2(m, cfg, h_max=4.0, h_sequence=seq, kappa=1e-7).integrate(y0, 200.0, t_out)
     assert res.n_steps == len(seq)
     _, om = oracle_model(over, 0, st)
     oi = osv.OracleTRBDF2(om, cfg, h_max=4.0)
