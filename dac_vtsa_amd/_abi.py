@@ -79,7 +79,7 @@ _SIGS = {
     "vtsa_jacobian_get": [P, P, P, P, P],
     "vtsa_precond_setup": [P, D, P, I32, I32, I32, P],
     "vtsa_precond_apply": [P, P, P, P],
-    "vtsa_gmres": [P, D, P, P, P, D, D, P, P, D, D, I32, C.POINTER(GmresInfo), P],
+    "vtsa_gmres": [P, D, P, P, P, D, D, P, P, P, D, D, I32, C.POINTER(GmresInfo), P],
     "vtsa_scale": [P, P, D, P, P],
     "vtsa_rms": [P, P, P, I32, C.POINTER(D), P],
     "vtsa_rms_noalloc": [P, P, I32, P, P],

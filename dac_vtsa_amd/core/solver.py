@@ -183,12 +183,15 @@ class DeviceOps:
         self.m.nfev += 1
         return rn.value, pn.value, step.value
 
-    def gmres(self, t, z, f, s, dh, zs_norm, b, w, rtol, atol, restart):
-        """One GMRES cycle; zs_norm None: max(1, rms(z/s)) formed on the device."""
+    def gmres(self, t, z, f, s, dh, zs_norm, b, w, rtol, atol, restart, pw=None):
+        """One GMRES cycle; zs_norm None: max(1, rms(z/s)) formed on the device.  ``pw`` (optional
+        device vector): receives M^-1 (w*s), the step upstream forms with one more solve
+        (core/solver.py:240-241), from the apply of the cycle's closing residual."""
         zs_norm = -1.0 if zs_norm is None else zs_norm
         info = _abi.GmresInfo()
         _abi.call("vtsa_gmres", self.h, float(t), dv.ptr(z), dv.ptr(f), dv.ptr(s), float(dh), float(zs_norm),
-                  dv.ptr(b), dv.ptr(w), float(rtol), float(atol), int(restart), C.byref(info), self.st())
+                  dv.ptr(b), dv.ptr(w), dv.ptr(pw) if pw is not None else None, float(rtol), float(atol),
+                  int(restart), C.byref(info), self.st())
         self.m.nfev += info.iters + 1
         if self.lowrank_on:
             self.gmres_iters_corrected += info.iters
@@ -390,7 +393,7 @@ def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, kryl
     rn, pn = ops.newton_eval(t, z, dh, rhs_known, s, f, R)
     if not _finite(rn):
         return z0, False, 0, np.inf, None
-    b, w, dz, ws = dv.empty(N), dv.empty(N), dv.empty(N), dv.empty(N)
+    b, w, dz = dv.empty(N), dv.empty(N), dv.empty(N)
     zt, ft, Rt = dv.empty(N), dv.empty(N), dv.empty(N)
     step_prev = None
     polish = 0
@@ -404,7 +407,7 @@ def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, kryl
         ops.vec(8, b, x=R, y=s)
         zs_norm = None   # max(1, rms(z/s)), formed on the device by the GMRES cycle
         floor = (1e-4 if polish else 0.1 * kappa) * np.sqrt(N)
-        info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, (1e-3 if polish else eta), floor, krylov_dim)
+        info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, (1e-3 if polish else eta), floor, krylov_dim, pw=dz)
         if polish and stats is not None:
             stats["gmres_polish"] = stats.get("gmres_polish", 0) + info.iters
         elif counter is not None:
@@ -414,11 +417,10 @@ def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, kryl
             rn, pn = ops.newton_eval(t, z, dh, rhs_known, s, f, R)
             if stats is not None:
                 stats["jac_refresh_in_newton"] = stats.get("jac_refresh_in_newton", 0) + 1
-            info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, eta, floor, krylov_dim)
+            info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, eta, floor, krylov_dim, pw=dz)
             if counter is not None:
                 counter[0] += info.iters
-        ops.vec(9, ws, x=w, y=s)
-        ops.prec_apply(ws, dz)
+        # dz = M^-1 (w*s) (core/solver.py:240-241) came with the GMRES cycle
         lam = 1.0
         ls = 0
         while ls < 4:
@@ -436,11 +438,9 @@ def newton_krylov_dev(ops, t, z0, rhs_known, dh, lu, rtol, max_iter, kappa, kryl
                 # correction fails the sufficient-decrease test: the correction is dropped until
                 # the next factorisation and the direction is found again without it
                 ops.lowrank_drop()
-                info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, eta, floor, krylov_dim)
+                info = ops.gmres(t, z, f, s, dh, zs_norm, b, w, eta, floor, krylov_dim, pw=dz)
                 if counter is not None:
                     counter[0] += info.iters
-                ops.vec(9, ws, x=w, y=s)
-                ops.prec_apply(ws, dz)
                 continue
             lam *= 0.5
             ls += 1
@@ -702,14 +702,12 @@ class TRBDF2:
         if self.error_filter == "precond":
             return ops.error_norm(y, z2, y3, self.rtol)
         N = ops.N
-        d, s, b, w, ws, e = (dv.empty(N) for _ in range(6))
+        d, s, b, w, e = (dv.empty(N) for _ in range(5))
         ops.vec(7, d, x=y3, y=z2)                       # z2 - y3
         ops.scale(z2, self.rtol, s)
         ops.vec(8, b, x=d, y=s)                         # -(z2 - y3)/s
-        info = ops.gmres(t, z2, f2, s, dh, None, b, w, 1e-10, 0.0, 64)
+        info = ops.gmres(t, z2, f2, s, dh, None, b, w, 1e-10, 0.0, 64, pw=e)   # e = -M^-1 (z2 - y3)
         self.stats["exact_filter_unconverged"] = self.stats.get("exact_filter_unconverged", 0) + int(info.info != 0)
-        ops.vec(9, ws, x=w, y=s)
-        ops.prec_apply(ws, e)                           # -M^-1 (z2 - y3); the sign drops out of the rms
         eh, yh, zh = dv.to_host(e), dv.to_host(y), dv.to_host(z2)
         sc = self.atol + self.rtol * np.maximum(np.abs(yh), np.abs(zh))
         return float(np.sqrt(np.mean((eh / sc) ** 2)))

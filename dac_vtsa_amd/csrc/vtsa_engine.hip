@@ -366,14 +366,14 @@ int vtsa_precond_apply(vtsa_handle* H, const double* r, double* out, vtsa_stream
 }
 
 int vtsa_gmres(vtsa_handle* H, double t, const double* z, const double* f, const double* sc, double dh,
-               double zs_norm, const double* b, double* w, double rtol, double atol, int32_t restart,
+               double zs_norm, const double* b, double* w, double* pw, double rtol, double atol, int32_t restart,
                vtsa_gmres_info* info, vtsa_stream s) {
   if (!H || !z || !f || !sc || !b || !w || !info) { set_error("vtsa_gmres: null argument"); return -1; }
   Handle& h = H->h;
   if (!h.prec_ready) { set_error("vtsa_gmres: no factorisation"); return -2; }
   if (restart < 1 || restart > MAX_RESTART) { set_error("vtsa_gmres: restart out of range"); return -2; }
   if (grow_basis(h, restart)) return -3;
-  int rc = gmres_cycle(h, t, z, f, sc, dh, zs_norm, b, w, rtol, atol, restart, info, S(s));
+  int rc = gmres_cycle(h, t, z, f, sc, dh, zs_norm, b, w, pw, rtol, atol, restart, info, S(s));
   if (rc) return rc;
   return finish(S(s));
 }

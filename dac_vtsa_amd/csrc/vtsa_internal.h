@@ -145,8 +145,8 @@ int defl_set_impl(Handle& h, int k, int m, const double* G, const double* D, hip
 void defl_apply_launch(Handle& h, const double* in, const double* in_scale, double* store, double* out,
                        int respect_done, hipStream_t st);
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
-                const double* b, double* w, double rtol, double atol, int restart, vtsa_gmres_info* info,
-                hipStream_t st);
+                const double* b, double* w, double* pw, double rtol, double atol, int restart,
+                vtsa_gmres_info* info, hipStream_t st);
 // reductions returning host values (synchronising)
 int reduce_rms(Handle& h, const double* a, const double* s, int len, double* out, hipStream_t st);
 int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mode, double* out, hipStream_t st);
@@ -176,6 +176,7 @@ size_t gs_pairs_count(int ncol, int Nz);   // values of Handle::P
 struct ApplyArgs {
   const double* in; const double* in_scale; double* store_in; const double* pre; const double* post; double* out;
   const double* red_s; double* red_out; GmresScalars* gs; const double* zs_rms; int respect_done;
+  double* out_raw;   // optional: the output before the division by post, i.e. M^-1 (pre * scale * in)
 };
 void apply_launch(Handle& h, const ApplyArgs& a, hipStream_t s);
 

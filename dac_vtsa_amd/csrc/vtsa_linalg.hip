@@ -943,8 +943,8 @@ void defl_apply_launch(Handle& h, const double* in, const double* in_scale, doub
 }
 
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
-                const double* b, double* w, double rtol, double atol, int restart, vtsa_gmres_info* info,
-                hipStream_t st) {
+                const double* b, double* w, double* pw, double rtol, double atol, int restart,
+                vtsa_gmres_info* info, hipStream_t st) {
   if (restart > MAX_RESTART) restart = MAX_RESTART;
   if (restart > h.N) restart = h.N;
   const int N = h.N;
@@ -1023,9 +1023,10 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);
   // with the low-rank correction the solution is x = P^-1 Q w: w is returned as Q w
   if (h.defl_k > 0) defl_apply_launch(h, w, nullptr, nullptr, w, 0, st);
-  // true residual r = b - A P^-1 w (one more matvec, as scipy does after the cycle)
+  // true residual r = b - A P^-1 w (one more matvec, as scipy does after the cycle); the apply
+  // also leaves M^-1 (w*s) in pw when asked, the step the caller would form with one more apply
   {
-    ApplyArgs a{w, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 0};
+    ApplyArgs a{w, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 0, pw};
     apply_launch(h, a, st);
   }
   launch_matvec_residual(h, t, z, U, s, f, dh, h.vtmp, false, st);

@@ -1073,6 +1073,7 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
         if (e < cnt) {
           const int r = e / Nz, k = e - r * Nz;
           double o = xs[k * 6 + r] * xs0[j];
+          if (a.out_raw) a.out_raw[r * n + X0 + k] = o;
           if (a.post) o = o / xpost[j];
           a.out[r * n + X0 + k] = o;
           double q = a.red_s ? o / xred[j] : o;
@@ -1106,6 +1107,7 @@ __global__ void __launch_bounds__(PW) k_prec_apply(int n, int Nz, int ncol, int 
           if (a.store_in) a.store_in[idx] = x;
           if (a.pre) x = x * xpre[j];
           double o = x / xfw[j];
+          if (a.out_raw) a.out_raw[idx] = o;
           if (a.post) o = o / xpost[j];
           a.out[idx] = o;
           double q = a.red_s ? o / xred[j] : o;
@@ -1228,6 +1230,7 @@ __device__ inline double gs_b(const ApplyArgs& a, const double* __restrict__ s0,
 __device__ inline double gs_out(const ApplyArgs& a, int idx, double u) {
   const double o = a.post ? u / a.post[idx] : u;
   a.out[idx] = o;
+  if (a.out_raw) a.out_raw[idx] = u;
   const double q = a.red_s ? o / a.red_s[idx] : o;
   return q * q;
 }
@@ -1334,6 +1337,29 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
       // The partial sums, [4][6][Nz], sit where the widened G goes later; an element adds its
       // four in slot order.
       double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(pbw) + widen_part_off(Nz));
+      // this thread's right-hand-side elements e = lane + u NT (6 Nz <= 2 NT whenever the widened
+      // matrices fit, Nz <= 112): their global inputs are loaded before the off-line products,
+      // so that the gather waits for one round of loads; the arithmetic is gs_b's
+      constexpr int EPT = 2;
+      double pin[EPT], ppre[EPT], ps0[EPT], pjw[EPT], puw[EPT];
+      const double sc = first_visit && a.in_scale ? *a.in_scale : 1.0;
+#pragma unroll
+      for (int u = 0; u < EPT; ++u) {
+        const int e = lane + u * NT;
+        const int ec = e < 6 * Nz ? e : 0;
+        const int r = ec / Nz, k = ec - r * Nz;
+        const int idx = r * n + X0 + k;
+        ps0[u] = g.s0[idx];
+        pin[u] = first_visit ? a.in[idx] : g.bs[idx];
+        ppre[u] = first_visit && a.pre ? a.pre[idx] : 1.0;
+        pjw[u] = 0.0;
+        puw[u] = 0.0;
+        if (!g.first && nw > 0) {
+          const int W = j * Nz + k;
+          pjw[u] = g.Jcw[W * 4 + (r <= 2 ? r : 3)];
+          puw[u] = g.U[6 * n + W];
+        }
+      }
       if (!g.first) {
         for (int t = lane; t < 4 * Nz; t += NT) {
           const int q = t / Nz, k = t - q * Nz;
@@ -1356,22 +1382,26 @@ __global__ void __launch_bounds__(MODE == 0 ? PW : GSW)
         }
         lds_barrier();
       }
-      for (int e = lane; e < 6 * Nz; e += NT) {
+#pragma unroll
+      for (int u = 0; u < EPT; ++u) {
+        const int e = lane + u * NT;
+        if (e >= 6 * Nz) continue;
         const int r = e / Nz, k = e - r * Nz;
-        const int X = X0 + k;
         double acc = 0.0;
         if (!g.first) {
-          double jw = 0.0, uw = 0.0;
-          if (nw > 0) {
-            const int W = j * Nz + k;
-            jw = g.Jcw[W * 4 + (r <= 2 ? r : 3)];
-            uw = g.U[6 * n + W];
-          }
           acc = ((part[r * Nz + k] + part[(6 + r) * Nz + k]) + part[(12 + r) * Nz + k]) + part[(18 + r) * Nz + k];
-          acc += (wall && (r <= 2 || r == 5) ? jw : 0.0) * uw;
+          acc += (wall && (r <= 2 || r == 5) ? pjw[u] : 0.0) * puw[u];
         }
-        const int idx = r * n + X;
-        xs[k * 6 + r] = gs_b(a, g.s0, g.bs, idx, first_visit) + (dh / g.s0[idx]) * acc;
+        const int idx = r * n + X0 + k;
+        double b = pin[u];
+        if (first_visit) {
+          double x = pin[u] * sc;
+          if (a.store_in) a.store_in[idx] = x;
+          if (a.pre) x = x * ppre[u];
+          b = x / ps0[u];
+          g.bs[idx] = b;
+        }
+        xs[k * 6 + r] = b + (dh / ps0[u]) * acc;
       }
     } else {
       for (int e = lane; e < 6 * Nz; e += NT) xs[(e % Nz) * 6 + e / Nz] = rhs(e);

@@ -141,10 +141,13 @@ int vtsa_precond_apply(vtsa_handle* h, const double* r, double* out, vtsa_stream
  * solving (A P^-1) w = b with tolerance max(atol, rtol |b|) (core/solver.py:196-224).
  * The step uses max(1, zs_norm); zs_norm < 0: max(1, rms(z/s)) is formed on the device instead
  * (no wait before the cycle).
- * On return w holds the GMRES solution (in the preconditioned variable, as upstream). */
+ * On return w holds the GMRES solution (in the preconditioned variable, as upstream) and, when pw
+ * is not NULL, pw = M^-1 (w*s): the Newton step dz that upstream forms from it with one more solve
+ * (u = prec(w), dz = u*s: core/solver.py:240-241), taken from the apply of the cycle's closing
+ * residual, which solves the same system (pw may not alias w). */
 int vtsa_gmres(vtsa_handle* h, double t, const double* z, const double* f, const double* s, double dh,
-               double zs_norm, const double* b, double* w, double rtol, double atol, int32_t restart,
-               vtsa_gmres_info* info, vtsa_stream s_);
+               double zs_norm, const double* b, double* w, double* pw, double rtol, double atol,
+               int32_t restart, vtsa_gmres_info* info, vtsa_stream s_);
 
 /* In place on a state vector y (device, N values): every cell's energy e replaced by
  * energy(temperature(e)), the round trip a hand-off through CycleState makes (it stores T, the
