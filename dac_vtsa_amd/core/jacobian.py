@@ -183,6 +183,14 @@ class FDJacobian:
                                   ([np.ones(model.n_w)] if model.n_w else []))
         self.nevals = 0
         self._color = None
+        self._theta = None
+
+    def _theta_pattern(self):
+        """(kept entries, row indices, column pointers) of the theta-block pattern (upstream
+        core/jacobian.py:71-91), formed on first use: the device path never needs it."""
+        if self._theta is not None:
+            return self._theta
+        model, g = self.model, self.model.grid
         nv = model.ng + model.na + 1
         n = g.ncell
         idx = np.arange(self.n)
@@ -197,9 +205,9 @@ class FDJacobian:
         erow = self.indices
         cut = model.cfg.solver.precond_theta_rings - 1
         keep = (jsec[erow] == jsec[ecol]) | ((ring[erow] <= cut) & (ring[ecol] <= cut))
-        self.keep_theta = np.where(keep)[0]
-        self.red_indices = erow[self.keep_theta]
-        self.red_indptr = np.concatenate([[0], np.cumsum(np.bincount(ecol[self.keep_theta], minlength=self.n))])
+        kept = np.where(keep)[0]
+        self._theta = (kept, erow[kept], np.concatenate([[0], np.cumsum(np.bincount(ecol[kept], minlength=self.n))]))
+        return self._theta
 
     @property
     def color(self):
@@ -240,4 +248,5 @@ class FDJacobian:
         return sp.csc_matrix((data, self.indices, self.indptr), shape=(self.n, self.n))
 
     def theta_block(self, J: sp.csc_matrix) -> sp.csc_matrix:
-        return sp.csc_matrix((J.data[self.keep_theta], self.red_indices, self.red_indptr), shape=(self.n, self.n))
+        kept, red_indices, red_indptr = self._theta_pattern()
+        return sp.csc_matrix((J.data[kept], red_indices, red_indptr), shape=(self.n, self.n))

@@ -35,7 +35,7 @@ static void free_handle(Handle* h) {
   if (!h) return;
   double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2, h->FF,
                     h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
-                    h->Vb, h->red.partial, h->red.result, h->Ud, h->defl_small, h->defl_partial, h->defl_tmp, h->dg};
+                    h->Vb, h->red.partial, h->Ud, h->defl_small, h->defl_partial, h->defl_tmp, h->dg};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (h->F) (void)hipFree(h->F);
@@ -160,13 +160,14 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   }
   rc |= dalloc(&h.gs, 1);
   rc |= check(hipHostMalloc((void**)&h.gs_host, sizeof(GmresScalars)), "hipHostMalloc");
-  rc |= check(hipHostMalloc((void**)&h.host_scalars, sizeof(double) * 64), "hipHostMalloc");
+  rc |= check(hipHostMalloc((void**)&h.host_scalars, sizeof(double) * 64, hipHostMallocCoherent | hipHostMallocMapped),
+              "hipHostMalloc");
+  if (rc == 0) rc |= check(hipHostGetDevicePointer((void**)&h.red.result, h.host_scalars, 0), "hipHostGetDevicePointer");
   rc |= check(hipHostMalloc((void**)&h.progress, 64, hipHostMallocCoherent | hipHostMallocMapped), "hipHostMalloc");
   if (rc == 0) rc |= check(hipHostGetDevicePointer((void**)&h.progress_dev, h.progress, 0), "hipHostGetDevicePointer");
   h.red.max_blocks = 512;
   rc |= dalloc(&h.red.partial, partial_size(h.ncol, h.nw));
   rc |= dalloc(&h.red.counter, 16);
-  rc |= dalloc(&h.red.result, 64);
   if (rc == 0) {
     rc |= check(hipMemset(h.red.counter, 0, sizeof(unsigned) * 16), "hipMemset");
     rc |= check(hipMemset(h.gs, 0, sizeof(GmresScalars)), "hipMemset");
@@ -329,7 +330,11 @@ int vtsa_precond_setup(vtsa_handle* H, double dh, const double* scale, int32_t k
 
 int vtsa_gmres_hessenberg(vtsa_handle* H, double* Hr, int32_t* m) {
   if (!H || !Hr || !m) { set_error("vtsa_gmres_hessenberg: null argument"); return -1; }
-  const GmresScalars& G = *H->h.gs_host;
+  Handle& h = H->h;
+  if (check(hipMemcpyAsync(h.gs_host, h.gs, sizeof(GmresScalars), hipMemcpyDeviceToHost, h.gs_stream), "memcpy"))
+    return -1;
+  if (check(hipStreamSynchronize(h.gs_stream), "sync")) return -1;
+  const GmresScalars& G = *h.gs_host;
   *m = G.iters;
   for (int r = 0; r <= MAX_RESTART; ++r)
     for (int c = 0; c < MAX_RESTART; ++c) Hr[r * MAX_RESTART + c] = G.Hr[r][c];

@@ -91,7 +91,10 @@ struct Handle {
   GmresScalars* gs = nullptr; // device
   GmresScalars* gs_host = nullptr;
   Reduce red;
-  double* host_scalars = nullptr;   // pinned [64]
+  double* host_scalars = nullptr;   // pinned, mapped [64]: reduction results the host reads; the
+                                    // kernels write them here directly (red.result is its device
+                                    // address), so a wait for the stream is all a read needs
+  hipStream_t gs_stream = nullptr;  // stream of the last GMRES cycle (its Hessenberg is copied on demand)
   double* dg = nullptr;              // diagnostics scratch of the outlet rows (allocated on first use)
   int* progress = nullptr;          // coherent pinned host word: GMRES columns completed (< 0: done)
   int* progress_dev = nullptr;      // its device address

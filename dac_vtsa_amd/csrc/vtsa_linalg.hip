@@ -632,7 +632,7 @@ __global__ void k_gm_update(int N, const double* __restrict__ Vb, size_t ld, con
 // r = b - A P^-1 x and |r|
 __global__ void __launch_bounds__(RBLK) k_gm_resid(int N, const double* __restrict__ b, const double* __restrict__ ax,
                                                    GmresScalars* __restrict__ gs, double* __restrict__ partial,
-                                                   unsigned* counter) {
+                                                   unsigned* counter, double* __restrict__ hout) {
   __shared__ double sh[RBLK];
   __shared__ bool slast;
   double loc = 0.0;
@@ -644,7 +644,17 @@ __global__ void __launch_bounds__(RBLK) k_gm_resid(int N, const double* __restri
   if (threadIdx.x == 0) store_partial(partial + blockIdx.x, bs);
   double tot;
   if (last_block_total(counter, &slast, partial, &tot)) {
-    gs->rnorm = sqrt(tot);
+    const double rn = sqrt(tot);
+    gs->rnorm = rn;
+    // the cycle's outcome for the host (mapped host memory): rnorm, bnorm, presid, atol_eff, done,
+    // iterations, breakdown
+    hout[0] = rn;
+    hout[1] = gs->bnorm;
+    hout[2] = gs->presid;
+    hout[3] = gs->atol_eff;
+    hout[4] = (double)gs->done;
+    hout[5] = (double)gs->iters;
+    hout[6] = (double)gs->breakdown;
   }
 }
 
@@ -654,7 +664,6 @@ static inline int eblk(long long N) { return (int)((N + 255) / 256); }
 int reduce_rms(Handle& h, const double* a, const double* s, int len, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_rms, dim3(rgrid(len)), dim3(RBLK), 0, st, len, a, s, h.red.partial, h.red.counter + 2,
                      h.red.result);
-  if (check(hipMemcpyAsync(h.host_scalars, h.red.result, sizeof(double), hipMemcpyDeviceToHost, st), "memcpy")) return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   *out = h.host_scalars[0];
   return 0;
@@ -678,8 +687,6 @@ int reduce_weighted(Handle& h, const double* a, const double* w, int len, int mo
   hipLaunchKernelGGL(k_weighted, dim3(rgrid(h.n + h.nw)), dim3(RBLK), 0, st, h.m, mode, a, w, h.red.partial,
                      h.red.counter + 3, h.red.result);
   const int nout = mode == 0 ? 8 : (mode == 1 ? 7 : 4);
-  if (check(hipMemcpyAsync(h.host_scalars, h.red.result, sizeof(double) * nout, hipMemcpyDeviceToHost, st), "memcpy"))
-    return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   for (int q = 0; q < nout; ++q) out[q] = h.host_scalars[q];
   return 0;
@@ -693,9 +700,6 @@ void cons_stash_impl(Handle& h, const double* R, const double* s, int slot, hipS
 
 int cons_fetch_impl(Handle& h, int n, double* out, hipStream_t st) {
   if (n <= 0) return 0;
-  if (check(hipMemcpyAsync(h.host_scalars + 16, h.red.result + 16, sizeof(double) * 8 * n, hipMemcpyDeviceToHost,
-                           st), "memcpy"))
-    return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   for (int q = 0; q < 8 * n; ++q) out[q] = h.host_scalars[16 + q];
   return 0;
@@ -728,8 +732,6 @@ int newton_residual_impl(Handle& h, double t, const double* z, double dh, const 
   hipLaunchKernelGGL(k_residual_combine, dim3(rgrid(h.N)), dim3(RBLK), 0, st, h.N, z, f, dh, rk, s, R,
                      h.red.partial, h.red.counter + 2, h.red.result + 0);
   if (!rn) return 0;
-  if (check(hipMemcpyAsync(h.host_scalars, h.red.result, sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
-    return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   *rn = h.host_scalars[0];
   return 0;
@@ -743,8 +745,6 @@ int newton_eval_impl(Handle& h, double t, const double* z, double dh, const doub
                      h.red.partial, h.red.counter + 2, h.red.result + 0);
   ApplyArgs a{R, nullptr, nullptr, nullptr, nullptr, h.vtmp, s, h.red.result + 1, nullptr, nullptr, 0};
   apply_launch(h, a, st);
-  if (check(hipMemcpyAsync(h.host_scalars, h.red.result, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
-    return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   *rn = h.host_scalars[0];
   *pn = h.host_scalars[1];
@@ -762,8 +762,6 @@ int newton_try_impl(Handle& h, double t, const double* z, const double* dz, doub
                      h.red.partial, h.red.counter + 2, h.red.result + 0);
   ApplyArgs a{Rt, nullptr, nullptr, nullptr, nullptr, h.vtmp, s, h.red.result + 1, nullptr, nullptr, 0};
   apply_launch(h, a, st);
-  if (check(hipMemcpyAsync(h.host_scalars, h.red.result, 3 * sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
-    return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
   *rn = h.host_scalars[0];
   *pn = h.host_scalars[1];
@@ -782,10 +780,8 @@ int error_norm_impl(Handle& h, const double* y, const double* z2, const double* 
   hipLaunchKernelGGL(k_scale2, dim3(eblk(h.N)), dim3(256), 0, st, h.m, y, z2, rtol, h.vtmp2);
   ApplyArgs a{h.vtmp, nullptr, nullptr, nullptr, nullptr, h.vtmp, h.vtmp2, h.red.result + 3, nullptr, nullptr, 0};
   apply_launch(h, a, st);
-  if (check(hipMemcpyAsync(h.host_scalars, h.red.result + 3, sizeof(double), hipMemcpyDeviceToHost, st), "memcpy"))
-    return -1;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
-  *en = h.host_scalars[0];
+  *en = h.host_scalars[3];
   return 0;
 }
 
@@ -1030,23 +1026,24 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
     apply_launch(h, a, st);
   }
   launch_matvec_residual(h, t, z, U, s, f, dh, h.vtmp, false, st);
-  hipLaunchKernelGGL(k_gm_resid, dim3(g), dim3(RBLK), 0, st, N, b, h.vtmp, h.gs, h.red.partial, h.red.counter + 5);
-  if (check(hipMemcpyAsync(h.gs_host, h.gs, sizeof(GmresScalars), hipMemcpyDeviceToHost, st), "memcpy")) return -1;
+  const double* out = h.host_scalars + 48;
+  hipLaunchKernelGGL(k_gm_resid, dim3(g), dim3(RBLK), 0, st, N, b, h.vtmp, h.gs, h.red.partial, h.red.counter + 5,
+                     h.red.result + 48);
+  h.gs_stream = st;
   if (check(hipStreamSynchronize(st), "sync")) return -1;
-  const GmresScalars& G = *h.gs_host;
-  const int done_flag = G.done;
-  info->iters = G.iters;
-  info->breakdown = G.breakdown;
-  info->bnorm = G.bnorm;
-  info->presid = G.presid;
-  info->atol_eff = G.atol_eff;
+  const int done_flag = (int)out[4];
+  info->iters = (int)out[5];
+  info->breakdown = (int)out[6];
+  info->bnorm = out[1];
+  info->presid = out[2];
+  info->atol_eff = out[3];
   if (done_flag == 2) {   // exited before the first iteration: x = 0 (or b == 0)
-    info->rnorm = G.bnorm;
+    info->rnorm = out[1];
     info->info = 0;
     info->iters = 0;
   } else {
-    info->rnorm = G.rnorm;
-    info->info = (G.rnorm <= G.atol_eff) ? 0 : 1;
+    info->rnorm = out[0];
+    info->info = (out[0] <= out[3]) ? 0 : 1;
   }
   return 0;
 }
