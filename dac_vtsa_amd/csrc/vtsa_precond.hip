@@ -28,22 +28,23 @@ constexpr int AC = 4;            // apply gather/scatter: elements per lane whos
 // with partial pivoting (the first row of largest |entry| wins), rows in parallel through LDS.
 // Block products accumulate over m in order in T, as C = A B and C -= A B of 6x6 matrices.
 // F blocks per cell, rows of 6: 0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2.
-// Blocks from nbt on factor the same lines from their last cell upwards (the line with its cells
-// in reverse order: the blocks of slots z+1, z+2 become the lower ones), cells 0 .. kend_rev - 1
-// of the reversed line, into Fr at the reversed cell index.
+// Blocks before nbt factor cells 0 .. kend_top - 1 of each line (all of it, or down to the twist
+// pair of the twisted form); blocks from nbt on factor the same lines from their last cell upwards
+// (the line with its cells in reverse order: the blocks of slots z+1, z+2 become the lower ones),
+// cells 0 .. kend_rev - 1 of the reversed line, into Fr at the reversed cell index.
 constexpr int FSTR = 258;   // LDS doubles per column: Dinv/U1/U2 ring of two cells + pivot scratch
 
 template <typename T>
 __global__ void __launch_bounds__(PW) k_prec_factor(int n, int Nz, int ncol, int kind,
                                                     const double* __restrict__ Jc, const double* __restrict__ s0,
                                                     double dh, T* __restrict__ F, T* __restrict__ Fr, int nbt,
-                                                    int kend_rev) {
+                                                    int kend_top, int kend_rev) {
   __shared__ double lds[PCOLS * FSTR];
   const int lane = threadIdx.x;
   const int grp = lane / 6, r = lane - grp * 6;
   const bool rev = (int)blockIdx.x >= nbt;
   const int col0 = ((int)blockIdx.x - (rev ? nbt : 0)) * PCOLS;
-  const int kend = rev ? kend_rev : Nz;
+  const int kend = rev ? kend_rev : kend_top;
   T* const Fo = rev ? Fr : F;
   // slots and cell offsets of the blocks k-2, k-1, k+1, k+2 along the factorisation order
   const int sm2 = rev ? S_ZP2 : S_ZM2, sm1 = rev ? S_ZP1 : S_ZM1, sp1 = rev ? S_ZM1 : S_ZP1,
@@ -505,60 +506,82 @@ __host__ __device__ constexpr size_t gs_g_lds_bytes(int Nz) {
 
 size_t gs_pairs_count(int ncol, int Nz) { return (size_t)ncol * npairs(Nz) * PP; }
 
-// Pair form of the factors, one thread per (line, pair, row r of 6).  Products accumulate over m
-// in order in FP64 and are stored in T.  Twisted form (tw_m >= 0): pairs K < tw_m from F, pairs
-// K > tw_m from the reversed factor Fr (pair NP - 1 - K of the reversed line, its two cells in
-// reverse order), pair tw_m written by k_gs_twist.
+// Pair form of the factors, one thread per (line, pair, row r of 6), PPB pairs per block: the
+// products every row needs, ^U1_{2K+1} = Dinv U1 and ^U2_{2K+1} = Dinv U2 of the pair's second cell,
+// are formed once per pair (row r by thread r) into LDS.  Products accumulate over m in order in
+// FP64 and are stored in T.  Twisted form (tw_m >= 0): pairs K < tw_m from F, pairs K > tw_m from
+// the reversed factor Fr (pair NP - 1 - K of the reversed line, its two cells in reverse order),
+// pair tw_m written by k_gs_twist.
+constexpr int PPB = 32;   // pairs per block of the pair-form builder (6 threads each)
 template <typename T>
-__global__ void __launch_bounds__(256) k_gs_pairs(int Nz, int ncol, const T* __restrict__ F, T* __restrict__ P,
-                                                  const T* __restrict__ Fr, int tw_m) {
+__global__ void __launch_bounds__(PPB * 6) k_gs_pairs(int Nz, int ncol, const T* __restrict__ F, T* __restrict__ P,
+                                                      const T* __restrict__ Fr, int tw_m) {
+  __shared__ double su1[PPB][36], su2[PPB][36];
   const int NP = npairs(Nz);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncol * NP * 6) return;
-  const int r = t % 6, pr = t / 6;
+  const int lp = threadIdx.x / 6, r = threadIdx.x - lp * 6;
+  const int pr = blockIdx.x * PPB + lp;
   const int L = pr / NP, K = pr - L * NP;
-  if (tw_m >= 0 && K == tw_m) return;
+  const bool valid = pr < ncol * NP && !(tw_m >= 0 && K == tw_m);
   const bool bot = tw_m >= 0 && K > tw_m;
   const T* src = bot ? Fr : F;
   const int Ks = bot ? NP - 1 - K : K;
   const int k0 = 2 * Ks, k1 = 2 * Ks + 1;
   // entry (i, c) of block b (0 L2, 1 L1, 2 Dinv, 3 U1, 4 U2) of cell k; zero for the phantom
   auto fb = [&](int k, int b, int i, int c) -> double {
-    return k < Nz ? (double)src[(size_t)(L * Nz + k) * 180 + b * 36 + i * 6 + c] : 0.0;
+    return valid && k < Nz ? (double)src[(size_t)(L * Nz + k) * 180 + b * 36 + i * 6 + c] : 0.0;
   };
-  double u1[36], u2[36];   // ^U1_{2K+1}, ^U2_{2K+1}
-  for (int i = 0; i < 6; ++i)
+  {   // row r of ^U1_{2K+1}, ^U2_{2K+1}
+    double d[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) d[m] = fb(k1, 2, r, m);
+#pragma unroll
     for (int c = 0; c < 6; ++c) {
       double a1 = 0.0, a2 = 0.0;
-      for (int m = 0; m < 6; ++m) { a1 += fb(k1, 2, i, m) * fb(k1, 3, m, c); a2 += fb(k1, 2, i, m) * fb(k1, 4, m, c); }
-      u1[i * 6 + c] = a1;
-      u2[i * 6 + c] = a2;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) { a1 += d[m] * fb(k1, 3, m, c); a2 += d[m] * fb(k1, 4, m, c); }
+      su1[lp][r * 6 + c] = a1;
+      su2[lp][r * 6 + c] = a2;
     }
-  double w1[6], w2[6];   // row r of ^U1_{2K}, ^U2_{2K}
-  for (int c = 0; c < 6; ++c) {
-    double a1 = 0.0, a2 = 0.0;
-    for (int m = 0; m < 6; ++m) { a1 += fb(k0, 2, r, m) * fb(k0, 3, m, c); a2 += fb(k0, 2, r, m) * fb(k0, 4, m, c); }
-    w1[c] = a1;
-    w2[c] = a2;
   }
+  __syncthreads();
+  if (!valid) return;
+  double w1[6], w2[6];   // row r of ^U1_{2K}, ^U2_{2K}
+  {
+    double d[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) d[m] = fb(k0, 2, r, m);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) { a1 += d[m] * fb(k0, 3, m, c); a2 += d[m] * fb(k0, 4, m, c); }
+      w1[c] = a1;
+      w2[c] = a2;
+    }
+  }
+  double l1[6];   // row r of L1_{2K+1}
+#pragma unroll
+  for (int m = 0; m < 6; ++m) l1[m] = fb(k1, 1, r, m);
   T* out = P + (size_t)pr * PP;
+#pragma unroll
   for (int c = 0; c < 6; ++c) {
     double a = 0.0, b2 = 0.0, p = 0.0, q = 0.0;
+#pragma unroll
     for (int m = 0; m < 6; ++m) {
-      a += fb(k1, 1, r, m) * fb(k0, 0, m, c);    // (L1_{2K+1} L2_{2K})[r][c]
-      b2 += fb(k1, 1, r, m) * fb(k0, 1, m, c);   // (L1_{2K+1} L1_{2K})[r][c]
-      p += w1[m] * u1[m * 6 + c];                // (^U1_{2K} ^U1_{2K+1})[r][c]
-      q += w1[m] * u2[m * 6 + c];                // (^U1_{2K} ^U2_{2K+1})[r][c]
+      a += l1[m] * fb(k0, 0, m, c);         // (L1_{2K+1} L2_{2K})[r][c]
+      b2 += l1[m] * fb(k0, 1, m, c);        // (L1_{2K+1} L1_{2K})[r][c]
+      p += w1[m] * su1[lp][m * 6 + c];      // (^U1_{2K} ^U1_{2K+1})[r][c]
+      q += w1[m] * su2[lp][m * 6 + c];      // (^U1_{2K} ^U2_{2K+1})[r][c]
     }
     out[PO_G + r * 12 + c] = (T)fb(k0, 0, r, c);
     out[PO_G + r * 12 + 6 + c] = (T)fb(k0, 1, r, c);
     out[PO_G + (6 + r) * 12 + c] = (T)(-a);
     out[PO_G + (6 + r) * 12 + 6 + c] = (T)(fb(k1, 0, r, c) - b2);
-    out[PO_C + r * 6 + c] = (T)fb(k1, 1, r, c);
+    out[PO_C + r * 6 + c] = (T)l1[c];
     out[PO_H + r * 12 + c] = (T)(w2[c] - p);
     out[PO_H + r * 12 + 6 + c] = (T)(-q);
-    out[PO_H + (6 + r) * 12 + c] = (T)u1[r * 6 + c];
-    out[PO_H + (6 + r) * 12 + 6 + c] = (T)u2[r * 6 + c];
+    out[PO_H + (6 + r) * 12 + c] = (T)su1[lp][r * 6 + c];
+    out[PO_H + (6 + r) * 12 + 6 + c] = (T)su2[lp][r * 6 + c];
     out[PO_D + r * 6 + c] = (T)fb(k0, 2, r, c);
     out[PO_D + 36 + r * 6 + c] = (T)fb(k1, 2, r, c);
     out[PO_U + r * 6 + c] = (T)w1[c];
@@ -1610,22 +1633,31 @@ void gs_colour_lines(int Nr, int Nth, int nwall_lines, std::vector<int>& order, 
   cstart[ncolour] = (int)order.size();
 }
 
+template <typename T>
+static int gs_pick_mode(Handle& h, int o);
+
 void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int prec, int sweeps, hipStream_t s) {
   (void)hipMemcpyAsync(h.s0, scale, sizeof(double) * h.N, hipMemcpyDeviceToDevice, s);
   const int fkind = kind == VTSA_PREC_BLOCK_JACOBI ? VTSA_PREC_BLOCK_JACOBI : VTSA_PREC_ZLINE;
   const int nb = (h.ncol + PCOLS - 1) / PCOLS;
-  // twisted pair form (line Gauss-Seidel, an even number of cells, at least three pairs): the
-  // reversed factor of each line down to the twist pair, in the same launch as the full top one
+  // twisted pair form (line Gauss-Seidel, an even number of cells, at least three pairs, a pair
+  // form granted both for the staging option and automatically): each line factored from its
+  // first cell down and from its last cell up, both only as far as the twist pair, in one launch
   const int Nz = h.desc.Nz, NP = npairs(Nz);
-  const int tw_m = kind == VTSA_PREC_ZLINE_GS && h.gs_twist && Nz % 2 == 0 && NP >= 3 ? (NP - 1) / 2 : -1;
+  bool twist = kind == VTSA_PREC_ZLINE_GS && h.gs_twist && Nz % 2 == 0 && NP >= 3;
+  if (twist)
+    twist = prec == VTSA_FP32 ? gs_pick_mode<float>(h, h.gs_full) >= 1 && gs_pick_mode<float>(h, 1) >= 1
+                              : gs_pick_mode<double>(h, h.gs_full) >= 1 && gs_pick_mode<double>(h, 1) >= 1;
+  const int tw_m = twist ? (NP - 1) / 2 : -1;
   const int nbr = tw_m >= 0 ? nb : 0;
+  const int kend_top = tw_m >= 0 ? 2 * tw_m + 2 : Nz;
   const int kend_rev = tw_m >= 0 ? 2 * (NP - 1 - tw_m) + 2 : 0;
   if (prec == VTSA_FP32)
     hipLaunchKernelGGL(k_prec_factor<float>, dim3(nb + nbr), dim3(PW), 0, s, h.n, Nz, h.ncol, fkind, h.Jc, h.s0, dh,
-                       (float*)h.F, (float*)h.Fr, nb, kend_rev);
+                       (float*)h.F, (float*)h.Fr, nb, kend_top, kend_rev);
   else
     hipLaunchKernelGGL(k_prec_factor<double>, dim3(nb + nbr), dim3(PW), 0, s, h.n, Nz, h.ncol, fkind, h.Jc, h.s0, dh,
-                       (double*)h.F, (double*)h.Fr, nb, kend_rev);
+                       (double*)h.F, (double*)h.Fr, nb, kend_top, kend_rev);
   if (h.nw > 0) hipLaunchKernelGGL(k_prec_wall, dim3((h.nw + 255) / 256), dim3(256), 0, s, h.nw, h.Jw, dh, h.Fw);
   if (kind == VTSA_PREC_ZLINE_GS) {
     // the off-line blocks are stored in the precision of the factors (the FP32 form halves the
@@ -1639,10 +1671,10 @@ void launch_prec_setup(Handle& h, double dh, const double* scale, int kind, int 
                          h.Jw, h.s0, dh, h.Tw);
     const int nt = h.ncol * NP * 6;
     if (prec == VTSA_FP32)
-      hipLaunchKernelGGL(k_gs_pairs<float>, dim3((nt + 255) / 256), dim3(256), 0, s, Nz, h.ncol, (const float*)h.F,
-                         (float*)h.P, (const float*)h.Fr, tw_m);
+      hipLaunchKernelGGL(k_gs_pairs<float>, dim3((nt + PPB * 6 - 1) / (PPB * 6)), dim3(PPB * 6), 0, s, Nz, h.ncol,
+                         (const float*)h.F, (float*)h.P, (const float*)h.Fr, tw_m);
     else
-      hipLaunchKernelGGL(k_gs_pairs<double>, dim3((nt + 255) / 256), dim3(256), 0, s, Nz, h.ncol,
+      hipLaunchKernelGGL(k_gs_pairs<double>, dim3((nt + PPB * 6 - 1) / (PPB * 6)), dim3(PPB * 6), 0, s, Nz, h.ncol,
                          (const double*)h.F, (double*)h.P, (const double*)h.Fr, tw_m);
     if (tw_m >= 0) {
       if (prec == VTSA_FP32)
@@ -1752,14 +1784,15 @@ static bool gs_mode_ok(const Handle& h) {
   return ok;
 }
 
+// Half-sweep variant for staging option o: whole-line staging when the option allows it and the
+// device grants the LDS: the whole pair factor (MODE 1) while the blocks a CU can hold at that LDS
+// size cover the widest colour in one round, else G and H only (MODE 2), else the segmented
+// cell-by-cell form (MODE 0).
 template <typename T>
-static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
-  const int N = h.N, Nz = h.desc.Nz;
-  // Whole-line staging when the option allows it and the device grants the LDS: the whole pair
-  // factor (MODE 1) while the blocks a CU can hold at that LDS size cover the widest colour in
-  // one round, else G and H only (MODE 2), else the segmented cell-by-cell form (MODE 0).
+static int gs_pick_mode(Handle& h, int o) {
+  const int Nz = h.desc.Nz;
   const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds2 = gs_lds_bytes<T, 2>(Nz), lds3 = gs_lds_bytes<T, 3>(Nz);
-  const size_t lds0 = gs_lds_bytes<T, 0>(Nz), lds4 = gs_lds_bytes<T, 4>(Nz);
+  const size_t lds4 = gs_lds_bytes<T, 4>(Nz);
   constexpr bool narrow = sizeof(T) == sizeof(float);
   int widest = 0;
   for (size_t c = 0; c + 1 < h.gs_cstart.size(); ++c) widest = std::max(widest, h.gs_cstart[c + 1] - h.gs_cstart[c]);
@@ -1768,7 +1801,6 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   // 4 G then H, 5 G then H widened to FP64 (FP32 factors; FP64 ones: as 4); automatic takes the
   // first of G then H widened (FP32 factors: FP64 operands on the recurrences), the whole
   // factor, G and H, G then H whose blocks all fit on the CUs at once, else the first that fits
-  const int o = h.gs_full;
   bool ok4 = false;
   if constexpr (narrow) ok4 = (o == 1 || o == 5) && gs_mode_ok<T, 4>(h);
   const bool ok1 = (o == 1 || o == 3) && gs_mode_ok<T, 1>(h);
@@ -1784,6 +1816,19 @@ static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
   else if (ok1) mode = 1;
   else if (ok2) mode = 2;
   else if (ok3) mode = 3;
+  return mode;
+}
+
+template <typename T>
+static void gs_launch(Handle& h, const ApplyArgs& a, hipStream_t s) {
+  const int N = h.N, Nz = h.desc.Nz;
+  const size_t lds1 = gs_lds_bytes<T, 1>(Nz), lds2 = gs_lds_bytes<T, 2>(Nz), lds3 = gs_lds_bytes<T, 3>(Nz);
+  const size_t lds0 = gs_lds_bytes<T, 0>(Nz), lds4 = gs_lds_bytes<T, 4>(Nz);
+  constexpr bool narrow = sizeof(T) == sizeof(float);
+  // a twisted set-up keeps the cell factors only down to the twist pair, so it runs a pair form
+  // whatever staging option was set after it (one is granted: the set-up checked)
+  int mode = gs_pick_mode<T>(h, h.gs_full);
+  if (mode == 0 && h.gs_tw_m >= 0) mode = gs_pick_mode<T>(h, 1);
   GsArgs g{h.Jo, h.Jcw, h.Jw, h.Jwc, h.s0, h.gsb, h.gsu, h.gs_order, h.prec_dh, 1, h.Tw};
   const int C = (int)h.gs_cstart.size() - 1;
   // With two colours no line's first visit reads a line not yet visited, so the right-hand side

@@ -281,8 +281,10 @@ int vtsa_time_fdjac(vtsa_handle* h, double t, const double* y, const double* f0,
  * VTSA_OPT_GS_TWIST: 1 (default) lines of an even number of cells (at least 6) are solved by a
  * twisted factorisation, top half from the first cell down and bottom half from the last cell up,
  * meeting in a middle pair (half as many dependent steps; other roundings), 0 the classic top-down
- * factorisation; read by the next preconditioner set-up.  Staging levels 1-5 then run the twisted
- * form, level 0 the classic cell factors, which every set-up keeps. */
+ * factorisation; read by the next preconditioner set-up, which keeps the classic form when the
+ * staging level is 0 or no pair form is granted.  A twisted set-up keeps the cell factors only
+ * down to the twist pair: staging levels 1-5 run the twisted form, and level 0 set after it runs
+ * the automatic pair form. */
 enum { VTSA_OPT_GS_WHOLE_LINES = 1, VTSA_OPT_GS_FORWARD = 2, VTSA_OPT_GS_TWIST = 3 };
 int vtsa_set_option(vtsa_handle* h, int32_t option, int32_t value);
 

@@ -308,7 +308,9 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     sc = dv.to_host(s0)
     outs = []
     # whole pair factor staged, both recurrence matrices staged, the forward then the backward one
-    # (pair recurrences), the same widened to FP64 in LDS (FP32 factors), segmented (cell by cell)
+    # (pair recurrences), the same widened to FP64 in LDS (FP32 factors), segmented (cell by cell;
+    # a twisted set-up keeps the cell factors only down to the twist pair and runs the automatic
+    # pair form instead)
     for level, lt in ((3, tol_pair), (2, tol_pair), (4, tol_pair), (5, tol_pair), (0, tol)):
         _abi.call("vtsa_set_option", m.handle, 1, level)
         out = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
@@ -322,6 +324,10 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
         # neighbour first (other roundings of the right-hand side)
         assert np.max(np.abs(outs[0] - outs[3]) / sc) / np.max(np.abs(outs[0]) / sc) < 1e-11
     assert np.max(np.abs(outs[0] - outs[4]) / sc) / np.max(np.abs(ref) / sc) < tol_pair
+    if over != "odd":
+        _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_WHOLE_LINES, 1)
+        auto = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
+        assert np.array_equal(outs[4], auto)
     # forward sweeps only (colours 0..C-1 per sweep)
     ref_f = _line_gs_reference(np.eye(m.n_core) - dh * J, m, r, sweeps, forward=True)
     _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_FORWARD, 1)
@@ -330,14 +336,18 @@ def test_line_gauss_seidel_matches_host_reference(fixture, over, sweeps, prec, t
     assert np.max(np.abs(out - ref_f) / sc) / np.max(np.abs(ref_f) / sc) < tol_pair
     assert np.max(np.abs(out - outs[0]) / sc) > 0.0     # a different operator from the symmetric one
     _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_FORWARD, 0)
-    # the classic top-down pair form (read at set-up)
+    # the classic top-down pair form and the segmented cell-by-cell solve (read at set-up)
     _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_TWIST, 0)
     ops.precond_setup(dh, s0, PRECONDITIONERS["zline_gs"], prec, sweeps)
     _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_WHOLE_LINES, 3)
     classic = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
     assert np.max(np.abs(classic - ref) / sc) / np.max(np.abs(ref) / sc) < tol_pair
+    _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_WHOLE_LINES, 0)
+    seg = dv.to_host(ops.prec_apply(dv.to_dev(r), dv.empty(m.n_core)))
+    assert np.max(np.abs(seg - ref) / sc) / np.max(np.abs(ref) / sc) < tol
+    assert np.max(np.abs(seg - classic) / sc) / np.max(np.abs(ref) / sc) < tol_pair
     if over == "odd":
-        assert np.array_equal(classic, outs[0])
+        assert np.array_equal(classic, outs[0]) and np.array_equal(seg, outs[4])
     else:
         assert not np.array_equal(classic, outs[0])
     _abi.call("vtsa_set_option", m.handle, _abi.OPT_GS_TWIST, 1)
