@@ -191,12 +191,17 @@ def main(argv=None) -> int:
     ap.add_argument("--max-cycles", type=int, default=None, help="cycle limit per case (default: run.max_cycles)")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32", help="preconditioner factors")
     ap.add_argument("--lowrank", type=int, default=0, help="rank of the preconditioner correction (0 off)")
+    ap.add_argument("--part", default="0/1", help="k/n: run only the drawn cases whose id is k mod n (one share "
+                                                  "of a batch split across processes)")
     ap.add_argument("--json", default=None, help="also write the result line to this file")
     args = ap.parse_args(argv)
+    k_part, n_part = (int(v) for v in args.part.split("/"))
+    if not 0 <= k_part < n_part:
+        ap.error("--part k/n needs 0 <= k < n")
 
     def say(msg):
         print(msg, flush=True)
-    cases = draw_cases(args.cases, seed=args.seed)
+    cases = [c for c in draw_cases(args.cases, seed=args.seed) if c["id"] % n_part == k_part]
     ikw = {"gs_sweeps": 2, "conserve": "project", "precision": args.precision}
     if args.lowrank:
         ikw["lowrank"] = args.lowrank
@@ -204,6 +209,7 @@ def main(argv=None) -> int:
                     max_cycles=args.max_cycles if args.css else 1,
                     base_overrides={"solver.conservation_tol": 1e-5}, integrator_kw=ikw)
     keys = ("recovery_pct", "purity_dry_pct", "specific_total_kJ_per_kg")
+    by_id = {c["id"]: c for c in cases}
     rows = []
     for c in out["cases"]:
         row = {"id": c["id"], "grid": c["grid"], "cells": c["cells"], "ok": c["ok"], "wall_s": round(c["wall_s"], 2)}
@@ -211,12 +217,12 @@ def main(argv=None) -> int:
             row.update(cycles=c["cycles"], css=c["css"], sim_s=c["sim_s"],
                        max_mass_rel_err=c["max_mass_rel_err"], max_energy_rel_err=c["max_energy_rel_err"],
                        **{k: c["kpis"][k] for k in keys},
-                       **{k: v for k, v in cases[c["id"]]["overrides"].items() if k.startswith(("operation.", "feed.RH"))})
+                       **{k: v for k, v in by_id[c["id"]]["overrides"].items() if k.startswith(("operation.", "feed.RH"))})
         else:
             row["error"] = c["error"]
         rows.append(row)
     ok = [r for r in rows if r["ok"]]
-    line = {"workload": f"{args.cases} drawn cases (seed {args.seed}), "
+    line = {"workload": f"{len(cases)} of {args.cases} drawn cases (seed {args.seed}, part {args.part}), "
                         + ("each to cyclic steady state" if args.css else "one cycle each")
                         + f" from the clean bed, {args.precision} factors"
                         + (f", rank-{args.lowrank} correction" if args.lowrank else "") + f", {args.workers} HIP streams",
