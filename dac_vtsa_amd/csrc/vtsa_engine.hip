@@ -34,7 +34,7 @@ static int upload(double** dst, const double* src, size_t count) {
 static void free_handle(Handle* h) {
   if (!h) return;
   double* bufs[] = {h->V, h->Az, h->Ath, h->arc, h->Ar, h->drc, h->feed_flux, h->N_nom, h->cov, h->pb, h->pb2, h->FF,
-                    h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
+                    h->ytmp, h->ftmp, h->vtmp, h->vtmp2, h->accbuf, h->Jc, h->Pfd, h->Jo, h->Jcw, h->Jw, h->Jwc, h->Fw, h->Tw, h->s0,
                     h->Vb, h->red.partial, h->Ud, h->defl_small, h->defl_partial, h->defl_tmp, h->dg};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
@@ -128,6 +128,7 @@ int vtsa_create(const vtsa_model_desc* desc, const double* V, const double* Az, 
   rc |= dalloc(&h.vtmp2, h.N);
   rc |= dalloc(&h.accbuf, NACC);
   rc |= dalloc(&h.Jc, (size_t)h.n * NSLOT * 36);
+  rc |= dalloc(&h.Pfd, (size_t)h.n * NV * 14);
   rc |= dalloc(&h.Jcw, nwall * 4);
   rc |= dalloc(&h.Jw, nwall * 5);
   rc |= dalloc(&h.Jwc, nwall * 6);

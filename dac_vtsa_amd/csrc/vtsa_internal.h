@@ -55,6 +55,7 @@ struct Handle {
   double* accbuf = nullptr;  // [NACC]
   // Jacobian blocks
   double *Jc = nullptr, *Jcw = nullptr, *Jw = nullptr, *Jwc = nullptr;
+  double* Pfd = nullptr;      // [n][NV][14] per finite-difference column: the perturbed cell's Props, LDF rates
   double* Jo = nullptr;       // [144][n] off-line blocks (r-1, r+1, theta-1, theta+1), cell fastest
   // preconditioner factors (FP64 or FP32), scale at factorisation
   void* F = nullptr;          // [n][5][36] blocks (L2, L1, Dinv, U1, U2) in the chosen precision

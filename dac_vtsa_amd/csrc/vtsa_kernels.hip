@@ -12,7 +12,9 @@
 //             n .. n+nw-1).
 // The finite-difference Jacobian perturbs one column per thread and recomputes only the
 // affected row cell (k_fd_cells / k_fd_wall): the values equal the upstream coloured forward
-// differences (core/jacobian.py:71-86) because a row reads only the columns of its stencil.
+// differences (core/jacobian.py:71-86) because a row reads only the columns of its stencil.  The
+// perturbed cell's properties are formed once per column (k_fd_props) and a row re-evaluates
+// only the faces that read the perturbed cell.
 #include "vtsa_internal.h"
 
 namespace vtsa {
@@ -440,17 +442,66 @@ __device__ inline int slot_cell(const vtsa_model_desc& d, int i, int j, int k, i
   return (ii * Nth + jj) * Nz + kk;
 }
 
-// Thread (X, slot, v): column v of stencil cell A = slot(X); writes Jc[X][slot][0..5][v].
+// Values of one finite-difference column's record: the perturbed cell's Props in member order,
+// then its two LDF rates.
+constexpr int FDREC = 14;
+
+// Thread (A, v): cell A with variable v moved by its finite-difference step, its properties and
+// LDF rates into record Pfd[A NV + v] (each is read by the up to 9 rows whose stencil holds A).
+__global__ void __launch_bounds__(BLK) k_fd_props(Model m, const double* __restrict__ y,
+                                                  double* __restrict__ Pfd) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.n;
+  if (tid >= n * NV) return;
+  const vtsa_model_desc& d = m.d;
+  const int v = tid % NV, A = tid / NV;
+  double ya[NV];
+  for (int vv = 0; vv < NV; ++vv) ya[vv] = y[vv * n + A];
+  const double h = fd_step(ya[v], d.typ[v]);
+  ya[v] = ya[v] + h;
+  Props p;
+  double rate[NA], qs[NA];
+  cell_props(d, ya, ya + 3, ya[5], p, rate, qs);
+  double* r = Pfd + (size_t)tid * FDREC;
+  r[0] = p.T; r[1] = p.P; r[2] = p.Cpos; r[3] = p.y[0]; r[4] = p.y[1]; r[5] = p.y[2];
+  r[6] = p.mu; r[7] = p.kg; r[8] = p.rho; r[9] = p.cpm; r[10] = p.Dm; r[11] = p.lam0;
+  r[12] = rate[0]; r[13] = rate[1];
+}
+
+// Whether face (kind, fi, fj, fk) of a cell's divergence reads cell (ai, aj, ak): the z face
+// between cells k and k+1 of a column reads its cells k-1 .. k+2, an end face its end cell, a
+// radial or theta face the two cells it separates.
+__device__ inline bool face_reads(const vtsa_model_desc& d, int kind, int fi, int fj, int fk, int ai, int aj,
+                                  int ak) {
+  if (kind == 0 || kind == 3) {
+    if (ai != fi || aj != fj) return false;
+    if (kind == 0 && fk == 0) return ak == 0;
+    if (kind == 3 && fk == d.Nz - 1) return ak == d.Nz - 1;
+    const int lo = kind == 0 ? fk - 1 : fk;   // lower cell of the face
+    return ak >= lo - 1 && ak <= lo + 2;
+  }
+  if (ak != fk) return false;
+  if (kind == 1) return aj == fj && (ai == fi || ai == fi + 1);
+  const int jn = (fj + 1 == d.Nth) ? 0 : fj + 1;
+  return ai == fi && (aj == fj || aj == jn);
+}
+
+// Thread (slot, X, v): column v of stencil cell A = slot(X); writes Jc[X][slot][0..5][v].  A
+// wavefront holds one slot (n NV threads per slot), so the faces of X that read A, the only ones
+// evaluated here, are the same for all its lanes; the others come from the face table of the
+// base point (the values the base residual f0 was summed from), and the divergence is summed in
+// the order of the residual.  A's properties and rates come from its k_fd_props record.
 __global__ void __launch_bounds__(BLK) k_fd_cells(Model m, double t, const double* __restrict__ y,
-                                                  const double* __restrict__ pb, const double* __restrict__ f0,
+                                                  const double* __restrict__ pb, const double* __restrict__ FF,
+                                                  const double* __restrict__ Pfd, const double* __restrict__ f0,
                                                   double* __restrict__ Jc) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = m.n;
   if (tid >= n * NSLOT * NV) return;
   const vtsa_model_desc& d = m.d;
   const int v = tid % NV;
-  const int slot = (tid / NV) % NSLOT;
-  const int X = tid / (NV * NSLOT);
+  const int X = (tid / NV) % n;
+  const int slot = tid / (NV * n);
   int i, j, k;
   cell_ijk(d, X, i, j, k);
   double* out = Jc + ((size_t)X * NSLOT + slot) * 36;
@@ -463,23 +514,40 @@ __global__ void __launch_bounds__(BLK) k_fd_cells(Model m, double t, const doubl
   for (int vv = 0; vv < NV; ++vv) ya[vv] = y[vv * n + A];
   const double h = fd_step(ya[v], d.typ[v]);
   ya[v] = ya[v] + h;
+  const double* rec = Pfd + ((size_t)A * NV + v) * FDREC;
   Props pa;
-  double rate_a[NA], qs[NA];
-  cell_props(d, ya, ya + 3, ya[5], pa, rate_a, qs);
+  pa.T = rec[0]; pa.P = rec[1]; pa.Cpos = rec[2]; pa.y[0] = rec[3]; pa.y[1] = rec[4]; pa.y[2] = rec[5];
+  pa.mu = rec[6]; pa.kg = rec[7]; pa.rho = rec[8]; pa.cpm = rec[9]; pa.Dm = rec[10]; pa.lam0 = rec[11];
   SwapAccess acc{y, pb, n, A, &pa, ya};
   double rate[NA];
-  if (A == X) { rate[0] = rate_a[0]; rate[1] = rate_a[1]; }
+  if (A == X) { rate[0] = rec[12]; rate[1] = rec[13]; }
   else { rate[0] = f0[3 * n + X]; rate[1] = f0[4 * n + X]; }
   double Tw = (d.wall_on && i == d.Nr - 1) ? y[6 * n + j * d.Nz + k] : 0.0;
+  int ai, aj, ak;
+  cell_ijk(d, A, ai, aj, ak);
+  const FaceTable ft = FaceTable::of(d);
+  const int nf = ft.count();
+  double div[NG + 1];
+  cell_divergence(d, i, j, k,
+                  [&](int kind, int fi, int fj, int fk, double* F, double& H) {
+                    if (face_reads(d, kind, fi, fj, fk, ai, aj, ak)) {
+                      any_face(m, acc, kind, fi, fj, fk, t, F, H);
+                    } else {
+                      const int x = ft.index(d, kind, fi, fj, fk);
+                      for (int s = 0; s < NG; ++s) F[s] = FF[(size_t)s * nf + x];
+                      H = FF[(size_t)NG * nf + x];
+                    }
+                  },
+                  div);
   double fl[4];
-  cell_flux(m, acc, Tw, i, j, k, t, rate, fl);
+  cell_rows(m, acc, div, Tw, i, j, k, t, rate, fl);
   out[0 * NV + v] = (fl[0] - f0[X]) / h;
   out[1 * NV + v] = (fl[1] - f0[n + X]) / h;
   out[2 * NV + v] = (fl[2] - f0[2 * n + X]) / h;
   out[5 * NV + v] = (fl[3] - f0[5 * n + X]) / h;
   if (A == X) {
-    out[3 * NV + v] = (rate_a[0] - f0[3 * n + X]) / h;
-    out[4 * NV + v] = (rate_a[1] - f0[4 * n + X]) / h;
+    out[3 * NV + v] = (rec[12] - f0[3 * n + X]) / h;
+    out[4 * NV + v] = (rec[13] - f0[4 * n + X]) / h;
   } else {
     out[3 * NV + v] = 0.0;
     out[4 * NV + v] = 0.0;
@@ -574,8 +642,12 @@ void launch_diag(Handle& h, double t, const double* y, const double* pb, double*
 void launch_fdjac(Handle& h, double t, const double* y, const double* f0, hipStream_t s) {
   // properties at y (the base point of every column)
   launch_props(h, y, h.pb, nullptr, nullptr, s);
-  hipLaunchKernelGGL(k_fd_cells, dim3(nblk((long long)h.n * NSLOT * NV)), dim3(BLK), 0, s, h.m, t, y, h.pb, f0,
-                     h.Jc);
+  // its face table (the faces a column does not reach are read from it)
+  const int nf = FaceTable::of(h.desc).count();
+  hipLaunchKernelGGL(k_faces, dim3(nblk(nf, RB)), dim3(RB), 0, s, h.m, t, y, h.pb, h.FF, h.gs, 0);
+  hipLaunchKernelGGL(k_fd_props, dim3(nblk((long long)h.n * NV)), dim3(BLK), 0, s, h.m, y, h.Pfd);
+  hipLaunchKernelGGL(k_fd_cells, dim3(nblk((long long)h.n * NSLOT * NV)), dim3(BLK), 0, s, h.m, t, y, h.pb, h.FF,
+                     h.Pfd, f0, h.Jc);
   if (h.nw > 0)
     hipLaunchKernelGGL(k_fd_wall, dim3(nblk((long long)h.nw * 11)), dim3(BLK), 0, s, h.m, t, y, h.pb, f0, h.Jcw,
                        h.Jw, h.Jwc);
