@@ -61,8 +61,9 @@ WINDOW_S = CYCLE_S / 20.0
 TRAFFIC_BYTES_PER_LAUNCH = 56659456
 # FP64 operations per launch on the kernel-probe workload (the heating step's model on the clean
 # bed at t = 0) from the rocprofv3 counters SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave
-# instructions x 64 lanes, an FMA counted as 2).
-FP64_FLOP_PER_LAUNCH = {"fd_jacobian": 20264750976, "residual": 280426880}
+# instructions x 64 lanes, an FMA counted as 2); the FD Jacobian's figure sums every kernel its
+# launch runs (k_props, k_faces, k_fd_props, k_fd_cells, k_fd_wall).
+FP64_FLOP_PER_LAUNCH = {"fd_jacobian": 5127644800, "residual": 280426880}
 METRIC = "cycle sim-s/wall-s, 1 GPU: learned+FP32 vs plain FP64 at MB/EB err <=1e-6"
 
 
@@ -206,8 +207,9 @@ def train_network(args, base_kw, log=_log):
 
 
 def kernel_probes(cfg, grid, props, state, n_rhs=200, n_jac=20):
-    """Average duration of one residual evaluation f(t, y) (k_props + k_flux) and of one coloured
-    FD Jacobian (k_fd_cells + k_fd_wall) on the heating step's model at the bed ``state``, each
+    """Average duration of one residual evaluation f(t, y) (k_props + k_faces + k_flux) and of one
+    coloured FD Jacobian (base-point properties and face table, then k_fd_props + k_fd_cells +
+    k_fd_wall) on the heating step's model at the bed ``state``, each
     timed with HIP events recorded on the stream it is launched on; the residual's algorithmic
     traffic (read y, write f) and, where measured, both kernels' FP64 throughput."""
     import ctypes as C
@@ -227,9 +229,9 @@ def kernel_probes(cfg, grid, props, state, n_rhs=200, n_jac=20):
     jac_s = ms.value / 1e3 / n_jac
     algo = 96 * grid.ncell + 16 * m.n_w
     m.close()
-    out = {"residual": {"kernels": "k_props + k_flux", "avg_launch_us": rhs_s * 1e6, "algo_bytes_per_launch": algo,
-                        "achieved_GBs": algo / rhs_s / 1e9},
-           "fd_jacobian": {"kernels": "k_fd_cells + k_fd_wall (all colours in one launch)",
+    out = {"residual": {"kernels": "k_props + k_faces + k_flux", "avg_launch_us": rhs_s * 1e6,
+                        "algo_bytes_per_launch": algo, "achieved_GBs": algo / rhs_s / 1e9},
+           "fd_jacobian": {"kernels": "k_props + k_faces + k_fd_props + k_fd_cells + k_fd_wall (all colours)",
                            "avg_launch_us": jac_s * 1e6}}
     for key, sec in (("residual", rhs_s), ("fd_jacobian", jac_s)):
         flop = FP64_FLOP_PER_LAUNCH[key]

@@ -486,22 +486,26 @@ __device__ inline bool face_reads(const vtsa_model_desc& d, int kind, int fi, in
   return ai == fi && (aj == fj || aj == jn);
 }
 
-// Thread (slot, X, v): column v of stencil cell A = slot(X); writes Jc[X][slot][0..5][v].  A
-// wavefront holds one slot (n NV threads per slot), so the faces of X that read A, the only ones
-// evaluated here, are the same for all its lanes; the others come from the face table of the
-// base point (the values the base residual f0 was summed from), and the divergence is summed in
-// the order of the residual.  A's properties and rates come from its k_fd_props record.
+// Thread (group, slot, X, v): column v of stencil cell A = slot(X); writes Jc[X][slot][0..5][v].
+// Cells go in groups of FDG; a group's threads are its 9 slots one after another, FDG NV threads
+// (whole wavefronts) each, so a wavefront holds one slot and the faces of X that read A, the only
+// ones evaluated here, are the same for all its lanes, while the 9 slots of a group run side by
+// side and share its cells' data in cache.  The other faces come from the face table of the base
+// point (the values the base residual f0 was summed from), and the divergence is summed in the
+// order of the residual.  A's properties and rates come from its k_fd_props record.
+constexpr int FDG = 32;
+static_assert((FDG * NV) % 64 == 0, "a slot's share of a cell group is whole wavefronts");
 __global__ void __launch_bounds__(BLK) k_fd_cells(Model m, double t, const double* __restrict__ y,
                                                   const double* __restrict__ pb, const double* __restrict__ FF,
                                                   const double* __restrict__ Pfd, const double* __restrict__ f0,
                                                   double* __restrict__ Jc) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = m.n;
-  if (tid >= n * NSLOT * NV) return;
+  const int grp = tid / (NSLOT * FDG * NV), r = tid - grp * (NSLOT * FDG * NV);
+  const int slot = r / (FDG * NV), q = r - slot * (FDG * NV);
+  const int X = grp * FDG + q / NV, v = q % NV;
+  if (X >= n) return;
   const vtsa_model_desc& d = m.d;
-  const int v = tid % NV;
-  const int X = (tid / NV) % n;
-  const int slot = tid / (NV * n);
   int i, j, k;
   cell_ijk(d, X, i, j, k);
   double* out = Jc + ((size_t)X * NSLOT + slot) * 36;
@@ -510,15 +514,17 @@ __global__ void __launch_bounds__(BLK) k_fd_cells(Model m, double t, const doubl
     for (int r = 0; r < NV; ++r) out[r * NV + v] = 0.0;
     return;
   }
-  double ya[NV];
-  for (int vv = 0; vv < NV; ++vv) ya[vv] = y[vv * n + A];
-  const double h = fd_step(ya[v], d.typ[v]);
-  ya[v] = ya[v] + h;
+  const double yv = y[v * n + A];
+  const double h = fd_step(yv, d.typ[v]);
   const double* rec = Pfd + ((size_t)A * NV + v) * FDREC;
-  Props pa;
+  SwapAccess acc;
+  acc.y = y; acc.pb = pb; acc.n = n; acc.cell_swap = A;
+  Props& pa = acc.ps;
   pa.T = rec[0]; pa.P = rec[1]; pa.Cpos = rec[2]; pa.y[0] = rec[3]; pa.y[1] = rec[4]; pa.y[2] = rec[5];
   pa.mu = rec[6]; pa.kg = rec[7]; pa.rho = rec[8]; pa.cpm = rec[9]; pa.Dm = rec[10]; pa.lam0 = rec[11];
-  SwapAccess acc{y, pb, n, A, &pa, ya};
+  acc.cs0 = v == 0 ? yv + h : y[A];
+  acc.cs1 = v == 1 ? yv + h : y[n + A];
+  acc.cs2 = v == 2 ? yv + h : y[2 * n + A];
   double rate[NA];
   if (A == X) { rate[0] = rec[12]; rate[1] = rec[13]; }
   else { rate[0] = f0[3 * n + X]; rate[1] = f0[4 * n + X]; }
@@ -556,8 +562,12 @@ __global__ void __launch_bounds__(BLK) k_fd_cells(Model m, double t, const doubl
 
 // Thread (W, col): col 0..4 = wall columns (self, z-1, z+1, t-1, t+1), col 5..10 = outer-cell
 // variables.  Writes Jw[W][col], Jwc[W][col-5] and, for col 0, Jcw[W][0..3].
+// The outer cell's rows wrt its wall temperature change no face: their divergence is the face
+// table's of the base point.  The wall row wrt an outer-cell variable takes the perturbed cell's
+// properties from its k_fd_props record.
 __global__ void __launch_bounds__(BLK) k_fd_wall(Model m, double t, const double* __restrict__ y,
-                                                 const double* __restrict__ pb, const double* __restrict__ f0,
+                                                 const double* __restrict__ pb, const double* __restrict__ FF,
+                                                 const double* __restrict__ Pfd, const double* __restrict__ f0,
                                                  double* __restrict__ Jcw, double* __restrict__ Jw,
                                                  double* __restrict__ Jwc) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -590,7 +600,7 @@ __global__ void __launch_bounds__(BLK) k_fd_wall(Model m, double t, const double
       BaseAccess acc{y, pb, n};
       double rate[NA] = {f0[3 * n + X], f0[4 * n + X]};
       double fl[4];
-      cell_flux(m, acc, tw_new, d.Nr - 1, j, k, t, rate, fl);
+      cell_flux_table(m, acc, FF, FaceTable::of(d), tw_new, d.Nr - 1, j, k, t, rate, fl);
       Jcw[W * 4 + 0] = (fl[0] - f0[X]) / h;
       Jcw[W * 4 + 1] = (fl[1] - f0[n + X]) / h;
       Jcw[W * 4 + 2] = (fl[2] - f0[2 * n + X]) / h;
@@ -598,13 +608,11 @@ __global__ void __launch_bounds__(BLK) k_fd_wall(Model m, double t, const double
     }
   } else {
     const int v = col - 5;
-    double ya[NV];
-    for (int vv = 0; vv < NV; ++vv) ya[vv] = y[vv * n + X];
-    const double h = fd_step(ya[v], d.typ[v]);
-    ya[v] = ya[v] + h;
+    const double h = fd_step(y[v * n + X], d.typ[v]);
+    const double* rec = Pfd + ((size_t)X * NV + v) * FDREC;
     Props po;
-    double rate[NA], qs[NA];
-    cell_props(d, ya, ya + 3, ya[5], po, rate, qs);
+    po.T = rec[0]; po.P = rec[1]; po.Cpos = rec[2]; po.y[0] = rec[3]; po.y[1] = rec[4]; po.y[2] = rec[5];
+    po.mu = rec[6]; po.kg = rec[7]; po.rho = rec[8]; po.cpm = rec[9]; po.Dm = rec[10]; po.lam0 = rec[11];
     double fp = wall_row(m, Tw_all, po, j, k, t);
     Jwc[W * 6 + v] = (fp - f0w) / h;
   }
@@ -646,11 +654,12 @@ void launch_fdjac(Handle& h, double t, const double* y, const double* f0, hipStr
   const int nf = FaceTable::of(h.desc).count();
   hipLaunchKernelGGL(k_faces, dim3(nblk(nf, RB)), dim3(RB), 0, s, h.m, t, y, h.pb, h.FF, h.gs, 0);
   hipLaunchKernelGGL(k_fd_props, dim3(nblk((long long)h.n * NV)), dim3(BLK), 0, s, h.m, y, h.Pfd);
-  hipLaunchKernelGGL(k_fd_cells, dim3(nblk((long long)h.n * NSLOT * NV)), dim3(BLK), 0, s, h.m, t, y, h.pb, h.FF,
+  const long long ngrp = (h.n + FDG - 1) / FDG;
+  hipLaunchKernelGGL(k_fd_cells, dim3(nblk(ngrp * NSLOT * FDG * NV)), dim3(BLK), 0, s, h.m, t, y, h.pb, h.FF,
                      h.Pfd, f0, h.Jc);
   if (h.nw > 0)
-    hipLaunchKernelGGL(k_fd_wall, dim3(nblk((long long)h.nw * 11)), dim3(BLK), 0, s, h.m, t, y, h.pb, f0, h.Jcw,
-                       h.Jw, h.Jwc);
+    hipLaunchKernelGGL(k_fd_wall, dim3(nblk((long long)h.nw * 11)), dim3(BLK), 0, s, h.m, t, y, h.pb, h.FF, h.Pfd, f0,
+                       h.Jcw, h.Jw, h.Jwc);
 }
 
 void launch_equilibrium(Handle& h, int count, const double* pc, const double* pw, const double* T,

@@ -461,18 +461,22 @@ struct BaseAccess {
   __device__ inline double T(int cell) const { return pb[cell]; }
 };
 
+// The perturbed cell's values are held by value (no pointer to a thread's local arrays, which
+// would put them in scratch memory).
 struct SwapAccess {
   const double* __restrict__ y;
   const double* __restrict__ pb;
   int n;
   int cell_swap;
-  const Props* ps;
-  const double* cs;   // perturbed c of cell_swap
+  Props ps;                 // properties of cell_swap
+  double cs0, cs1, cs2;     // perturbed c of cell_swap
   __device__ inline void props(int cell, Props& p) const {
-    if (cell == cell_swap) p = *ps; else load_props(pb, n, cell, p);
+    if (cell == cell_swap) p = ps; else load_props(pb, n, cell, p);
   }
-  __device__ inline double c(int s, int cell) const { return cell == cell_swap ? cs[s] : y[s * n + cell]; }
-  __device__ inline double T(int cell) const { return cell == cell_swap ? ps->T : pb[cell]; }
+  __device__ inline double c(int s, int cell) const {
+    return cell == cell_swap ? (s == 0 ? cs0 : (s == 1 ? cs1 : cs2)) : y[s * n + cell];
+  }
+  __device__ inline double T(int cell) const { return cell == cell_swap ? ps.T : pb[cell]; }
 };
 
 // MUSCL face value on the z face k+1/2 from phi at k-1..k+2 (numerics/stencils.py:79-95)
