@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(RBLK) k_gm_init(int N, const double* __restric
 // Classical Gram-Schmidt with one reorthogonalisation (CGS2) in three passes per Arnoldi step, two
 // of them grid reductions: k_gm_dots (h = V^T w, |w|^2), k_gm_orth (w' = w - V h, h2 = V^T w',
 // |w'|^2, and in its last block the end of the column), k_gm_project (w'' = w' - V h2, no
-// reduction).  The norm of w'' comes from |w''|^2 = |w'|^2 - |h2|^2 (V orthonormal), so no pass
+// reduction; with the low-rank correction the next column's k_defl_axpy forms w'' as it reads it).  The norm of w'' comes from |w''|^2 = |w'|^2 - |h2|^2 (V orthonormal), so no pass
 // reduces over w''.  Every pass reads the K = col + 1 basis vectors once per element; per-vector
 // block sums go through wavefront butterflies (no per-thread arrays indexed by K), and the last
 // block sums the per-block partials with one wavefront per row.
@@ -455,8 +455,9 @@ __device__ inline void lartg(double f, double g, double& c, double& s, double& r
 
 // Pass 2: w' = w - V h_tmp (stored over w); h2 = V . w' and |w'|^2; DEFL: Ud . w' folded in.
 // Its last block then ends the column (scipy gmres inner loop): H[:, col] = h + h2 (kept raw in
-// Hr), |w''| = sqrt(|w'|^2 - sum_k h2_k^2) with w'' = w' - V h2 (k_gm_project forms it), the
-// Givens sweep of the column, the convergence test and the host-visible progress; DEFL: the next
+// Hr), |w''| = sqrt(|w'|^2 - sum_k h2_k^2) with w'' = w' - V h2 (formed by k_gm_project or by the
+// next column's k_defl_axpy), the Givens sweep of the column, the convergence test and the
+// host-visible progress; DEFL: the next
 // vector's dots Ud . w'' = Ud . w' - sum_k (Ud . v_k) h2_k, scaled by 1 / |w''|, into uv[., col + 1]
 // and d = Dd c.  Everything the tail reads from gs is loaded at kernel start with the
 // coefficients, so that the serial tail waits for no load.
@@ -854,13 +855,25 @@ __global__ void __launch_bounds__(RBLK) k_defl_dots(int N, const double* in, con
 
 // out = v + Ud d with v = in * scale, v also stored to `store` when given (out and store may alias
 // in).
-__global__ void k_defl_axpy(int N, int k, const double* in, const double* __restrict__ in_scale, double* store,
-                            const double* __restrict__ Ud, const double* __restrict__ d, double* out,
-                            const GmresScalars* __restrict__ gs, int respect_done) {
+// K > 0: `in` is the previous Arnoldi column's w' and is first reorthogonalised, w'' = w' -
+// sum_k h2_k V_k (k < K, h2 = gs->h_tmp), with the arithmetic of k_gm_project, which the GMRES
+// cycle then skips for that column.
+__global__ void __launch_bounds__(256) k_defl_axpy(int N, int k, const double* in, const double* __restrict__ in_scale,
+                                                   double* store, const double* __restrict__ Ud,
+                                                   const double* __restrict__ d, double* out,
+                                                   const GmresScalars* __restrict__ gs, int respect_done,
+                                                   const double* __restrict__ Vb, size_t ld, int K) {
+  __shared__ double hk[GMW];
   if (respect_done && gs->done) return;
+  if (K > 0) {
+    if ((int)threadIdx.x < K) hk[threadIdx.x] = gs->h_tmp[threadIdx.x];
+    __syncthreads();
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double v = in[i] * (in_scale ? *in_scale : 1.0);
+  double x = in[i];
+  if (K > 0) x = project_out(Vb, ld, hk, K, i, x);
+  double v = x * (in_scale ? *in_scale : 1.0);
   if (store) store[i] = v;
   for (int j = 0; j < k; ++j) v = fma(Ud[(size_t)j * N + i], d[j], v);
   out[i] = v;
@@ -935,7 +948,7 @@ void defl_apply_launch(Handle& h, const double* in, const double* in_scale, doub
   const double* v = store ? store : in;
   const double* vs = store ? nullptr : in_scale;
   hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(h.N)), dim3(256), 0, st, h.N, h.defl_k, v, vs, (double*)nullptr, h.Ud, d,
-                     out, h.gs, respect_done);
+                     out, h.gs, respect_done, (const double*)nullptr, (size_t)0, 0);
 }
 
 int gmres_cycle(Handle& h, double t, const double* z, const double* f, const double* s, double dh, double zs_norm,
@@ -990,11 +1003,12 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
       if (v < 0) break;
     }
     // u = P^-1 v_col with v_col = w_prev / |w_prev| normalised in place (u = P^-1 Q v_col with the
-    // low-rank correction)
+    // low-rank correction, whose axpy also forms w_prev = w' - V h2 of the previous column)
     const double* vin = col == 0 ? b : Vb + (size_t)col * ld;
     if (dk > 0) {
       hipLaunchKernelGGL(k_defl_axpy, dim3(eblk(N)), dim3(256), 0, st, N, dk, vin, &h.gs->inv_h1,
-                         Vb + (size_t)col * ld, (const double*)h.Ud, (const double*)dvec, h.defl_tmp, h.gs, 1);
+                         Vb + (size_t)col * ld, (const double*)h.Ud, (const double*)dvec, h.defl_tmp, h.gs, 1,
+                         (const double*)Vb, ld, col);
       ApplyArgs a{h.defl_tmp, nullptr, nullptr, s, s, U, nullptr, nullptr, h.gs, &h.gs->zs_rms, 1};
       apply_launch(h, a, st);
     } else {
@@ -1013,7 +1027,9 @@ int gmres_cycle(Handle& h, double t, const double* z, const double* f, const dou
       hipLaunchKernelGGL(k_gm_orth<false>, dim3(g), dim3(RBLK), 0, st, N, col, Vb, ld, wcol, h.gs, h.red.partial,
                          h.red.counter + 5, h.progress_dev, (const double*)nullptr, 0, (const double*)nullptr,
                          (double*)nullptr, (double*)nullptr);
-    hipLaunchKernelGGL(k_gm_project, dim3(eblk(N)), dim3(256), 0, st, N, col, Vb, ld, wcol, h.gs);
+    // w'' = w' - V h2; with the correction the next column's axpy forms it instead
+    if (dk == 0 || col + 1 >= restart)
+      hipLaunchKernelGGL(k_gm_project, dim3(eblk(N)), dim3(256), 0, st, N, col, Vb, ld, wcol, h.gs);
   }
   hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, h.gs);
   hipLaunchKernelGGL(k_gm_update, dim3(eblk(N)), dim3(256), 0, st, N, Vb, ld, h.gs, w);
