@@ -498,12 +498,19 @@ __global__ void __launch_bounds__(RBLK) k_gm_orth(int N, int col, const double* 
 #pragma unroll
   for (int j = 0; j < DEFL_MAX; ++j) cl[j] = 0.0;
   for (int i = i0; i < N; i += stride) {
+    // DEFL: the element's correction vectors are requested before its projection, whose loads
+    // they then share a round trip with
+    double ud[DEFL ? DEFL_MAX : 1];
+    if constexpr (DEFL) {
+#pragma unroll
+      for (int j = 0; j < DEFL_MAX; ++j) ud[j] = j < dk ? Ud[(size_t)j * N + i] : 0.0;
+    }
     const double wi = project_out(Vb, ld, hk, K, i, w[i]);
     w[i] = wi;
     if constexpr (DEFL) {
 #pragma unroll
       for (int j = 0; j < DEFL_MAX; ++j)
-        if (j < dk) cl[j] += Ud[(size_t)j * N + i] * wi;
+        if (j < dk) cl[j] += ud[j] * wi;
     }
   }
   // rows 0 .. K-1: V_k . w'; row K: |w'|^2 (w' is stored as column K); DEFL rows K+1 ..: Ud . w'
