@@ -16,7 +16,7 @@ Pinning (what makes this restatement trustworthy):
 * known answers held by the upstream tests (isotherm hand values, p_sat(373.15));
 * invariants (species/energy conservation of rhs + accumulators, T<->e round trip,
   axisymmetry, Ergun inversion round trip, softplus split identity);
-* golden vectors written from the upstream code in the survey container
+* golden vectors written by running the upstream code where it is installed
   (``tests/golden/make_golden.py``): rhs/acc/diag on seeded states for every step type,
   FD Jacobian values, and a short SDIRK2 and TR-BDF2 integration;
 * brute force on tiny inputs (FD Jacobian vs one-column FD, integrator vs RK4).
